@@ -1,0 +1,270 @@
+/*
+ * pourlab3d.h — C ABI of libpourlab3d.so, the MI355X (gfx950) implementation of PourLab's 3D
+ * PIC/FLIP free-surface pour (reference: src/sim3d of marcelpetrick/teapotFluidSimulation).
+ *
+ * The records and entry points mirror the reference's solver class API so that a host written like
+ * benchmarks/run3d.ts (CPU class Simulation3D) or src/gpu3d/verify.ts (GPU class GpuSimulation3D)
+ * can drive this library through a thin FFI layer (ctypes, N-API):
+ *
+ *   reference                                            this ABI
+ *   new Simulation3D(cfg)  solver3d.ts:116                pl3d_create
+ *   sim.step(dt)           solver3d.ts:288                pl3d_step
+ *   sim.advance(seconds)   solver3d.ts:281                pl3d_advance
+ *   sim.stableDt()         solver3d.ts:268                pl3d_stable_dt
+ *   sim.finished           solver3d.ts:264                pl3d_finished
+ *   sim.diagnostics()      solver3d.ts:796                pl3d_get_diagnostics
+ *   sim.dischargedFractions() solver3d.ts:817             pl3d_discharged_fractions
+ *   public fields x,y,z,vx,vy,vz,origin,status,y0,
+ *     bandCounts, dischargeOrder  solver3d.ts:62-116       pl3d_get_info, pl3d_read_state,
+ *                                                         pl3d_discharge_order
+ *   GpuSimulation3D.stats / decodeStats gpu3d/solver.ts:24-47, STAT gpu3d/layout.ts:20-37
+ *                                                         pl3d_stats
+ *   GpuSimulation3D.readParticles  gpu3d/solver.ts:420    pl3d_read_particles
+ *   GpuSimulation3D.readRenderPositions gpu3d/solver.ts:447 pl3d_read_render_positions
+ *   GpuSimulation3D.destroy        gpu3d/solver.ts:486    pl3d_destroy
+ *   PourController.setTarget       sim/motion.ts:40       pl3d_set_target
+ *   new Pcg3D(nx,ny,nz).solve(cell,b,q,tol,maxIter) solver3d.ts:827-937
+ *                                                         pl3d_pcg_solve
+ *
+ * Conventions (kept from the reference, solver3d.ts:540, :761-764; gpu3d/solver.ts:69):
+ *   - every entry point returns PL3D_OK (0) or a negative PL3D_E* code; setup failures are errors,
+ *     the message is available from pl3d_last_error() (thread-local);
+ *   - pressure non-convergence is counted in the diagnostics, not returned as an error;
+ *   - a parcel whose position turns non-finite becomes PL3D_LOST;
+ *   - a handle owns its device memory and one HIP stream; calls on one handle must not overlap;
+ *     separate handles are independent.  All calls block until their device work is done.
+ *   - host arrays are caller-allocated; the library copies into them.
+ */
+#ifndef POURLAB3D_H
+#define POURLAB3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define PL3D_OK 0
+#define PL3D_EINVAL (-1)   /* bad argument / config */
+#define PL3D_EDEVICE (-2)  /* HIP runtime error */
+#define PL3D_ECAPACITY (-3) /* problem exceeds a compiled capacity */
+#define PL3D_ESTATE (-4)   /* call not valid in the current state (e.g. after destroy) */
+
+/* cell flags (solver3d.ts:17-19) */
+#define PL3D_AIR 0
+#define PL3D_FLUID 1
+#define PL3D_SOLID 2
+
+/* parcel status (solver3d.ts:20-23) */
+#define PL3D_IN_POT 0
+#define PL3D_DISCHARGED 1
+#define PL3D_SPILLED 2
+#define PL3D_LOST 3
+
+/* MotionConfig.kind (sim/types.ts:57) */
+#define PL3D_MOTION_STILL 0
+#define PL3D_MOTION_POUR 1
+#define PL3D_MOTION_TILT 2
+#define PL3D_MOTION_MANUAL 3
+
+/* MotionPhase (sim/motion.ts:10); pl3d_phase_name() gives the reference's strings */
+#define PL3D_PHASE_WAIT 0
+#define PL3D_PHASE_TILT_IN 1
+#define PL3D_PHASE_HOLD 2
+#define PL3D_PHASE_RETURN 3
+#define PL3D_PHASE_REST 4
+#define PL3D_PHASE_DONE 5
+#define PL3D_PHASE_MANUAL 6
+
+/* pressure preconditioners of the device solver (no counterpart in the reference's API; the
+ * reference CPU solver is MIC(0)-PCG, solver3d.ts:826) */
+#define PL3D_PRECOND_MIC0 0
+#define PL3D_PRECOND_MG 1
+#define PL3D_PRECOND_JACOBI 2
+
+/* particle-to-grid transfer (reference: FLIP/PIC blend only, solver3d.ts:607-624) */
+#define PL3D_TRANSFER_FLIP 0
+#define PL3D_TRANSFER_APIC 1
+
+/* Teapot3DConfig (sim3d/vessel3d.ts:9-20); lengths in metres */
+typedef struct pl3d_teapot_config {
+  double radius;
+  double height;
+  double wall;
+  double inlet_height;
+  double bore_radius;
+  double spout_angle_deg;
+  double spout_length;
+} pl3d_teapot_config;
+
+/* MotionConfig (sim/types.ts:55-84) */
+typedef struct pl3d_motion_config {
+  int32_t kind; /* PL3D_MOTION_* */
+  int32_t repeats;
+  double start_delay;
+  double tilt_duration;
+  double hold_angle_deg;
+  double hold_tilt_rate;
+  double max_angle_deg;
+  double return_duration;
+  double max_hold;
+  double hold_time;
+  double target_volume; /* m^3 */
+  double anticipation;
+  double repeat_gap;
+  double settle_time;
+} pl3d_motion_config;
+
+/* Config3D (sim3d/solver3d.ts:25-43) plus two device-only choices at the end */
+typedef struct pl3d_config {
+  pl3d_teapot_config teapot;
+  pl3d_motion_config motion;
+  double h;              /* cell size (m) */
+  double initial_volume; /* m^3, whole pot */
+  double pivot[2];       /* tilt pivot, pot-local x, y (m) */
+  double gravity;
+  double flip_ratio;
+  double density_correction;
+  double cfl;
+  double max_substep;
+  double pressure_tolerance;
+  int32_t pressure_max_iterations;
+  uint32_t seed;
+  double duration;
+  int32_t preconditioner; /* PL3D_PRECOND_* (device solver only) */
+  int32_t transfer;       /* PL3D_TRANSFER_* */
+} pl3d_config;
+
+/* Diagnostics3D (sim3d/solver3d.ts:45-60) */
+typedef struct pl3d_diagnostics {
+  double time;
+  int64_t steps;
+  double angle_deg;
+  int32_t phase; /* PL3D_PHASE_* */
+  int32_t particles;
+  int32_t grid[3];
+  int32_t pressure_iterations;
+  double discharged_ml;
+  double spilled_ml;
+  double lost_ml;
+  double in_pot_ml;
+  int32_t pressure_failures;
+  int32_t correction_iterations; /* STAT.correctionIterations (gpu3d/layout.ts:34) */
+  double max_divergence;
+  double interior_density;
+} pl3d_diagnostics;
+
+/* Fixed per-simulation facts (public readonly fields of Simulation3D, solver3d.ts:63-85) */
+typedef struct pl3d_info {
+  int32_t n; /* parcels in the simulated half */
+  int32_t nx, ny, nz;
+  double h;
+  double fill_height;
+  double pivot_world[2];
+  double parcel_volume; /* m^3 per parcel */
+  int32_t band_counts[3];
+  int32_t max_liquid_cells; /* capacity of the device pressure solver */
+} pl3d_info;
+
+/* Pcg3D.solve result (solver3d.ts:882, :925, :936) plus the FP64 true residual max|b - A q| */
+typedef struct pl3d_solve_result {
+  int32_t iterations;
+  int32_t converged;
+  double residual;      /* the solver's own (recursive) residual, max norm, as the reference */
+  double true_residual; /* max |b - A q| recomputed in FP64 after the solve */
+  int32_t liquid_cells;
+  int32_t pad;
+} pl3d_solve_result;
+
+typedef struct pl3d_sim pl3d_sim;
+
+/* STAT slot meanings of gpu3d/layout.ts:20-37 (pl3d_stats fills int32[16]). */
+#define PL3D_STAT_IN_POT 0
+#define PL3D_STAT_DISCHARGED 1
+#define PL3D_STAT_SPILLED 2
+#define PL3D_STAT_LOST 3
+#define PL3D_STAT_ORIGIN_BOTTOM 4
+#define PL3D_STAT_ORIGIN_MIDDLE 5
+#define PL3D_STAT_ORIGIN_TOP 6
+#define PL3D_STAT_PRESSURE_FAILURES 7
+#define PL3D_STAT_MAX_SPEED_SQ 8  /* float bits */
+#define PL3D_STAT_MAX_DIVERGENCE 9 /* float bits */
+#define PL3D_STAT_INTERIOR_COUNT 10
+#define PL3D_STAT_INTERIOR_SUM 11 /* interior density sum in units of 2^-16 (SCALE_RATIO) */
+#define PL3D_STAT_PRESSURE_ITERATIONS 12
+#define PL3D_STAT_CORRECTION_ITERATIONS 13
+#define PL3D_STAT_SLOTS 16
+
+/* ---- library ------------------------------------------------------------------------------ */
+
+/* Thread-local text of the last error (empty string when none). */
+const char *pl3d_last_error(void);
+/* Reference phase string ('wait', 'tilt-in', ...) for a PL3D_PHASE_* value. */
+const char *pl3d_phase_name(int32_t phase);
+/* Library version string. */
+const char *pl3d_version(void);
+
+/* gentlePour3D(cellMm, seconds) (sim3d/presets3d.ts:9-26 with sim/config.ts:168-186). */
+int pl3d_gentle_pour(double cell_mm, double seconds, pl3d_config *out);
+
+/* ---- simulation handle (Simulation3D / GpuSimulation3D) ------------------------------------ */
+
+/* new Simulation3D(cfg): sizes the grid, seeds the parcels (mulberry32, solver3d.ts:185-226) and
+ * uploads everything to the current HIP device. */
+int pl3d_create(const pl3d_config *cfg, pl3d_sim **out);
+/* GpuSimulation3D.destroy(): frees the device memory; the handle is invalid afterwards. */
+int pl3d_destroy(pl3d_sim *sim);
+int pl3d_get_info(pl3d_sim *sim, pl3d_info *out);
+
+/* step(dt): one substep of length dt (solver3d.ts:288-309). */
+int pl3d_step(pl3d_sim *sim, double dt);
+/* advance(seconds): whole stable substeps until time >= start + seconds or finished
+ * (solver3d.ts:281-286); fills *out (may be NULL) with diagnostics(). */
+int pl3d_advance(pl3d_sim *sim, double seconds, pl3d_diagnostics *out);
+int pl3d_stable_dt(pl3d_sim *sim, double *out);
+int pl3d_finished(pl3d_sim *sim, int32_t *out);
+int pl3d_get_diagnostics(pl3d_sim *sim, pl3d_diagnostics *out);
+/* dischargedFractions(): *has = 0 before anything was poured (the reference returns null). */
+int pl3d_discharged_fractions(pl3d_sim *sim, double out[3], int32_t *has);
+/* The 16-slot STAT block of the last completed substep. */
+int pl3d_stats(pl3d_sim *sim, int32_t out[PL3D_STAT_SLOTS]);
+
+/* Public parcel fields of Simulation3D (any pointer may be NULL to skip that field). */
+int pl3d_read_state(pl3d_sim *sim, double *x, double *y, double *z, double *vx, double *vy,
+                    double *vz, uint8_t *status, uint8_t *origin, float *y0);
+/* dischargeOrder: parcel indices in discharge order (by substep, then index). *count receives the
+ * full length; at most cap entries are written. */
+int pl3d_discharge_order(pl3d_sim *sim, int32_t *out, int32_t cap, int32_t *count);
+/* readParticles(): pos/vel as float4 (w unused), status as uint32 (gpu3d/solver.ts:420-437). */
+int pl3d_read_particles(pl3d_sim *sim, float *pos4, float *vel4, uint32_t *status);
+/* readRenderPositions(): xyz of the simulated half, then the mirror image z -> 2h - z
+ * (gpu3d/shaders.ts:779-801); out holds 6n floats. */
+int pl3d_read_render_positions(pl3d_sim *sim, float *out);
+/* Manual tilt target, degrees (PourController.setTarget, sim/motion.ts:40-44). */
+int pl3d_set_target(pl3d_sim *sim, double angle_deg);
+
+/* ---- standalone pressure solve (Pcg3D) ------------------------------------------------------ */
+
+/* Pcg3D(nx, ny, nz).solve(cell, b, q, tol, maxIter) (solver3d.ts:858-937): solves the 7-point
+ * Poisson system on the FLUID cells of `cell` (AIR = p 0 ghost, SOLID = Neumann), host arrays of
+ * nx*ny*nz entries, stop rule max|r| <= max(tol * max|b|, 1e-14).  The solve runs on the device in
+ * FP32 with FP64 dot products and FP64 true-residual refinement; q receives the solution (0 off the
+ * liquid).  precond is a PL3D_PRECOND_* value. */
+int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *b,
+                   double *q, double tol, int32_t max_iter, int32_t precond,
+                   pl3d_solve_result *out);
+
+/* ---- measurement ---------------------------------------------------------------------------- */
+
+/* Per-kernel device time accumulated by HIP events around each launch since the last reset, when
+ * timing is enabled (pl3d_set_timing).  names/ms/launches receive up to cap entries. */
+int pl3d_set_timing(pl3d_sim *sim, int32_t enabled);
+int pl3d_kernel_times(pl3d_sim *sim, int32_t cap, const char **names, double *total_ms,
+                      int64_t *launches, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POURLAB3D_H */

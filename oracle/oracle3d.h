@@ -1,0 +1,88 @@
+/*
+ * oracle3d.h — C API of the FP64 CPU oracle (liboracle3d.so).
+ *
+ * TEST INFRASTRUCTURE ONLY.  The oracle is a plain C++ restatement of the reference's 3D CPU pour
+ * (src/sim3d/solver3d.ts, src/sim3d/vessel3d.ts, src/sim3d/presets3d.ts, src/sim/motion.ts,
+ * src/sim/rng.ts).  It is linked only by tests/, by __graft_entry__.smoke() and by bench.py's
+ * CPU-baseline leg, always as the checker; the product library never calls it.
+ *
+ * It reuses the record types of include/pourlab3d.h so that configs and diagnostics can be compared
+ * field by field with the device library.
+ */
+#ifndef ORACLE3D_H
+#define ORACLE3D_H
+
+#include <stdint.h>
+
+#include "../include/pourlab3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or3d_sim or3d_sim;
+typedef struct or3d_controller or3d_controller;
+
+/* presets3d.ts:9-26 */
+void or3d_gentle_pour(double cell_mm, double seconds, pl3d_config *out);
+/* vessel3d.ts:22-32 */
+void or3d_default_teapot(pl3d_teapot_config *out);
+/* config.ts:168-186 */
+void or3d_default_motion(pl3d_motion_config *out);
+
+/* rng.ts:5-14: count successive draws of mulberry32(seed) */
+void or3d_rng(uint32_t seed, int32_t count, double *out);
+
+/* vessel3d.ts:46-103 */
+double or3d_teapot_sdf(const pl3d_teapot_config *cfg, double x, double y, double z);
+int32_t or3d_in_cavity(const pl3d_teapot_config *cfg, double x, double y, double z);
+/* buildTeapot3D: out = {ax, ay, dx, dy, tipX, tipY} */
+void or3d_build_teapot(const pl3d_teapot_config *cfg, double out[6]);
+
+/* motion.ts PourController */
+or3d_controller *or3d_controller_create(const pl3d_motion_config *cfg);
+void or3d_controller_destroy(or3d_controller *c);
+double or3d_controller_advance(or3d_controller *c, double t, double dt, double discharged);
+int32_t or3d_controller_phase(or3d_controller *c);
+double or3d_controller_angle_deg(or3d_controller *c);
+void or3d_controller_set_target(or3d_controller *c, double angle_deg, double time);
+
+/* Simulation3D */
+or3d_sim *or3d_create(const pl3d_config *cfg);
+void or3d_destroy(or3d_sim *s);
+void or3d_info(or3d_sim *s, pl3d_info *out);
+void or3d_step(or3d_sim *s, double dt);
+void or3d_advance(or3d_sim *s, double seconds, pl3d_diagnostics *out);
+double or3d_stable_dt(or3d_sim *s);
+int32_t or3d_finished(or3d_sim *s);
+void or3d_diagnostics(or3d_sim *s, pl3d_diagnostics *out);
+/* returns 0 when nothing was discharged (reference: null) */
+int32_t or3d_discharged_fractions(or3d_sim *s, double out[3]);
+void or3d_read_state(or3d_sim *s, double *x, double *y, double *z, double *vx, double *vy,
+                     double *vz, uint8_t *status, uint8_t *origin, float *y0);
+/* overwrite parcel state (lockstep comparisons start both solvers from one state) */
+void or3d_write_state(or3d_sim *s, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz,
+                      const uint8_t *status);
+int32_t or3d_discharge_order(or3d_sim *s, int32_t *out, int32_t cap);
+double or3d_phi(or3d_sim *s);
+/* Grid fields of the last step, by name: "cell" (flags as double), "u","v","w" (final, after
+ * extrapolation), "u_p2g","v_p2g","w_p2g" (uPrev), "u_proj","v_proj","w_proj" (after the pressure
+ * gradient, before extrapolation), "u_valid","v_valid","w_valid", "rhs_p" (pressure rhs), "q",
+ * "density", "rhs_d" (density rhs), "qc", "uc","vc","wc".  Returns the element count, or -1. */
+int64_t or3d_grid_field(or3d_sim *s, const char *name, double *out);
+/* Last step's solve results: [pressure iterations, converged, correction iterations, converged] */
+void or3d_last_solves(or3d_sim *s, int32_t out[4]);
+
+/* Pcg3D(nx, ny, nz).solve(cell, b, q, tol, maxIter): solver3d.ts:858-937 */
+void or3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *b,
+                    double *q, double tol, int32_t max_iter, pl3d_solve_result *out);
+/* y = A x on the FLUID cells (7-point operator of solver3d.ts:872-881, :939-953); 0 elsewhere */
+void or3d_apply_laplacian(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *x,
+                          double *y);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORACLE3D_H */

@@ -1,0 +1,253 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle3d.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, always as the checker.  The product package never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from teapotfluidsimulation_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle3d.so")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle3d.so with the committed Makefile (g++)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=HERE, check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        d, i32, u32, vp = C.c_double, C.c_int32, C.c_uint32, C.c_void_p
+        sigs = {
+            "or3d_gentle_pour": (None, [d, d, P(abi.Config)]),
+            "or3d_default_teapot": (None, [P(abi.TeapotConfig)]),
+            "or3d_default_motion": (None, [P(abi.MotionConfig)]),
+            "or3d_rng": (None, [u32, i32, P(d)]),
+            "or3d_teapot_sdf": (d, [P(abi.TeapotConfig), d, d, d]),
+            "or3d_in_cavity": (i32, [P(abi.TeapotConfig), d, d, d]),
+            "or3d_build_teapot": (None, [P(abi.TeapotConfig), P(d)]),
+            "or3d_controller_create": (vp, [P(abi.MotionConfig)]),
+            "or3d_controller_destroy": (None, [vp]),
+            "or3d_controller_advance": (d, [vp, d, d, d]),
+            "or3d_controller_phase": (i32, [vp]),
+            "or3d_controller_angle_deg": (d, [vp]),
+            "or3d_controller_set_target": (None, [vp, d, d]),
+            "or3d_create": (vp, [P(abi.Config)]),
+            "or3d_destroy": (None, [vp]),
+            "or3d_info": (None, [vp, P(abi.Info)]),
+            "or3d_step": (None, [vp, d]),
+            "or3d_advance": (None, [vp, d, P(abi.Diagnostics)]),
+            "or3d_stable_dt": (d, [vp]),
+            "or3d_finished": (i32, [vp]),
+            "or3d_diagnostics": (None, [vp, P(abi.Diagnostics)]),
+            "or3d_discharged_fractions": (i32, [vp, P(d)]),
+            "or3d_read_state": (None, [vp] + [vp] * 9),
+            "or3d_write_state": (None, [vp] + [vp] * 7),
+            "or3d_discharge_order": (i32, [vp, P(i32), i32]),
+            "or3d_phi": (d, [vp]),
+            "or3d_grid_field": (C.c_int64, [vp, C.c_char_p, vp]),
+            "or3d_last_solves": (None, [vp, P(i32)]),
+            "or3d_pcg_solve": (None, [i32, i32, i32, vp, vp, vp, d, i32, P(abi.SolveResult)]),
+            "or3d_apply_laplacian": (None, [i32, i32, i32, vp, vp, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def gentle_pour(cell_mm: float, seconds: float) -> abi.Config:
+    c = abi.Config()
+    lib().or3d_gentle_pour(cell_mm, seconds, C.byref(c))
+    return c
+
+
+def default_teapot() -> abi.TeapotConfig:
+    t = abi.TeapotConfig()
+    lib().or3d_default_teapot(C.byref(t))
+    return t
+
+
+def default_motion() -> abi.MotionConfig:
+    m = abi.MotionConfig()
+    lib().or3d_default_motion(C.byref(m))
+    return m
+
+
+def rng(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count, np.float64)
+    lib().or3d_rng(seed, count, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
+
+
+def teapot_sdf(t: abi.TeapotConfig, x, y, z) -> float:
+    return lib().or3d_teapot_sdf(C.byref(t), x, y, z)
+
+
+def in_cavity(t: abi.TeapotConfig, x, y, z) -> bool:
+    return bool(lib().or3d_in_cavity(C.byref(t), x, y, z))
+
+
+def build_teapot(t: abi.TeapotConfig) -> dict:
+    out = (C.c_double * 6)()
+    lib().or3d_build_teapot(C.byref(t), out)
+    return dict(zip(["ax", "ay", "dx", "dy", "tipX", "tipY"], list(out)))
+
+
+class Controller:
+    """PourController (sim/motion.ts:21-159) from the oracle."""
+
+    def __init__(self, motion: abi.MotionConfig):
+        self._h = lib().or3d_controller_create(C.byref(motion))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or3d_controller_destroy(self._h)
+            self._h = None
+
+    def advance(self, t, dt, discharged) -> float:
+        return lib().or3d_controller_advance(self._h, t, dt, discharged)
+
+    @property
+    def phase(self) -> str:
+        return abi.PHASES[lib().or3d_controller_phase(self._h)]
+
+    @property
+    def angleDeg(self) -> float:
+        return lib().or3d_controller_angle_deg(self._h)
+
+    def setTarget(self, angle_deg, time):
+        lib().or3d_controller_set_target(self._h, angle_deg, time)
+
+
+class OracleSim:
+    """Simulation3D (solver3d.ts:62-824) in FP64 on the CPU."""
+
+    def __init__(self, cfg: abi.Config):
+        self.cfg = cfg
+        self._h = lib().or3d_create(C.byref(cfg))
+        info = abi.Info()
+        lib().or3d_info(self._h, C.byref(info))
+        self.info = info
+        self.n, self.nx, self.ny, self.nz = info.n, info.nx, info.ny, info.nz
+        self.h = info.h
+        self.pivotWorld = [info.pivot_world[0], info.pivot_world[1]]
+        self.parcelVolume = info.parcel_volume
+        self.bandCounts = list(info.band_counts)
+        self.fillHeight = info.fill_height
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().or3d_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def step(self, dt):
+        lib().or3d_step(self._h, dt)
+
+    def advance(self, seconds) -> dict:
+        d = abi.Diagnostics()
+        lib().or3d_advance(self._h, seconds, C.byref(d))
+        return abi.diagnostics_dict(d)
+
+    def stableDt(self) -> float:
+        return lib().or3d_stable_dt(self._h)
+
+    @property
+    def finished(self) -> bool:
+        return bool(lib().or3d_finished(self._h))
+
+    def diagnostics(self) -> dict:
+        d = abi.Diagnostics()
+        lib().or3d_diagnostics(self._h, C.byref(d))
+        return abi.diagnostics_dict(d)
+
+    def raw_diagnostics(self) -> abi.Diagnostics:
+        d = abi.Diagnostics()
+        lib().or3d_diagnostics(self._h, C.byref(d))
+        return d
+
+    def dischargedFractions(self):
+        out = (C.c_double * 3)()
+        if not lib().or3d_discharged_fractions(self._h, out):
+            return None
+        return list(out)
+
+    def state(self) -> dict:
+        n = self.n
+        f = {k: np.zeros(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        st = np.zeros(n, np.uint8)
+        org = np.zeros(n, np.uint8)
+        y0 = np.zeros(n, np.float32)
+        lib().or3d_read_state(self._h, *[_ptr(f[k]) for k in ("x", "y", "z", "vx", "vy", "vz")],
+                              _ptr(st), _ptr(org), _ptr(y0))
+        f.update(status=st, origin=org, y0=y0)
+        return f
+
+    def write_state(self, x=None, y=None, z=None, vx=None, vy=None, vz=None, status=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64)
+                for a in (x, y, z, vx, vy, vz)]
+        st = None if status is None else np.ascontiguousarray(status, np.uint8)
+        lib().or3d_write_state(self._h, *[_ptr(a) for a in arrs], _ptr(st))
+
+    def dischargeOrder(self) -> np.ndarray:
+        cnt = lib().or3d_discharge_order(self._h, None, 0)
+        out = np.zeros(max(cnt, 1), np.int32)
+        lib().or3d_discharge_order(self._h, out.ctypes.data_as(C.POINTER(C.c_int32)), cnt)
+        return out[:cnt]
+
+    @property
+    def phi(self) -> float:
+        return lib().or3d_phi(self._h)
+
+    def grid_field(self, name: str) -> np.ndarray:
+        cnt = lib().or3d_grid_field(self._h, name.encode(), None)
+        if cnt < 0:
+            raise KeyError(name)
+        out = np.zeros(cnt, np.float64)
+        lib().or3d_grid_field(self._h, name.encode(), _ptr(out))
+        return out
+
+    def last_solves(self):
+        out = (C.c_int32 * 4)()
+        lib().or3d_last_solves(self._h, out)
+        return list(out)
+
+
+def pcg_solve(nx, ny, nz, cell, b, tol, max_iter):
+    """Pcg3D.solve (solver3d.ts:858-937); returns (q, result)."""
+    cell = np.ascontiguousarray(cell, np.uint8)
+    b = np.ascontiguousarray(b, np.float64)
+    q = np.zeros(nx * ny * nz, np.float64)
+    res = abi.SolveResult()
+    lib().or3d_pcg_solve(nx, ny, nz, _ptr(cell), _ptr(b), _ptr(q), tol, max_iter, C.byref(res))
+    return q, res
+
+
+def apply_laplacian(nx, ny, nz, cell, x):
+    cell = np.ascontiguousarray(cell, np.uint8)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(nx * ny * nz, np.float64)
+    lib().or3d_apply_laplacian(nx, ny, nz, _ptr(cell), _ptr(x), _ptr(y))
+    return y

@@ -1,0 +1,1257 @@
+// sim3d_oracle.cpp — FP64 CPU oracle of the reference 3D pour.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the device library and the CPU-baseline leg of
+// bench.py.  Nothing in the product links it.
+//
+// A plain restatement, function by function, of the reference's FP64 TypeScript:
+//   src/sim/rng.ts            mulberry32                         -> Rng
+//   src/sim3d/vessel3d.ts     Teapot3D, teapotSdf, inCavity       -> Teapot, teapot_sdf, in_cavity
+//   src/sim/motion.ts         PourController                      -> Controller
+//   src/sim/config.ts:168-186 defaultMotion; presets3d.ts:9-26 gentlePour3D
+//   src/sim3d/solver3d.ts     Simulation3D, Pcg3D                 -> Sim, Pcg
+// Arithmetic follows the TypeScript expression by expression (left-to-right evaluation, no FMA
+// contraction: build with -ffp-contract=off), so results track the reference closely; Math.hypot
+// is restated with the compensated scaled sum V8 uses (js_hypot).
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "oracle3d.h"
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+constexpr double DEG = PI / 180.0;  // motion.ts:12
+constexpr uint8_t AIR = 0, FLUID = 1, SOLID = 2;                       // solver3d.ts:17-19
+constexpr uint8_t IN_POT = 0, DISCHARGED = 1, SPILLED = 2, LOST = 3;  // solver3d.ts:20-23
+
+// Math.hypot as V8 evaluates it: max-scaled, Kahan-compensated sum of squares.
+double js_hypot(std::initializer_list<double> xs) {
+  double mx = 0;
+  bool nan = false;
+  for (double x : xs) {
+    double a = std::fabs(x);
+    if (std::isnan(a)) nan = true;
+    else if (a > mx) mx = a;
+  }
+  if (std::isinf(mx)) return INFINITY;
+  if (nan) return NAN;
+  if (mx == 0) return 0;
+  double sum = 0, comp = 0;
+  for (double x : xs) {
+    double n = std::fabs(x) / mx;
+    double summand = n * n - comp;
+    double prelim = sum + summand;
+    comp = (prelim - sum) - summand;
+    sum = prelim;
+  }
+  return std::sqrt(sum) * mx;
+}
+
+// ------------------------------------------------------------------ rng.ts:5-14 (mulberry32)
+struct Rng {
+  uint32_t a;
+  explicit Rng(uint32_t seed) : a(seed) {}
+  double operator()() {
+    a = a + 0x6d2b79f5u;
+    uint32_t t = a;
+    t = (t ^ (t >> 15)) * (t | 1u);
+    t ^= t + (t ^ (t >> 7)) * (t | 61u);
+    return double(t ^ (t >> 14)) / 4294967296.0;
+  }
+};
+
+// ------------------------------------------------------------------ vessel3d.ts
+struct Teapot {
+  pl3d_teapot_config cfg;
+  double ax, ay, dx, dy, tipX, tipY;
+};
+
+Teapot build_teapot(const pl3d_teapot_config &cfg) {  // vessel3d.ts:46-59
+  Teapot t;
+  t.cfg = cfg;
+  double a = (cfg.spout_angle_deg * PI) / 180;
+  t.ax = cfg.radius - cfg.wall;
+  t.ay = cfg.inlet_height;
+  t.dx = std::cos(a);
+  t.dy = std::sin(a);
+  t.tipX = t.ax + std::cos(a) * cfg.spout_length;
+  t.tipY = t.ay + std::sin(a) * cfg.spout_length;
+  return t;
+}
+
+// vessel3d.ts:62-71: distance to the spout axis segment and the unclamped axial coordinate
+inline void spout_axis(const Teapot &t, double x, double y, double z, double &dist, double &s) {
+  double px = x - t.ax;
+  double py = y - t.ay;
+  s = px * t.dx + py * t.dy;
+  double L = t.cfg.spout_length;
+  double sc = s < 0 ? 0 : s > L ? L : s;
+  double qx = px - sc * t.dx;
+  double qy = py - sc * t.dy;
+  dist = std::sqrt(qx * qx + qy * qy + z * z);
+}
+
+// vessel3d.ts:74-95
+double teapot_sdf(const Teapot &t, double x, double y, double z) {
+  const pl3d_teapot_config &c = t.cfg;
+  double hw = c.wall / 2;
+  double r = std::sqrt(x * x + z * z);
+  double ring = std::fabs(r - (c.radius + hw)) - hw;
+  double yIn = std::max(-(y + c.wall), y - c.height);
+  double side = std::max(ring, yIn);
+  double floor_ = std::max(std::fabs(y + hw) - hw, r - (c.radius + c.wall));
+  double solid = std::min(side, floor_);
+  double dist, s;
+  spout_axis(t, x, y, z, dist, s);
+  double L = c.spout_length;
+  double along = std::max(-s, s - L);
+  double tube = std::max(std::fabs(dist - (c.bore_radius + hw)) - hw, along);
+  solid = std::min(solid, std::max(tube, c.radius - r));
+  double bore = std::max(std::max(dist - c.bore_radius, -s - c.wall), s - L);
+  return std::max(solid, -bore);
+}
+
+// vessel3d.ts:98-103
+bool in_cavity(const Teapot &t, double x, double y, double z) {
+  const pl3d_teapot_config &c = t.cfg;
+  if (y >= 0 && y <= c.height && x * x + z * z <= c.radius * c.radius) return true;
+  double dist, s;
+  spout_axis(t, x, y, z, dist, s);
+  return s >= 0 && s <= c.spout_length && dist <= c.bore_radius;
+}
+
+// ------------------------------------------------------------------ motion.ts PourController
+struct Controller {
+  pl3d_motion_config cfg;
+  int phase = PL3D_PHASE_WAIT;
+  double angle = 0;
+  double phaseStart = 0, startAngle = 0, startRate = 0, lastRate = 0;
+  int pour = 0;
+  double flowRate = 0, lastDischarged = 0;
+  double target = 0, manualRate = 0;
+
+  explicit Controller(const pl3d_motion_config &c) : cfg(c) {  // motion.ts:46-50
+    if (c.kind == PL3D_MOTION_STILL) phase = PL3D_PHASE_DONE;
+    if (c.kind == PL3D_MOTION_MANUAL) phase = PL3D_PHASE_MANUAL;
+  }
+  void set_target(double angleDeg, double) {  // motion.ts:40-44
+    double a = std::max(0.0, std::min(cfg.max_angle_deg, angleDeg));
+    target = a * DEG;
+  }
+  double pour_target() const {  // motion.ts:53-56
+    int reps = std::max(1, cfg.repeats);
+    return (cfg.target_volume * (pour + 1)) / reps;
+  }
+  void enter(int ph, double t) {  // motion.ts:62-68
+    phase = ph;
+    phaseStart = t;
+    startAngle = angle;
+    startRate = lastRate;
+  }
+  bool should_return(double t, double discharged) const {  // motion.ts:145-154
+    const pl3d_motion_config &c = cfg;
+    if (c.kind == PL3D_MOTION_TILT) return phase == PL3D_PHASE_HOLD && t - phaseStart >= c.hold_time;
+    if (phase == PL3D_PHASE_HOLD && t - phaseStart >= c.max_hold) return true;
+    if (c.target_volume <= 0) return false;
+    double inTransit = std::max(0.0, flowRate) * c.anticipation * c.return_duration;
+    return discharged + inTransit >= pour_target();
+  }
+  double advance(double t, double dt, double discharged) {  // motion.ts:74-143
+    const pl3d_motion_config &c = cfg;
+    if (dt > 0) {
+      double inst = (discharged - lastDischarged) / dt;
+      double k = std::min(1.0, dt / 0.25);
+      flowRate += (inst - flowRate) * k;
+    }
+    lastDischarged = discharged;
+    double prev = angle;
+    if (phase == PL3D_PHASE_MANUAL) {
+      if (dt > 0) {
+        double desired = std::max(-90 * DEG, std::min(90 * DEG, (target - angle) / 0.15));
+        double maxChange = 600 * DEG * dt;
+        manualRate += std::max(-maxChange, std::min(maxChange, desired - manualRate));
+        angle += manualRate * dt;
+        lastRate = (angle - prev) / dt;
+      }
+      return angle;
+    }
+    double hold = c.hold_angle_deg * DEG;
+    double maxA = c.max_angle_deg * DEG;
+    int reps = std::max(1, c.repeats);
+    for (int guard = 0; guard < 4; guard++) {
+      double el = t - phaseStart;
+      if (phase == PL3D_PHASE_WAIT && t >= c.start_delay) {
+        enter(PL3D_PHASE_TILT_IN, c.start_delay);
+      } else if ((phase == PL3D_PHASE_TILT_IN || phase == PL3D_PHASE_HOLD) &&
+                 should_return(t, discharged)) {
+        enter(PL3D_PHASE_RETURN, t);
+      } else if (phase == PL3D_PHASE_TILT_IN && el >= c.tilt_duration) {
+        angle = hold;
+        enter(PL3D_PHASE_HOLD, phaseStart + c.tilt_duration);
+      } else if (phase == PL3D_PHASE_RETURN && el >= c.return_duration) {
+        angle = 0;
+        pour++;
+        enter(PL3D_PHASE_REST, phaseStart + c.return_duration);
+      } else if (phase == PL3D_PHASE_REST && pour < reps && el >= c.repeat_gap) {
+        enter(PL3D_PHASE_TILT_IN, phaseStart + c.repeat_gap);
+      } else if (phase == PL3D_PHASE_REST && pour >= reps) {
+        enter(PL3D_PHASE_DONE, t);
+      } else
+        break;
+    }
+    double el = t - phaseStart;
+    switch (phase) {
+      case PL3D_PHASE_TILT_IN: {
+        double s = std::min(1.0, el / c.tilt_duration);
+        angle = startAngle + (hold - startAngle) * 0.5 * (1 - std::cos(PI * s));
+        break;
+      }
+      case PL3D_PHASE_HOLD:
+        angle = std::min(maxA, startAngle + c.hold_tilt_rate * DEG * el);
+        break;
+      case PL3D_PHASE_RETURN: {
+        double T = c.return_duration;
+        double s = std::min(1.0, el / T);
+        double h00 = 2 * std::pow(s, 3) - 3 * (s * s) + 1;
+        double h10 = std::pow(s, 3) - 2 * (s * s) + s;
+        angle = h00 * startAngle + h10 * T * startRate;
+        break;
+      }
+      default:
+        angle = 0;
+    }
+    if (dt > 0) lastRate = (angle - prev) / dt;
+    return angle;
+  }
+  double angle_deg() const { return angle / DEG; }
+};
+
+// ------------------------------------------------------------------ solver3d.ts:826-974 Pcg3D
+struct SolveResult {
+  int iterations;
+  bool converged;
+  double residual;
+};
+
+struct Pcg {
+  int nx, ny;
+  std::vector<double> adiag, ai, aj, ak, pc, r, z, s, as;
+  std::vector<int32_t> list;
+  int m = 0;
+  Pcg(int nx_, int ny_, int nz_) : nx(nx_), ny(ny_) {
+    size_t n = size_t(nx_) * ny_ * nz_;
+    adiag.assign(n, 0); ai.assign(n, 0); aj.assign(n, 0); ak.assign(n, 0); pc.assign(n, 0);
+    r.assign(n, 0); z.assign(n, 0); s.assign(n, 0); as.assign(n, 0);
+    list.assign(n, 0);
+  }
+
+  SolveResult solve(const uint8_t *cell, size_t ncell, const double *b, double *q, double tol,
+                    int maxIter) {
+    const int nxy = nx * ny;
+    std::fill(adiag.begin(), adiag.end(), 0.0);
+    std::fill(ai.begin(), ai.end(), 0.0);
+    std::fill(aj.begin(), aj.end(), 0.0);
+    std::fill(ak.begin(), ak.end(), 0.0);
+    std::fill(pc.begin(), pc.end(), 0.0);
+    std::fill(q, q + ncell, 0.0);
+    m = 0;
+    for (size_t c = 0; c < ncell; c++)
+      if (cell[c] == FLUID) list[m++] = int32_t(c);
+    double bnorm = 0;
+    const int offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
+    for (int t = 0; t < m; t++) {  // :872-881
+      int c = list[t];
+      int d = 0;
+      for (int o : offs)
+        if (cell[c + o] != SOLID) d++;
+      adiag[c] = d;
+      ai[c] = cell[c + 1] == FLUID ? -1 : 0;
+      aj[c] = cell[c + nx] == FLUID ? -1 : 0;
+      ak[c] = cell[c + nxy] == FLUID ? -1 : 0;
+      bnorm = std::max(bnorm, std::fabs(b[c]));
+    }
+    if (m == 0 || bnorm < 1e-14) return {0, true, bnorm};
+    const double tau = 0.97;
+    for (int t = 0; t < m; t++) {  // MIC(0) factor :884-900
+      int c = list[t];
+      double pi = pc[c - 1], pj = pc[c - nx], pk = pc[c - nxy];
+      double e1 = ai[c - 1] * pi, e2 = aj[c - nx] * pj, e3 = ak[c - nxy] * pk;
+      double e = adiag[c] - e1 * e1 - e2 * e2 - e3 * e3 -
+                 tau * (ai[c - 1] * (aj[c - 1] + ak[c - 1]) * pi * pi +
+                        aj[c - nx] * (ai[c - nx] + ak[c - nx]) * pj * pj +
+                        ak[c - nxy] * (ai[c - nxy] + aj[c - nxy]) * pk * pk);
+      if (e < 0.25 * adiag[c]) e = adiag[c];
+      pc[c] = e > 0 ? 1 / std::sqrt(e) : 0;
+    }
+    for (int t = 0; t < m; t++) r[list[t]] = b[list[t]];
+    precondition();
+    double sigma = 0;
+    for (int t = 0; t < m; t++) {
+      int c = list[t];
+      s[c] = z[c];
+      sigma += z[c] * r[c];
+    }
+    double target = std::max(tol * bnorm, 1e-14);
+    double res = bnorm;
+    for (int it = 1; it <= maxIter; it++) {  // :912-935
+      applyA();
+      double sas = 0;
+      for (int t = 0; t < m; t++) sas += s[list[t]] * as[list[t]];
+      if (!(sas > 0)) break;
+      double alpha = sigma / sas;
+      res = 0;
+      for (int t = 0; t < m; t++) {
+        int c = list[t];
+        q[c] += alpha * s[c];
+        r[c] -= alpha * as[c];
+        res = std::max(res, std::fabs(r[c]));
+      }
+      if (res <= target) return {it, true, res};
+      precondition();
+      double sn = 0;
+      for (int t = 0; t < m; t++) sn += z[list[t]] * r[list[t]];
+      double beta = sn / sigma;
+      for (int t = 0; t < m; t++) {
+        int c = list[t];
+        s[c] = z[c] + beta * s[c];
+      }
+      sigma = sn;
+    }
+    return {maxIter, res <= target, res};
+  }
+
+  void applyA() {  // :939-953 (x = s, y = as)
+    const int nxy = nx * ny;
+    for (int t = 0; t < m; t++) {
+      int c = list[t];
+      as[c] = adiag[c] * s[c] + ai[c] * s[c + 1] + ai[c - 1] * s[c - 1] + aj[c] * s[c + nx] +
+              aj[c - nx] * s[c - nx] + ak[c] * s[c + nxy] + ak[c - nxy] * s[c - nxy];
+    }
+  }
+
+  void precondition() {  // :955-973 (r -> z, scratch in as)
+    const int nxy = nx * ny;
+    double *qv = as.data();
+    for (int t = 0; t < m; t++) {
+      int c = list[t];
+      double v = r[c] - ai[c - 1] * pc[c - 1] * qv[c - 1] - aj[c - nx] * pc[c - nx] * qv[c - nx] -
+                 ak[c - nxy] * pc[c - nxy] * qv[c - nxy];
+      qv[c] = v * pc[c];
+    }
+    for (int t = m - 1; t >= 0; t--) {
+      int c = list[t];
+      double v = qv[c] - ai[c] * pc[c] * z[c + 1] - aj[c] * pc[c] * z[c + nx] -
+                 ak[c] * pc[c] * z[c + nxy];
+      z[c] = v * pc[c];
+    }
+  }
+};
+
+// ------------------------------------------------------------------ solver3d.ts:62-824 Simulation3D
+struct Sim {
+  pl3d_config cfg;
+  Teapot pot;
+  Controller controller;
+  int nx, ny, nz;
+  double h, fillHeight;
+  double pivotWorld[2];
+  int n;
+  double parcelVolume;
+  std::vector<double> x, y, z, vx, vy, vz;
+  std::vector<uint8_t> origin, status;
+  std::vector<float> y0;
+  int bandCounts[3] = {0, 0, 0};
+  std::vector<int32_t> dischargeOrder;
+  double time = 0;
+  int64_t steps = 0;
+  double phi = 0, omega = 0;
+  std::vector<uint8_t> cell;
+  std::vector<double> u, v, w, uPrev, vPrev, wPrev, uw, vw, ww;
+  std::vector<uint8_t> uValid, vValid, wValid;
+  std::vector<double> rhs, q, density, uc, vc, wc, qc;
+  // copies kept for phase-by-phase comparisons (never read by the algorithm)
+  std::vector<double> uProj, vProj, wProj, rhsP, rhsD;
+  Pcg pcg;
+  int lastIterations = 0, failures = 0;
+  double maxDiv = 0, interior = 1;
+  int counts[4] = {0, 0, 0, 0};
+  int lastSolves[4] = {0, 0, 0, 0};
+  double st[3];
+
+  explicit Sim(const pl3d_config &c)
+      : cfg(c), pot(build_teapot(c.teapot)), controller(c.motion), pcg(1, 1, 1) {
+    h = cfg.h;
+    const pl3d_teapot_config &t = cfg.teapot;
+    fillHeight = cfg.initial_volume / (PI * t.radius * t.radius);
+    const double pts[6][2] = {
+        {-t.radius - t.wall, -t.wall},          {-t.radius - t.wall, t.height + 0.02},
+        {t.radius + t.wall, -t.wall},           {t.radius + t.wall, t.height + 0.02},
+        {pot.tipX + 0.02, pot.tipY + 0.02},     {pot.tipX + 0.02, pot.tipY - 0.03},
+    };
+    double X0 = INFINITY, X1 = -INFINITY, Y0 = INFINITY, Y1 = -INFINITY;
+    double maxA = std::max(cfg.motion.max_angle_deg, cfg.motion.hold_angle_deg);
+    for (int a = 0; a <= maxA; a += 2) {  // :138-152
+      double ph = (-a * PI) / 180;
+      double cc = std::cos(ph), ss = std::sin(ph);
+      for (const auto &p : pts) {
+        double rx = p[0] - cfg.pivot[0];
+        double ry = p[1] - cfg.pivot[1];
+        double wx = cc * rx - ss * ry;
+        double wy = ss * rx + cc * ry;
+        X0 = std::min(X0, wx);
+        X1 = std::max(X1, wx);
+        Y0 = std::min(Y0, wy);
+        Y1 = std::max(Y1, wy);
+      }
+    }
+    double mg = 3 * h;
+    pivotWorld[0] = -X0 + mg;
+    pivotWorld[1] = -Y0 + mg;
+    nx = int(std::ceil((X1 - X0 + 2 * mg) / h));
+    ny = int(std::ceil((Y1 - Y0 + 2 * mg) / h));
+    nz = int(std::ceil((t.radius + t.wall + 2 * h) / h)) + 1;
+    size_t nc = size_t(nx) * ny * nz;
+    cell.assign(nc, 0);
+    size_t nu = size_t(nx + 1) * ny * nz, nv = size_t(nx) * (ny + 1) * nz,
+           nw = size_t(nx) * ny * (nz + 1);
+    for (auto *f : {&u, &uPrev, &uw, &uc, &uProj}) f->assign(nu, 0);
+    for (auto *f : {&v, &vPrev, &vw, &vc, &vProj}) f->assign(nv, 0);
+    for (auto *f : {&w, &wPrev, &ww, &wc, &wProj}) f->assign(nw, 0);
+    uValid.assign(nu, 0);
+    vValid.assign(nv, 0);
+    wValid.assign(nw, 0);
+    for (auto *f : {&rhs, &q, &density, &qc, &rhsP, &rhsD}) f->assign(nc, 0);
+    pcg = Pcg(nx, ny, nz);
+
+    // Seeding :185-200
+    Rng rng(cfg.seed);
+    double s = h / 2;
+    std::vector<double> xs;
+    double loc[3];
+    for (int k = 0; k < nz * 2; k++)
+      for (int j = 0; j < ny * 2; j++)
+        for (int i = 0; i < nx * 2; i++) {
+          double wx = (i + 0.5 + (rng() - 0.5) * 0.5) * s;
+          double wy = (j + 0.5 + (rng() - 0.5) * 0.5) * s;
+          double wz = (k + 0.5 + (rng() - 0.5) * 0.5) * s;
+          to_local(phi, wx, wy, wz, loc);
+          if (loc[2] < 0 || loc[1] < 0 || loc[1] > fillHeight) continue;
+          if (!in_cavity(pot, loc[0], loc[1], loc[2])) continue;
+          if (teapot_sdf(pot, loc[0], loc[1], loc[2]) < 0.4 * h) continue;
+          xs.push_back(wx);
+          xs.push_back(wy);
+          xs.push_back(wz);
+        }
+    n = int(xs.size() / 3);
+    parcelVolume = cfg.initial_volume / (2 * n);
+    x.resize(n); y.resize(n); z.resize(n);
+    vx.assign(n, 0); vy.assign(n, 0); vz.assign(n, 0);
+    origin.assign(n, 0);
+    status.assign(n, IN_POT);
+    y0.resize(n);
+    for (int p = 0; p < n; p++) {
+      x[p] = xs[3 * p];
+      y[p] = xs[3 * p + 1];
+      z[p] = xs[3 * p + 2];
+      to_local(phi, x[p], y[p], z[p], loc);
+      y0[p] = float(loc[1]);
+    }
+    // Equal-volume bands :220-226 (Float32 y0, ties by index)
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) {
+      double d = double(y0[a]) - double(y0[b]);
+      return d < 0 || (d == 0 && a < b);
+    });
+    for (int r = 0; r < n; r++) {
+      int k = std::min(2, int(std::floor(double(r * 3.0) / n)));
+      origin[order[r]] = uint8_t(k);
+      bandCounts[k]++;
+    }
+    counts[0] = n;
+  }
+
+  // frames :232-260
+  void to_local(double ph, double wx, double wy, double wz, double out[3]) const {
+    double c = std::cos(ph), s = std::sin(ph);
+    double rx = wx - pivotWorld[0];
+    double ry = wy - pivotWorld[1];
+    out[0] = c * rx + s * ry + cfg.pivot[0];
+    out[1] = -s * rx + c * ry + cfg.pivot[1];
+    out[2] = wz - h;
+  }
+  void to_world(double ph, double lx, double ly, double lz, double out[3]) const {
+    double c = std::cos(ph), s = std::sin(ph);
+    double rx = lx - cfg.pivot[0];
+    double ry = ly - cfg.pivot[1];
+    out[0] = c * rx - s * ry + pivotWorld[0];
+    out[1] = s * rx + c * ry + pivotWorld[1];
+    out[2] = lz + h;
+  }
+
+  bool finished() const { return time >= cfg.duration - 1e-9; }  // :264-266
+
+  double stable_dt() const {  // :268-278
+    double m = 0;
+    for (int p = 0; p < n; p++) {
+      if (status[p] != IN_POT) continue;
+      double s = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+      if (s > m) m = s;
+    }
+    double R = js_hypot({double(nx), double(ny)}) * h;
+    double speed = std::sqrt(m) + std::sqrt(5 * h * cfg.gravity) + std::fabs(omega) * R;
+    return std::min(cfg.max_substep, (cfg.cfl * h) / speed);
+  }
+
+  void advance(double duration) {  // :281-286
+    double end = time + duration;
+    while (time < end - 1e-12 && !finished()) step(std::min(stable_dt(), cfg.duration - time));
+  }
+
+  void step(double dt) {  // :288-309
+    double discharged = counts[DISCHARGED] * parcelVolume * 2;
+    double tilt1 = controller.advance(time + dt, dt, discharged);
+    double phi1 = -tilt1;
+    omega = (phi1 - phi) / dt;
+    classify();
+    particles_to_grid();
+    uPrev = u;
+    vPrev = v;
+    wPrev = w;
+    apply_gravity(dt);
+    set_solid_faces();
+    project();
+    mark_valid();
+    extrapolate();
+    grid_to_particles();
+    density_correction(dt);
+    advect(dt, phi1);
+    phi = phi1;
+    time += dt;
+    steps++;
+  }
+
+  void classify() {  // :313-336
+    double loc[3];
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          size_t c = i + size_t(nx) * (j + size_t(ny) * k);
+          if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) {
+            cell[c] = SOLID;
+            continue;
+          }
+          to_local(phi, (i + 0.5) * h, (j + 0.5) * h, (k + 0.5) * h, loc);
+          cell[c] = teapot_sdf(pot, loc[0], loc[1], loc[2]) < 0 ? SOLID : AIR;
+        }
+    for (int p = 0; p < n; p++) {
+      if (status[p] != IN_POT) continue;
+      double fi = std::floor(x[p] / h), fj = std::floor(y[p] / h), fk = std::floor(z[p] / h);
+      if (fi < 1 || fj < 1 || fk < 1 || fi >= nx - 1 || fj >= ny - 1 || fk >= nz - 1) continue;
+      size_t c = size_t(fi) + size_t(nx) * (size_t(fj) + size_t(ny) * size_t(fk));
+      if (cell[c] == AIR) cell[c] = FLUID;
+    }
+  }
+
+  // :339-362 — clamped trilinear stencil; fractions into out, returns the base index
+  size_t stencil(double px, double py, double pz, double ox, double oy, double oz, int sx, int sy,
+                 int sz, double *out) const {
+    double gx = px / h - ox, gy = py / h - oy, gz = pz / h - oz;
+    double i = std::max(0.0, std::min(double(sx - 2), std::floor(gx)));
+    double j = std::max(0.0, std::min(double(sy - 2), std::floor(gy)));
+    double k = std::max(0.0, std::min(double(sz - 2), std::floor(gz)));
+    out[0] = std::max(0.0, std::min(1.0, gx - i));
+    out[1] = std::max(0.0, std::min(1.0, gy - j));
+    out[2] = std::max(0.0, std::min(1.0, gz - k));
+    return size_t(i) + size_t(sx) * (size_t(j) + size_t(sy) * size_t(k));
+  }
+
+  // :366-377 corner order and weights
+  template <class F>
+  static inline void for_each_corner(const double *f3, size_t base, int sx, int sy, F &&cb) {
+    double fx = f3[0], fy = f3[1], fz = f3[2];
+    size_t sxy = size_t(sx) * sy;
+    cb(base, (1 - fx) * (1 - fy) * (1 - fz));
+    cb(base + 1, fx * (1 - fy) * (1 - fz));
+    cb(base + sx, (1 - fx) * fy * (1 - fz));
+    cb(base + sx + 1, fx * fy * (1 - fz));
+    cb(base + sxy, (1 - fx) * (1 - fy) * fz);
+    cb(base + sxy + 1, fx * (1 - fy) * fz);
+    cb(base + sxy + sx, (1 - fx) * fy * fz);
+    cb(base + sxy + sx + 1, fx * fy * fz);
+  }
+
+  void dims(int comp, double &ox, double &oy, double &oz, int &sx, int &sy, int &sz) const {
+    if (comp == 0) { ox = 0; oy = 0.5; oz = 0.5; sx = nx + 1; sy = ny; sz = nz; }
+    else if (comp == 1) { ox = 0.5; oy = 0; oz = 0.5; sx = nx; sy = ny + 1; sz = nz; }
+    else { ox = 0.5; oy = 0.5; oz = 0; sx = nx; sy = ny; sz = nz + 1; }
+  }
+
+  // :386-408
+  double sample(const std::vector<double> &field, int comp, double px, double py, double pz,
+                const std::vector<uint8_t> *valid) {
+    double ox, oy, oz;
+    int sx, sy, sz;
+    dims(comp, ox, oy, oz, sx, sy, sz);
+    size_t base = stencil(px, py, pz, ox, oy, oz, sx, sy, sz, st);
+    double acc = 0, ws = 0, all = 0;
+    for_each_corner(st, base, sx, sy, [&](size_t idx, double wgt) {
+      all += wgt * field[idx];
+      if (!valid || (*valid)[idx]) {
+        acc += wgt * field[idx];
+        ws += wgt;
+      }
+    });
+    if (!valid) return all;
+    return ws > 1e-9 ? acc / ws : all;
+  }
+
+  void particles_to_grid() {  // :410-431
+    std::vector<double> *fs[3] = {&u, &v, &w};
+    std::vector<double> *wts[3] = {&uw, &vw, &ww};
+    std::vector<double> *vel[3] = {&vx, &vy, &vz};
+    for (int comp = 0; comp < 3; comp++) {
+      std::vector<double> &f = *fs[comp];
+      std::vector<double> &wt = *wts[comp];
+      const std::vector<double> &vv = *vel[comp];
+      std::fill(f.begin(), f.end(), 0.0);
+      std::fill(wt.begin(), wt.end(), 0.0);
+      double ox, oy, oz;
+      int sx, sy, sz;
+      dims(comp, ox, oy, oz, sx, sy, sz);
+      for (int p = 0; p < n; p++) {
+        if (status[p] != IN_POT) continue;
+        size_t base = stencil(x[p], y[p], z[p], ox, oy, oz, sx, sy, sz, st);
+        double vp = vv[p];
+        for_each_corner(st, base, sx, sy, [&](size_t idx, double wgt) {
+          f[idx] += wgt * vp;
+          wt[idx] += wgt;
+        });
+      }
+      for (size_t i = 0; i < f.size(); i++) f[i] = wt[i] > 1e-9 ? f[i] / wt[i] : 0;
+    }
+  }
+
+  void apply_gravity(double dt) {  // :433-443
+    double g = cfg.gravity * dt;
+    for (int k = 0; k < nz; k++)
+      for (int j = 1; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          uint8_t below = cell[i + size_t(nx) * (j - 1 + size_t(ny) * k)];
+          uint8_t above = cell[i + size_t(nx) * (j + size_t(ny) * k)];
+          if (below == FLUID || above == FLUID) v[i + size_t(nx) * (j + size_t(ny + 1) * k)] -= g;
+        }
+  }
+
+  void set_solid_faces() {  // :446-471
+    double om = omega;
+    double pwx = pivotWorld[0], pwy = pivotWorld[1];
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i <= nx; i++) {
+          uint8_t a = i > 0 ? cell[i - 1 + size_t(nx) * (j + size_t(ny) * k)] : SOLID;
+          uint8_t b = i < nx ? cell[i + size_t(nx) * (j + size_t(ny) * k)] : SOLID;
+          if (a == SOLID || b == SOLID)
+            u[i + size_t(nx + 1) * (j + size_t(ny) * k)] = -om * ((j + 0.5) * h - pwy);
+        }
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j <= ny; j++)
+        for (int i = 0; i < nx; i++) {
+          uint8_t a = j > 0 ? cell[i + size_t(nx) * (j - 1 + size_t(ny) * k)] : SOLID;
+          uint8_t b = j < ny ? cell[i + size_t(nx) * (j + size_t(ny) * k)] : SOLID;
+          if (a == SOLID || b == SOLID)
+            v[i + size_t(nx) * (j + size_t(ny + 1) * k)] = om * ((i + 0.5) * h - pwx);
+        }
+    for (int k = 0; k <= nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          uint8_t a = k > 0 ? cell[i + size_t(nx) * (j + size_t(ny) * (k - 1))] : SOLID;
+          uint8_t b = k < nz ? cell[i + size_t(nx) * (j + size_t(ny) * k)] : SOLID;
+          if (a == SOLID || b == SOLID) w[i + size_t(nx) * (j + size_t(ny) * k)] = 0;
+        }
+  }
+
+  double divergence_rhs() {  // :473-493
+    std::fill(rhs.begin(), rhs.end(), 0.0);
+    double m = 0;
+    for (int k = 1; k < nz - 1; k++)
+      for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+          size_t c = i + size_t(nx) * (j + size_t(ny) * k);
+          if (cell[c] != FLUID) continue;
+          double d = u[i + 1 + size_t(nx + 1) * (j + size_t(ny) * k)] -
+                     u[i + size_t(nx + 1) * (j + size_t(ny) * k)] +
+                     v[i + size_t(nx) * (j + 1 + size_t(ny + 1) * k)] -
+                     v[i + size_t(nx) * (j + size_t(ny + 1) * k)] +
+                     w[i + size_t(nx) * (j + size_t(ny) * (k + 1))] -
+                     w[i + size_t(nx) * (j + size_t(ny) * k)];
+          rhs[c] = -d;
+          if (std::fabs(d) > m) m = std::fabs(d);
+        }
+    return m / h;
+  }
+
+  // :496-528
+  void apply_gradient(const std::vector<double> &qq, std::vector<double> &fu,
+                      std::vector<double> &fv, std::vector<double> &fw, double sign) {
+    auto skip = [&](size_t a, size_t b) {
+      return cell[a] == SOLID || cell[b] == SOLID || (cell[a] != FLUID && cell[b] != FLUID);
+    };
+    for (int k = 0; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 1; i < nx; i++) {
+          size_t a = i - 1 + size_t(nx) * (j + size_t(ny) * k), b = a + 1;
+          if (skip(a, b)) continue;
+          fu[i + size_t(nx + 1) * (j + size_t(ny) * k)] += sign * (qq[b] - qq[a]);
+        }
+    for (int k = 0; k < nz; k++)
+      for (int j = 1; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          size_t a = i + size_t(nx) * (j - 1 + size_t(ny) * k), b = a + nx;
+          if (skip(a, b)) continue;
+          fv[i + size_t(nx) * (j + size_t(ny + 1) * k)] += sign * (qq[b] - qq[a]);
+        }
+    for (int k = 1; k < nz; k++)
+      for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+          size_t a = i + size_t(nx) * (j + size_t(ny) * (k - 1)), b = a + size_t(nx) * ny;
+          if (skip(a, b)) continue;
+          fw[i + size_t(nx) * (j + size_t(ny) * k)] += sign * (qq[b] - qq[a]);
+        }
+  }
+
+  void project() {  // :530-543
+    divergence_rhs();
+    rhsP = rhs;
+    SolveResult res = pcg.solve(cell.data(), cell.size(), rhs.data(), q.data(),
+                                cfg.pressure_tolerance, cfg.pressure_max_iterations);
+    lastIterations = res.iterations;
+    lastSolves[0] = res.iterations;
+    lastSolves[1] = res.converged;
+    if (!res.converged) failures++;
+    apply_gradient(q, u, v, w, -1);
+    uProj = u;
+    vProj = v;
+    wProj = w;
+    maxDiv = divergence_rhs();
+  }
+
+  void mark_valid() {  // :545-562
+    auto mark = [&](std::vector<uint8_t> &valid, int sx, int sy, int sz, int comp) {
+      for (int k = 0; k < sz; k++)
+        for (int j = 0; j < sy; j++)
+          for (int i = 0; i < sx; i++) {
+            int ia = comp == 0 ? i - 1 : i, ja = comp == 1 ? j - 1 : j, ka = comp == 2 ? k - 1 : k;
+            uint8_t a = ia >= 0 && ja >= 0 && ka >= 0
+                            ? cell[ia + size_t(nx) * (ja + size_t(ny) * ka)]
+                            : SOLID;
+            uint8_t b = i < nx && j < ny && k < nz ? cell[i + size_t(nx) * (j + size_t(ny) * k)]
+                                                   : SOLID;
+            valid[i + size_t(sx) * (j + size_t(sy) * k)] =
+                a == SOLID || b == SOLID || a == FLUID || b == FLUID ? 1 : 0;
+          }
+    };
+    mark(uValid, nx + 1, ny, nz, 0);
+    mark(vValid, nx, ny + 1, nz, 1);
+    mark(wValid, nx, ny, nz + 1, 2);
+  }
+
+  void extrapolate() {  // :565-605
+    auto ex = [&](std::vector<double> &f, const std::vector<uint8_t> &valid, int sx, int sy,
+                  int sz) {
+      std::vector<uint8_t> vm = valid;
+      std::vector<size_t> add;
+      std::vector<double> vals;
+      size_t sxy = size_t(sx) * sy;
+      for (int layer = 0; layer < 2; layer++) {
+        add.clear();
+        vals.clear();
+        for (int k = 0; k < sz; k++)
+          for (int j = 0; j < sy; j++)
+            for (int i = 0; i < sx; i++) {
+              size_t c = i + size_t(sx) * (j + size_t(sy) * k);
+              if (vm[c]) continue;
+              double s = 0;
+              int cnt = 0;
+              auto nb = [&](size_t d, bool ok) {
+                if (ok && vm[d]) {
+                  s += f[d];
+                  cnt++;
+                }
+              };
+              nb(c - 1, i > 0);
+              nb(c + 1, i < sx - 1);
+              nb(c - sx, j > 0);
+              nb(c + sx, j < sy - 1);
+              nb(c - sxy, k > 0);
+              nb(c + sxy, k < sz - 1);
+              if (cnt > 0) {
+                add.push_back(c);
+                vals.push_back(s / cnt);
+              }
+            }
+        for (size_t t = 0; t < add.size(); t++) {
+          f[add[t]] = vals[t];
+          vm[add[t]] = 1;
+        }
+      }
+    };
+    ex(u, uValid, nx + 1, ny, nz);
+    ex(v, vValid, nx, ny + 1, nz);
+    ex(w, wValid, nx, ny, nz + 1);
+  }
+
+  void grid_to_particles() {  // :607-624
+    double a = cfg.flip_ratio;
+    for (int p = 0; p < n; p++) {
+      if (status[p] != IN_POT) continue;
+      double px = x[p], py = y[p], pz = z[p];
+      double un = sample(u, 0, px, py, pz, &uValid);
+      double vn = sample(v, 1, px, py, pz, &vValid);
+      double wn = sample(w, 2, px, py, pz, &wValid);
+      double uo = sample(uPrev, 0, px, py, pz, &uValid);
+      double vo = sample(vPrev, 1, px, py, pz, &vValid);
+      double wo = sample(wPrev, 2, px, py, pz, &wValid);
+      vx[p] = (1 - a) * un + a * (vx[p] + un - uo);
+      vy[p] = (1 - a) * vn + a * (vy[p] + vn - vo);
+      vz[p] = (1 - a) * wn + a * (vz[p] + wn - wo);
+    }
+  }
+
+  void density_correction(double dt) {  // :627-675
+    double kappa = cfg.density_correction;
+    std::fill(uc.begin(), uc.end(), 0.0);
+    std::fill(vc.begin(), vc.end(), 0.0);
+    std::fill(wc.begin(), wc.end(), 0.0);
+    std::fill(density.begin(), density.end(), 0.0);
+    std::fill(qc.begin(), qc.end(), 0.0);
+    double ds[3];
+    for (int p = 0; p < n; p++) {
+      if (status[p] != IN_POT) continue;
+      size_t base = stencil(x[p], y[p], z[p], 0.5, 0.5, 0.5, nx, ny, nz, ds);
+      for_each_corner(ds, base, nx, ny, [&](size_t idx, double wgt) { density[idx] += wgt; });
+    }
+    std::fill(rhs.begin(), rhs.end(), 0.0);
+    int interiorN = 0;
+    double sum = 0;
+    bool any = false;
+    size_t nxy = size_t(nx) * ny;
+    for (int k = 1; k < nz - 1; k++)
+      for (int j = 1; j < ny - 1; j++)
+        for (int i = 1; i < nx - 1; i++) {
+          size_t c = i + size_t(nx) * (j + size_t(ny) * k);
+          if (cell[c] != FLUID) continue;
+          double ratio = density[c] / 8;
+          bool isInterior = cell[c - 1] != AIR && cell[c + 1] != AIR && cell[c - nx] != AIR &&
+                            cell[c + nx] != AIR && cell[c - nxy] != AIR && cell[c + nxy] != AIR;
+          if (isInterior) {
+            interiorN++;
+            sum += ratio;
+          }
+          if (kappa > 0 && (ratio > 1 || isInterior)) {
+            rhs[c] = (kappa * (ratio - 1) * h) / dt;
+            any = true;
+          }
+        }
+    interior = interiorN > 0 ? sum / interiorN : 1;
+    rhsD = rhs;
+    lastSolves[2] = 0;
+    lastSolves[3] = 1;
+    if (!any) return;
+    SolveResult r = pcg.solve(cell.data(), cell.size(), rhs.data(), qc.data(), 1e-4, 80);
+    lastSolves[2] = r.iterations;
+    lastSolves[3] = r.converged;
+    apply_gradient(qc, uc, vc, wc, -1);
+  }
+
+  void set_status(int p, uint8_t s) {  // :790-794
+    counts[status[p]]--;
+    counts[s]++;
+    status[p] = s;
+  }
+
+  void advect(double dt, double phi1) {  // :677-788
+    double out[3], loc0[3], loc1[3];
+    bool corr = cfg.density_correction > 0;
+    const double eps = 1e-4;
+    double margin = 0.1 * h;
+    double lo = h * 1.01;
+    double hiX = (nx - 1) * h, hiY = (ny - 1) * h, hiZ = (nz - 1) * h;
+    double g = cfg.gravity;
+    const pl3d_teapot_config &pc = pot.cfg;
+    for (int p = 0; p < n; p++) {
+      uint8_t stp = status[p];
+      if (stp == DISCHARGED || stp == SPILLED) {
+        if (y[p] > -1) {
+          vy[p] -= g * dt;
+          x[p] += vx[p] * dt;
+          y[p] += vy[p] * dt;
+          z[p] += vz[p] * dt;
+        }
+        continue;
+      }
+      if (stp != IN_POT) continue;
+      double x0 = x[p], y0_ = y[p], z0 = z[p];
+      to_local(phi, x0, y0_, z0, loc0);
+      double u1 = sample(u, 0, x0, y0_, z0, nullptr);
+      double v1 = sample(v, 1, x0, y0_, z0, nullptr);
+      double w1 = sample(w, 2, x0, y0_, z0, nullptr);
+      double xm = x0 + 0.5 * dt * u1, ym = y0_ + 0.5 * dt * v1, zm = z0 + 0.5 * dt * w1;
+      double x1 = x0 + dt * sample(u, 0, xm, ym, zm, nullptr);
+      double y1 = y0_ + dt * sample(v, 1, xm, ym, zm, nullptr);
+      double z1 = z0 + dt * sample(w, 2, xm, ym, zm, nullptr);
+      if (corr) {
+        x1 += dt * sample(uc, 0, x0, y0_, z0, nullptr);
+        y1 += dt * sample(vc, 1, x0, y0_, z0, nullptr);
+        z1 += dt * sample(wc, 2, x0, y0_, z0, nullptr);
+      }
+      to_local(phi1, x1, y1, z1, loc1);
+      double d = teapot_sdf(pot, loc1[0], loc1[1], loc1[2]);
+      if (d < margin) {
+        double gx = teapot_sdf(pot, loc1[0] + eps, loc1[1], loc1[2]) -
+                    teapot_sdf(pot, loc1[0] - eps, loc1[1], loc1[2]);
+        double gy = teapot_sdf(pot, loc1[0], loc1[1] + eps, loc1[2]) -
+                    teapot_sdf(pot, loc1[0], loc1[1] - eps, loc1[2]);
+        double gz = teapot_sdf(pot, loc1[0], loc1[1], loc1[2] + eps) -
+                    teapot_sdf(pot, loc1[0], loc1[1], loc1[2] - eps);
+        double gl = js_hypot({gx, gy, gz});
+        if (gl == 0 || std::isnan(gl)) gl = 1;  // `Math.hypot(...) || 1`: 0 and NaN are falsy
+        double push = margin - d;
+        loc1[0] += (gx / gl) * push;
+        loc1[1] += (gy / gl) * push;
+        loc1[2] += (gz / gl) * push;
+        to_world(phi1, loc1[0], loc1[1], loc1[2], out);
+        x1 = out[0];
+        y1 = out[1];
+        z1 = out[2];
+        double c = std::cos(phi1), s = std::sin(phi1);
+        double nwx = (c * gx - s * gy) / gl;
+        double nwy = (s * gx + c * gy) / gl;
+        double nwz = gz / gl;
+        double wvx = -omega * (y1 - pivotWorld[1]);
+        double wvy = omega * (x1 - pivotWorld[0]);
+        double rn = (vx[p] - wvx) * nwx + (vy[p] - wvy) * nwy + vz[p] * nwz;
+        if (rn < 0) {
+          vx[p] -= rn * nwx;
+          vy[p] -= rn * nwy;
+          vz[p] -= rn * nwz;
+        }
+      }
+      if (z1 < h) {
+        z1 = 2 * h - z1;
+        vz[p] = std::fabs(vz[p]);
+      }
+      if (!std::isfinite(x1 + y1 + z1)) {
+        set_status(p, LOST);
+        continue;
+      }
+      x[p] = x1;
+      y[p] = y1;
+      z[p] = z1;
+      to_local(phi1, x1, y1, z1, loc1);
+      double s0 = (loc0[0] - pot.tipX) * pot.dx + (loc0[1] - pot.tipY) * pot.dy;
+      double s1 = (loc1[0] - pot.tipX) * pot.dx + (loc1[1] - pot.tipY) * pot.dy;
+      if (s0 < 0 && s1 >= 0) {
+        double rx = loc1[0] - pot.tipX - s1 * pot.dx;
+        double ry = loc1[1] - pot.tipY - s1 * pot.dy;
+        if (js_hypot({rx, ry, loc1[2]}) < pc.bore_radius + 3 * pc.wall) {
+          set_status(p, DISCHARGED);
+          dischargeOrder.push_back(p);
+          continue;
+        }
+      }
+      double r = js_hypot({loc1[0], loc1[2]});
+      bool insideBody = r < pc.radius + 2 * pc.wall && loc1[1] > -2 * pc.wall;
+      bool nearSpout = loc1[0] > 0 && loc1[0] < pot.tipX + 0.03;
+      bool atEdge = x1 <= lo || y1 <= lo || x1 >= hiX || y1 >= hiY || z1 >= hiZ;
+      if ((!insideBody && !nearSpout) || atEdge) set_status(p, SPILLED);
+    }
+  }
+
+  void diagnostics(pl3d_diagnostics *d) const {  // :796-814
+    double vml = parcelVolume * 2 * 1e6;
+    std::memset(d, 0, sizeof(*d));
+    d->time = time;
+    d->steps = steps;
+    d->angle_deg = controller.angle_deg();
+    d->phase = controller.phase;
+    d->particles = n;
+    d->grid[0] = nx;
+    d->grid[1] = ny;
+    d->grid[2] = nz;
+    d->discharged_ml = counts[DISCHARGED] * vml;
+    d->spilled_ml = counts[SPILLED] * vml;
+    d->lost_ml = counts[LOST] * vml;
+    d->in_pot_ml = counts[IN_POT] * vml;
+    d->pressure_iterations = lastIterations;
+    d->pressure_failures = failures;
+    d->correction_iterations = lastSolves[2];
+    d->max_divergence = maxDiv;
+    d->interior_density = interior;
+  }
+};
+
+void default_teapot(pl3d_teapot_config *t) {  // vessel3d.ts:22-32
+  t->radius = 0.065;
+  t->height = 0.15;
+  t->wall = 0.008;
+  t->inlet_height = 0.035;
+  t->bore_radius = 0.01;
+  t->spout_angle_deg = 45;
+  t->spout_length = 0.14;
+}
+
+void default_motion(pl3d_motion_config *m) {  // config.ts:168-186
+  m->kind = PL3D_MOTION_POUR;
+  m->start_delay = 0.5;
+  m->tilt_duration = 2;
+  m->hold_angle_deg = 22;
+  m->hold_tilt_rate = 3;
+  m->max_angle_deg = 80;
+  m->return_duration = 1.2;
+  m->max_hold = 12;
+  m->hold_time = 1.5;
+  m->target_volume = 400 * 1e-6;
+  m->anticipation = 0.254;
+  m->repeats = 1;
+  m->repeat_gap = 1.5;
+  m->settle_time = 2;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C API
+struct or3d_sim {
+  Sim sim;
+  explicit or3d_sim(const pl3d_config &c) : sim(c) {}
+};
+struct or3d_controller {
+  Controller c;
+  explicit or3d_controller(const pl3d_motion_config &m) : c(m) {}
+};
+
+extern "C" {
+
+void or3d_default_teapot(pl3d_teapot_config *out) { default_teapot(out); }
+void or3d_default_motion(pl3d_motion_config *out) { default_motion(out); }
+
+void or3d_gentle_pour(double cell_mm, double seconds, pl3d_config *c) {  // presets3d.ts:9-26
+  std::memset(c, 0, sizeof(*c));
+  default_teapot(&c->teapot);
+  c->h = cell_mm / 1000;
+  c->initial_volume = 1500 * 1e-6;
+  default_motion(&c->motion);
+  c->motion.max_hold = 20;
+  c->pivot[0] = 0;
+  c->pivot[1] = 0.06;
+  c->gravity = 9.81;
+  c->flip_ratio = 0.95;
+  c->density_correction = 0.3;
+  c->cfl = 0.8;
+  c->max_substep = 1.0 / 200;
+  c->pressure_tolerance = 1e-6;
+  c->pressure_max_iterations = 400;
+  c->seed = 1;
+  c->duration = seconds;
+  c->preconditioner = PL3D_PRECOND_MIC0;
+  c->transfer = PL3D_TRANSFER_FLIP;
+}
+
+void or3d_rng(uint32_t seed, int32_t count, double *out) {
+  Rng r(seed);
+  for (int i = 0; i < count; i++) out[i] = r();
+}
+
+double or3d_teapot_sdf(const pl3d_teapot_config *cfg, double x, double y, double z) {
+  return teapot_sdf(build_teapot(*cfg), x, y, z);
+}
+int32_t or3d_in_cavity(const pl3d_teapot_config *cfg, double x, double y, double z) {
+  return in_cavity(build_teapot(*cfg), x, y, z) ? 1 : 0;
+}
+void or3d_build_teapot(const pl3d_teapot_config *cfg, double out[6]) {
+  Teapot t = build_teapot(*cfg);
+  out[0] = t.ax; out[1] = t.ay; out[2] = t.dx; out[3] = t.dy; out[4] = t.tipX; out[5] = t.tipY;
+}
+
+or3d_controller *or3d_controller_create(const pl3d_motion_config *cfg) {
+  return new or3d_controller(*cfg);
+}
+void or3d_controller_destroy(or3d_controller *c) { delete c; }
+double or3d_controller_advance(or3d_controller *c, double t, double dt, double discharged) {
+  return c->c.advance(t, dt, discharged);
+}
+int32_t or3d_controller_phase(or3d_controller *c) { return c->c.phase; }
+double or3d_controller_angle_deg(or3d_controller *c) { return c->c.angle_deg(); }
+void or3d_controller_set_target(or3d_controller *c, double angle_deg, double time) {
+  c->c.set_target(angle_deg, time);
+}
+
+or3d_sim *or3d_create(const pl3d_config *cfg) { return new or3d_sim(*cfg); }
+void or3d_destroy(or3d_sim *s) { delete s; }
+
+void or3d_info(or3d_sim *s, pl3d_info *o) {
+  const Sim &m = s->sim;
+  std::memset(o, 0, sizeof(*o));
+  o->n = m.n;
+  o->nx = m.nx; o->ny = m.ny; o->nz = m.nz;
+  o->h = m.h;
+  o->fill_height = m.fillHeight;
+  o->pivot_world[0] = m.pivotWorld[0];
+  o->pivot_world[1] = m.pivotWorld[1];
+  o->parcel_volume = m.parcelVolume;
+  for (int k = 0; k < 3; k++) o->band_counts[k] = m.bandCounts[k];
+  o->max_liquid_cells = m.nx * m.ny * m.nz;
+}
+
+void or3d_step(or3d_sim *s, double dt) { s->sim.step(dt); }
+void or3d_advance(or3d_sim *s, double seconds, pl3d_diagnostics *out) {
+  s->sim.advance(seconds);
+  if (out) s->sim.diagnostics(out);
+}
+double or3d_stable_dt(or3d_sim *s) { return s->sim.stable_dt(); }
+int32_t or3d_finished(or3d_sim *s) { return s->sim.finished() ? 1 : 0; }
+void or3d_diagnostics(or3d_sim *s, pl3d_diagnostics *out) { s->sim.diagnostics(out); }
+double or3d_phi(or3d_sim *s) { return s->sim.phi; }
+
+int32_t or3d_discharged_fractions(or3d_sim *s, double out[3]) {  // :817-823
+  const Sim &m = s->sim;
+  size_t cnt = m.dischargeOrder.size();
+  if (cnt == 0) return 0;
+  int c[3] = {0, 0, 0};
+  for (int p : m.dischargeOrder) c[m.origin[p]]++;
+  for (int k = 0; k < 3; k++) out[k] = double(c[k]) / double(cnt);
+  return 1;
+}
+
+void or3d_read_state(or3d_sim *s, double *x, double *y, double *z, double *vx, double *vy,
+                     double *vz, uint8_t *status, uint8_t *origin, float *y0) {
+  const Sim &m = s->sim;
+  size_t n = size_t(m.n);
+  if (x) std::copy(m.x.begin(), m.x.end(), x);
+  if (y) std::copy(m.y.begin(), m.y.end(), y);
+  if (z) std::copy(m.z.begin(), m.z.end(), z);
+  if (vx) std::copy(m.vx.begin(), m.vx.end(), vx);
+  if (vy) std::copy(m.vy.begin(), m.vy.end(), vy);
+  if (vz) std::copy(m.vz.begin(), m.vz.end(), vz);
+  if (status) std::copy(m.status.begin(), m.status.end(), status);
+  if (origin) std::copy(m.origin.begin(), m.origin.end(), origin);
+  if (y0) std::copy(m.y0.begin(), m.y0.end(), y0);
+  (void)n;
+}
+
+void or3d_write_state(or3d_sim *s, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz,
+                      const uint8_t *status) {
+  Sim &m = s->sim;
+  size_t n = size_t(m.n);
+  if (x) std::copy(x, x + n, m.x.begin());
+  if (y) std::copy(y, y + n, m.y.begin());
+  if (z) std::copy(z, z + n, m.z.begin());
+  if (vx) std::copy(vx, vx + n, m.vx.begin());
+  if (vy) std::copy(vy, vy + n, m.vy.begin());
+  if (vz) std::copy(vz, vz + n, m.vz.begin());
+  if (status) {
+    std::copy(status, status + n, m.status.begin());
+    for (int k = 0; k < 4; k++) m.counts[k] = 0;
+    for (size_t p = 0; p < n; p++) m.counts[m.status[p]]++;
+  }
+}
+
+int32_t or3d_discharge_order(or3d_sim *s, int32_t *out, int32_t cap) {
+  const Sim &m = s->sim;
+  int32_t cnt = int32_t(m.dischargeOrder.size());
+  for (int32_t i = 0; i < cnt && i < cap; i++) out[i] = m.dischargeOrder[i];
+  return cnt;
+}
+
+int64_t or3d_grid_field(or3d_sim *s, const char *name, double *out) {
+  Sim &m = s->sim;
+  std::string nm(name);
+  auto put = [&](const std::vector<double> &f) {
+    if (out) std::copy(f.begin(), f.end(), out);
+    return int64_t(f.size());
+  };
+  auto put8 = [&](const std::vector<uint8_t> &f) {
+    if (out)
+      for (size_t i = 0; i < f.size(); i++) out[i] = f[i];
+    return int64_t(f.size());
+  };
+  if (nm == "cell") return put8(m.cell);
+  if (nm == "u") return put(m.u);
+  if (nm == "v") return put(m.v);
+  if (nm == "w") return put(m.w);
+  if (nm == "u_p2g") return put(m.uPrev);
+  if (nm == "v_p2g") return put(m.vPrev);
+  if (nm == "w_p2g") return put(m.wPrev);
+  if (nm == "u_proj") return put(m.uProj);
+  if (nm == "v_proj") return put(m.vProj);
+  if (nm == "w_proj") return put(m.wProj);
+  if (nm == "u_valid") return put8(m.uValid);
+  if (nm == "v_valid") return put8(m.vValid);
+  if (nm == "w_valid") return put8(m.wValid);
+  if (nm == "rhs_p") return put(m.rhsP);
+  if (nm == "q") return put(m.q);
+  if (nm == "density") return put(m.density);
+  if (nm == "rhs_d") return put(m.rhsD);
+  if (nm == "qc") return put(m.qc);
+  if (nm == "uc") return put(m.uc);
+  if (nm == "vc") return put(m.vc);
+  if (nm == "wc") return put(m.wc);
+  return -1;
+}
+
+void or3d_last_solves(or3d_sim *s, int32_t out[4]) {
+  for (int k = 0; k < 4; k++) out[k] = s->sim.lastSolves[k];
+}
+
+void or3d_apply_laplacian(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *x,
+                          double *y) {
+  size_t nc = size_t(nx) * ny * nz;
+  size_t nxy = size_t(nx) * ny;
+  for (size_t c = 0; c < nc; c++) {
+    y[c] = 0;
+    if (cell[c] != FLUID) continue;
+    const ptrdiff_t offs[6] = {-1, 1, -ptrdiff_t(nx), ptrdiff_t(nx), -ptrdiff_t(nxy),
+                               ptrdiff_t(nxy)};
+    double acc = 0;
+    int d = 0;
+    for (ptrdiff_t o : offs) {
+      uint8_t f = cell[c + o];
+      if (f != SOLID) d++;
+      if (f == FLUID) acc -= x[c + o];
+    }
+    y[c] = d * x[c] + acc;
+  }
+}
+
+void or3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *b,
+                    double *q, double tol, int32_t max_iter, pl3d_solve_result *out) {
+  Pcg pcg(nx, ny, nz);
+  size_t nc = size_t(nx) * ny * nz;
+  SolveResult r = pcg.solve(cell, nc, b, q, tol, max_iter);
+  std::vector<double> aq(nc);
+  or3d_apply_laplacian(nx, ny, nz, cell, q, aq.data());
+  double tr = 0;
+  int m = 0;
+  for (size_t c = 0; c < nc; c++)
+    if (cell[c] == FLUID) {
+      m++;
+      tr = std::max(tr, std::fabs(b[c] - aq[c]));
+    }
+  if (out) {
+    out->iterations = r.iterations;
+    out->converged = r.converged ? 1 : 0;
+    out->residual = r.residual;
+    out->true_residual = tr;
+    out->liquid_cells = m;
+    out->pad = 0;
+  }
+}
+
+}  // extern "C"

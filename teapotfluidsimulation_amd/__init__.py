@@ -1,0 +1,17 @@
+"""MI355X (gfx950) implementation of PourLab's 3D PIC/FLIP free-surface pour.
+
+The work runs in libpourlab3d.so (hand-written HIP kernels behind the C ABI of
+include/pourlab3d.h); this package is the host-side mirror of the reference's solver API.
+"""
+from .sim3d import (  # noqa: F401
+    AIR,
+    DISCHARGED,
+    FLUID,
+    IN_POT,
+    LOST,
+    SOLID,
+    SPILLED,
+    Pcg3D,
+    Simulation3D,
+    gentlePour3D,
+)

@@ -1,0 +1,71 @@
+"""Loader of the in-tree HIP library libpourlab3d.so (C ABI of include/pourlab3d.h).
+
+There is no CPU fallback: if the library is missing or cannot be loaded this module raises.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpourlab3d.so")
+
+# every function declared in include/pourlab3d.h: name -> (restype, argtypes)
+_P = C.POINTER
+_vp, _i32, _d = C.c_void_p, C.c_int32, C.c_double
+SIGNATURES = {
+    "pl3d_last_error": (C.c_char_p, []),
+    "pl3d_phase_name": (C.c_char_p, [_i32]),
+    "pl3d_version": (C.c_char_p, []),
+    "pl3d_gentle_pour": (C.c_int, [_d, _d, _P(abi.Config)]),
+    "pl3d_create": (C.c_int, [_P(abi.Config), _P(_vp)]),
+    "pl3d_destroy": (C.c_int, [_vp]),
+    "pl3d_get_info": (C.c_int, [_vp, _P(abi.Info)]),
+    "pl3d_step": (C.c_int, [_vp, _d]),
+    "pl3d_advance": (C.c_int, [_vp, _d, _P(abi.Diagnostics)]),
+    "pl3d_stable_dt": (C.c_int, [_vp, _P(_d)]),
+    "pl3d_finished": (C.c_int, [_vp, _P(_i32)]),
+    "pl3d_get_diagnostics": (C.c_int, [_vp, _P(abi.Diagnostics)]),
+    "pl3d_discharged_fractions": (C.c_int, [_vp, _P(_d), _P(_i32)]),
+    "pl3d_stats": (C.c_int, [_vp, _P(_i32)]),
+    "pl3d_read_state": (C.c_int, [_vp] + [_vp] * 9),
+    "pl3d_discharge_order": (C.c_int, [_vp, _P(_i32), _i32, _P(_i32)]),
+    "pl3d_read_particles": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pl3d_read_render_positions": (C.c_int, [_vp, _vp]),
+    "pl3d_set_target": (C.c_int, [_vp, _d]),
+    "pl3d_pcg_solve": (C.c_int, [_i32, _i32, _i32, _vp, _vp, _vp, _d, _i32, _i32,
+                                 _P(abi.SolveResult)]),
+    "pl3d_set_timing": (C.c_int, [_vp, _i32]),
+    "pl3d_kernel_times": (C.c_int, [_vp, _i32, _P(C.c_char_p), _P(_d), _P(C.c_int64),
+                                    _P(_i32)]),
+}
+
+_lib = None
+
+
+class Pl3dError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"pourlab3d error {code}: {message}")
+        self.code = code
+
+
+def load():
+    """Load libpourlab3d.so (build it first with teapotfluidsimulation_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m teapotfluidsimulation_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != abi.OK:
+        msg = load().pl3d_last_error()
+        raise Pl3dError(rc, msg.decode() if msg else "")

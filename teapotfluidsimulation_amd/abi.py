@@ -1,0 +1,246 @@
+"""ctypes mirror of the records declared in include/pourlab3d.h.
+
+The field order and types must match the C header exactly; tests/test_abi.py checks the sizes and
+offsets against a probe compiled from the header.
+"""
+import ctypes as C
+
+OK = 0
+EINVAL = -1
+EDEVICE = -2
+ECAPACITY = -3
+ESTATE = -4
+
+AIR, FLUID, SOLID = 0, 1, 2
+IN_POT, DISCHARGED, SPILLED, LOST = 0, 1, 2, 3
+
+MOTION_KINDS = {"still": 0, "pour": 1, "tilt": 2, "manual": 3}
+PHASES = ["wait", "tilt-in", "hold", "return", "rest", "done", "manual"]
+PRECONDITIONERS = {"mic0": 0, "mg": 1, "jacobi": 2}
+TRANSFERS = {"flip": 0, "apic": 1}
+
+STAT = {
+    "inPot": 0,
+    "discharged": 1,
+    "spilled": 2,
+    "lost": 3,
+    "originBottom": 4,
+    "originMiddle": 5,
+    "originTop": 6,
+    "pressureFailures": 7,
+    "maxSpeedSq": 8,
+    "maxDivergence": 9,
+    "interiorCount": 10,
+    "interiorSum": 11,
+    "pressureIterations": 12,
+    "correctionIterations": 13,
+}
+STAT_SLOTS = 16
+SCALE_RATIO = 1 << 16
+
+
+class TeapotConfig(C.Structure):
+    _fields_ = [
+        ("radius", C.c_double),
+        ("height", C.c_double),
+        ("wall", C.c_double),
+        ("inlet_height", C.c_double),
+        ("bore_radius", C.c_double),
+        ("spout_angle_deg", C.c_double),
+        ("spout_length", C.c_double),
+    ]
+
+
+class MotionConfig(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("repeats", C.c_int32),
+        ("start_delay", C.c_double),
+        ("tilt_duration", C.c_double),
+        ("hold_angle_deg", C.c_double),
+        ("hold_tilt_rate", C.c_double),
+        ("max_angle_deg", C.c_double),
+        ("return_duration", C.c_double),
+        ("max_hold", C.c_double),
+        ("hold_time", C.c_double),
+        ("target_volume", C.c_double),
+        ("anticipation", C.c_double),
+        ("repeat_gap", C.c_double),
+        ("settle_time", C.c_double),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("teapot", TeapotConfig),
+        ("motion", MotionConfig),
+        ("h", C.c_double),
+        ("initial_volume", C.c_double),
+        ("pivot", C.c_double * 2),
+        ("gravity", C.c_double),
+        ("flip_ratio", C.c_double),
+        ("density_correction", C.c_double),
+        ("cfl", C.c_double),
+        ("max_substep", C.c_double),
+        ("pressure_tolerance", C.c_double),
+        ("pressure_max_iterations", C.c_int32),
+        ("seed", C.c_uint32),
+        ("duration", C.c_double),
+        ("preconditioner", C.c_int32),
+        ("transfer", C.c_int32),
+    ]
+
+
+class Diagnostics(C.Structure):
+    _fields_ = [
+        ("time", C.c_double),
+        ("steps", C.c_int64),
+        ("angle_deg", C.c_double),
+        ("phase", C.c_int32),
+        ("particles", C.c_int32),
+        ("grid", C.c_int32 * 3),
+        ("pressure_iterations", C.c_int32),
+        ("discharged_ml", C.c_double),
+        ("spilled_ml", C.c_double),
+        ("lost_ml", C.c_double),
+        ("in_pot_ml", C.c_double),
+        ("pressure_failures", C.c_int32),
+        ("correction_iterations", C.c_int32),
+        ("max_divergence", C.c_double),
+        ("interior_density", C.c_double),
+    ]
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("h", C.c_double),
+        ("fill_height", C.c_double),
+        ("pivot_world", C.c_double * 2),
+        ("parcel_volume", C.c_double),
+        ("band_counts", C.c_int32 * 3),
+        ("max_liquid_cells", C.c_int32),
+    ]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("residual", C.c_double),
+        ("true_residual", C.c_double),
+        ("liquid_cells", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+def diagnostics_dict(d: Diagnostics) -> dict:
+    """Diagnostics3D as the reference returns it (solver3d.ts:796-814), camelCase keys."""
+    return {
+        "time": d.time,
+        "steps": int(d.steps),
+        "angleDeg": d.angle_deg,
+        "phase": PHASES[d.phase] if 0 <= d.phase < len(PHASES) else str(d.phase),
+        "particles": d.particles,
+        "grid": [d.grid[0], d.grid[1], d.grid[2]],
+        "dischargedMl": d.discharged_ml,
+        "spilledMl": d.spilled_ml,
+        "lostMl": d.lost_ml,
+        "inPotMl": d.in_pot_ml,
+        "pressureIterations": d.pressure_iterations,
+        "pressureFailures": d.pressure_failures,
+        "maxDivergence": d.max_divergence,
+        "interiorDensity": d.interior_density,
+    }
+
+
+def config_to_dict(c: Config) -> dict:
+    """Config3D (solver3d.ts:25-43) as a nested camelCase dict."""
+    t, m = c.teapot, c.motion
+    kinds = {v: k for k, v in MOTION_KINDS.items()}
+    return {
+        "teapot": {
+            "radius": t.radius,
+            "height": t.height,
+            "wall": t.wall,
+            "inletHeight": t.inlet_height,
+            "boreRadius": t.bore_radius,
+            "spoutAngleDeg": t.spout_angle_deg,
+            "spoutLength": t.spout_length,
+        },
+        "h": c.h,
+        "initialVolume": c.initial_volume,
+        "motion": {
+            "kind": kinds[m.kind],
+            "startDelay": m.start_delay,
+            "tiltDuration": m.tilt_duration,
+            "holdAngleDeg": m.hold_angle_deg,
+            "holdTiltRate": m.hold_tilt_rate,
+            "maxAngleDeg": m.max_angle_deg,
+            "returnDuration": m.return_duration,
+            "maxHold": m.max_hold,
+            "holdTime": m.hold_time,
+            "targetVolume": m.target_volume,
+            "anticipation": m.anticipation,
+            "repeats": m.repeats,
+            "repeatGap": m.repeat_gap,
+            "settleTime": m.settle_time,
+        },
+        "pivot": [c.pivot[0], c.pivot[1]],
+        "gravity": c.gravity,
+        "flipRatio": c.flip_ratio,
+        "densityCorrection": c.density_correction,
+        "cfl": c.cfl,
+        "maxSubstep": c.max_substep,
+        "pressureTolerance": c.pressure_tolerance,
+        "pressureMaxIterations": c.pressure_max_iterations,
+        "seed": c.seed,
+        "duration": c.duration,
+        "preconditioner": {v: k for k, v in PRECONDITIONERS.items()}.get(c.preconditioner, "mic0"),
+        "transfer": {v: k for k, v in TRANSFERS.items()}.get(c.transfer, "flip"),
+    }
+
+
+def config_from_dict(d: dict) -> Config:
+    """Inverse of config_to_dict; missing device-only keys take their defaults."""
+    c = Config()
+    t, m = d["teapot"], d["motion"]
+    c.teapot.radius = t["radius"]
+    c.teapot.height = t["height"]
+    c.teapot.wall = t["wall"]
+    c.teapot.inlet_height = t["inletHeight"]
+    c.teapot.bore_radius = t["boreRadius"]
+    c.teapot.spout_angle_deg = t["spoutAngleDeg"]
+    c.teapot.spout_length = t["spoutLength"]
+    c.h = d["h"]
+    c.initial_volume = d["initialVolume"]
+    c.motion.kind = MOTION_KINDS[m["kind"]]
+    c.motion.start_delay = m["startDelay"]
+    c.motion.tilt_duration = m["tiltDuration"]
+    c.motion.hold_angle_deg = m["holdAngleDeg"]
+    c.motion.hold_tilt_rate = m["holdTiltRate"]
+    c.motion.max_angle_deg = m["maxAngleDeg"]
+    c.motion.return_duration = m["returnDuration"]
+    c.motion.max_hold = m["maxHold"]
+    c.motion.hold_time = m["holdTime"]
+    c.motion.target_volume = m["targetVolume"]
+    c.motion.anticipation = m["anticipation"]
+    c.motion.repeats = int(m["repeats"])
+    c.motion.repeat_gap = m["repeatGap"]
+    c.motion.settle_time = m["settleTime"]
+    c.pivot[0], c.pivot[1] = d["pivot"]
+    c.gravity = d["gravity"]
+    c.flip_ratio = d["flipRatio"]
+    c.density_correction = d["densityCorrection"]
+    c.cfl = d["cfl"]
+    c.max_substep = d["maxSubstep"]
+    c.pressure_tolerance = d["pressureTolerance"]
+    c.pressure_max_iterations = int(d["pressureMaxIterations"])
+    c.seed = int(d["seed"])
+    c.duration = d["duration"]
+    c.preconditioner = PRECONDITIONERS[d.get("preconditioner", "mic0")]
+    c.transfer = TRANSFERS[d.get("transfer", "flip")]
+    return c

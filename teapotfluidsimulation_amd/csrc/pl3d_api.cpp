@@ -1,0 +1,866 @@
+// pl3d_api.cpp — host side of libpourlab3d: the C ABI of include/pourlab3d.h.
+//
+// The constructor work of Simulation3D (solver3d.ts:116-228: domain sweep over the tilt range,
+// grid sizing, mulberry32-jittered seeding with rejection, equal-count origin bands) runs once on
+// the host in FP64; everything per substep runs on the device (pl3d_step.hip, pl3d_solver.hip).
+// advance() replays a captured one-substep HIP graph in batches and reads back one small control
+// block per batch: the substep length, the pour controller and the end test all live on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/pourlab3d.h"
+#include "pl3d_step.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_OK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(PL3D_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+constexpr double PI = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- host geometry (FP64)
+struct Pot {
+  pl3d_teapot_config c;
+  double ax, ay, dx, dy, tipX, tipY;
+};
+
+Pot make_pot(const pl3d_teapot_config &c) {  // buildTeapot3D, vessel3d.ts:46-59
+  Pot t;
+  t.c = c;
+  const double a = (c.spout_angle_deg * PI) / 180;
+  t.ax = c.radius - c.wall;
+  t.ay = c.inlet_height;
+  t.dx = std::cos(a);
+  t.dy = std::sin(a);
+  t.tipX = t.ax + std::cos(a) * c.spout_length;
+  t.tipY = t.ay + std::sin(a) * c.spout_length;
+  return t;
+}
+
+void axis(const Pot &t, double x, double y, double z, double &dist, double &s) {
+  const double px = x - t.ax, py = y - t.ay;
+  s = px * t.dx + py * t.dy;
+  const double L = t.c.spout_length;
+  const double sc = s < 0 ? 0 : (s > L ? L : s);
+  const double qx = px - sc * t.dx, qy = py - sc * t.dy;
+  dist = std::sqrt(qx * qx + qy * qy + z * z);
+}
+
+double sdf(const Pot &t, double x, double y, double z) {  // teapotSdf, vessel3d.ts:74-95
+  const pl3d_teapot_config &c = t.c;
+  const double hw = c.wall / 2;
+  const double r = std::sqrt(x * x + z * z);
+  const double ring = std::fabs(r - (c.radius + hw)) - hw;
+  const double side = std::max(ring, std::max(-(y + c.wall), y - c.height));
+  const double fl = std::max(std::fabs(y + hw) - hw, r - (c.radius + c.wall));
+  double solid = std::min(side, fl);
+  double dist, s;
+  axis(t, x, y, z, dist, s);
+  const double L = c.spout_length;
+  const double tube = std::max(std::fabs(dist - (c.bore_radius + hw)) - hw, std::max(-s, s - L));
+  solid = std::min(solid, std::max(tube, c.radius - r));
+  const double bore = std::max(std::max(dist - c.bore_radius, -s - c.wall), s - L);
+  return std::max(solid, -bore);
+}
+
+bool cavity(const Pot &t, double x, double y, double z) {  // inCavity, vessel3d.ts:98-103
+  const pl3d_teapot_config &c = t.c;
+  if (y >= 0 && y <= c.height && x * x + z * z <= c.radius * c.radius) return true;
+  double dist, s;
+  axis(t, x, y, z, dist, s);
+  return s >= 0 && s <= c.spout_length && dist <= c.bore_radius;
+}
+
+struct Mulberry32 {  // rng.ts:5-14
+  uint32_t a;
+  double next() {
+    a += 0x6d2b79f5u;
+    uint32_t t = a;
+    t = (t ^ (t >> 15)) * (t | 1u);
+    t ^= t + (t ^ (t >> 7)) * (t | 61u);
+    return double(t ^ (t >> 14)) / 4294967296.0;
+  }
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct KernelTimer {
+  const char *name;
+  double ms = 0;
+  int64_t launches = 0;
+};
+
+}  // namespace
+
+int pl3d_set_error(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+struct pl3d_sim {
+  pl3d_config cfg{};
+  DevParams P{};
+  pl3d_info info{};
+  Pot pot{};
+  hipStream_t stream = nullptr;
+  void *mem = nullptr;
+  size_t memBytes = 0;
+  DevBufs B{};
+  DevCtl *hctl = nullptr;  // pinned mirror of the device control block
+  hipGraphExec_t graph = nullptr;
+  bool timing = false;
+  std::vector<KernelTimer> timers;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float *render = nullptr;    // 6 n floats, allocated on the first readRenderPositions
+  std::vector<float> y0;      // Float32 local y at seeding (solver3d.ts:83, :218)
+  bool destroyed = false;
+  double lastDt = 0.005;
+  int gridN = 1, gridC = 1, gridF = 1, gridFC = 1, gridExtrap = 1, gridScan = 1;
+};
+
+namespace {
+
+int sync_ctl(pl3d_sim *s) {
+  HIP_OK(hipMemcpyAsync(s->hctl, s->B.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return PL3D_OK;
+}
+
+// the command fields (mode, pending, endTime, dtExplicit) are contiguous in DevCtl
+int push_command(pl3d_sim *s, int mode, int pending, double endTime, double dt) {
+  s->hctl->mode = mode;
+  s->hctl->pending = pending;
+  s->hctl->endTime = endTime;
+  s->hctl->dtExplicit = dt;
+  const size_t off = offsetof(DevCtl, mode);
+  const size_t len = offsetof(DevCtl, active) - off;
+  HIP_OK(hipMemcpyAsync(reinterpret_cast<char *>(s->B.ctl) + off,
+                        reinterpret_cast<char *>(s->hctl) + off, len, hipMemcpyHostToDevice,
+                        s->stream));
+  return PL3D_OK;
+}
+
+template <class K>
+int launch(pl3d_sim *s, int idx, K kernel, int grid, int block) {
+  if (s->timing) HIP_OK(hipEventRecord(s->ev0, s->stream));
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, s->B, s->P);
+  HIP_OK(hipGetLastError());
+  if (s->timing) {
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    HIP_OK(hipEventSynchronize(s->ev1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->timers[idx].ms += ms;
+    s->timers[idx].launches++;
+  }
+  return PL3D_OK;
+}
+
+// One substep's launches, in order (Simulation3D.step, solver3d.ts:288-309).
+int enqueue_substep(pl3d_sim *s) {
+  int rc;
+  if ((rc = launch(s, 0, k_begin, 1, 256))) return rc;
+  if ((rc = launch(s, 1, k_count, s->gridN, STEP_THREADS))) return rc;
+  if ((rc = launch(s, 2, k_classify_scan, s->gridScan, STEP_THREADS))) return rc;
+  if ((rc = launch(s, 3, k_scatter, s->gridN, STEP_THREADS))) return rc;
+  if ((rc = launch(s, 4, k_p2g, s->gridFC, STEP_THREADS))) return rc;
+  if ((rc = launch(s, 5, k_liquid, s->gridC, STEP_THREADS))) return rc;
+  {
+    PcgShared A;
+    A.m = &s->B.ctl->m;
+    A.nbr = s->B.nbr;
+    A.diag = s->B.diag;
+    A.mic = s->B.mic;
+    A.plane = s->B.plane;
+    A.stride = s->P.nc;
+    A.planes = s->P.nx + s->P.ny + s->P.nz - 2;
+    A.precond = s->cfg.preconditioner == PL3D_PRECOND_JACOBI ? 2 : 0;
+    A.overflow = &s->B.ctl->overflow;
+    if (s->timing) HIP_OK(hipEventRecord(s->ev0, s->stream));
+    hipLaunchKernelGGL(k_pcg, dim3(2), dim3(PL3D_SOLVE_THREADS), 0, s->stream, A, s->B.descs,
+                       &s->B.ctl->active);
+    HIP_OK(hipGetLastError());
+    if (s->timing) {
+      HIP_OK(hipEventRecord(s->ev1, s->stream));
+      HIP_OK(hipEventSynchronize(s->ev1));
+      float ms = 0;
+      HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      s->timers[6].ms += ms;
+      s->timers[6].launches++;
+    }
+  }
+  if ((rc = launch(s, 7, k_faces_post, s->gridF, STEP_THREADS))) return rc;
+  if ((rc = launch(s, 8, k_extrap, s->gridExtrap, STEP_THREADS))) return rc;
+  if ((rc = launch(s, 9, k_advect, s->gridN, STEP_THREADS))) return rc;
+  return PL3D_OK;
+}
+
+int build_graph(pl3d_sim *s) {
+  if (s->graph) return PL3D_OK;
+  hipGraph_t g;
+  HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_substep(s);
+  hipGraph_t gg = nullptr;
+  hipError_t e = hipStreamEndCapture(s->stream, &gg);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(PL3D_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+  g = gg;
+  HIP_OK(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(g));
+  return PL3D_OK;
+}
+
+int run_substeps(pl3d_sim *s, int count) {
+  if (s->timing) {
+    for (int i = 0; i < count; i++) {
+      int rc = enqueue_substep(s);
+      if (rc) return rc;
+    }
+    return PL3D_OK;
+  }
+  int rc = build_graph(s);
+  if (rc) return rc;
+  for (int i = 0; i < count; i++) HIP_OK(hipGraphLaunch(s->graph, s->stream));
+  return PL3D_OK;
+}
+
+void fill_gentle(double cell_mm, double seconds, pl3d_config *c) {
+  std::memset(c, 0, sizeof(*c));
+  // defaultTeapot3D (vessel3d.ts:22-32)
+  c->teapot.radius = 0.065;
+  c->teapot.height = 0.15;
+  c->teapot.wall = 0.008;
+  c->teapot.inlet_height = 0.035;
+  c->teapot.bore_radius = 0.01;
+  c->teapot.spout_angle_deg = 45;
+  c->teapot.spout_length = 0.14;
+  // defaultMotion (config.ts:168-186) with maxHold 20 (presets3d.ts:17)
+  pl3d_motion_config &m = c->motion;
+  m.kind = PL3D_MOTION_POUR;
+  m.start_delay = 0.5;
+  m.tilt_duration = 2;
+  m.hold_angle_deg = 22;
+  m.hold_tilt_rate = 3;
+  m.max_angle_deg = 80;
+  m.return_duration = 1.2;
+  m.max_hold = 20;
+  m.hold_time = 1.5;
+  m.target_volume = 400 * 1e-6;
+  m.anticipation = 0.254;
+  m.repeats = 1;
+  m.repeat_gap = 1.5;
+  m.settle_time = 2;
+  // gentlePour3D (presets3d.ts:12-26)
+  c->h = cell_mm / 1000;
+  c->initial_volume = 1500 * 1e-6;
+  c->pivot[0] = 0;
+  c->pivot[1] = 0.06;
+  c->gravity = 9.81;
+  c->flip_ratio = 0.95;
+  c->density_correction = 0.3;
+  c->cfl = 0.8;
+  c->max_substep = 1.0 / 200;
+  c->pressure_tolerance = 1e-6;
+  c->pressure_max_iterations = 400;
+  c->seed = 1;
+  c->duration = seconds;
+  c->preconditioner = PL3D_PRECOND_MIC0;
+  c->transfer = PL3D_TRANSFER_FLIP;
+}
+
+int check_sim(pl3d_sim *s) {
+  if (!s) return fail(PL3D_EINVAL, "null simulation handle");
+  if (s->destroyed) return fail(PL3D_ESTATE, "simulation was destroyed");
+  return PL3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pl3d_last_error(void) { return g_err.c_str(); }
+
+const char *pl3d_phase_name(int32_t phase) {
+  static const char *names[] = {"wait", "tilt-in", "hold", "return", "rest", "done", "manual"};
+  return phase >= 0 && phase < 7 ? names[phase] : "unknown";
+}
+
+const char *pl3d_version(void) { return "pourlab3d 0.1 gfx950"; }
+
+int pl3d_gentle_pour(double cell_mm, double seconds, pl3d_config *out) {
+  if (!out || !(cell_mm > 0) || !(seconds >= 0)) return fail(PL3D_EINVAL, "bad gentle pour args");
+  fill_gentle(cell_mm, seconds, out);
+  return PL3D_OK;
+}
+
+int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
+  g_err.clear();
+  if (!cfg || !out) return fail(PL3D_EINVAL, "null argument");
+  *out = nullptr;
+  const pl3d_config &c = *cfg;
+  if (!(c.h > 0) || !(c.initial_volume > 0) || !(c.cfl > 0) || !(c.max_substep > 0) ||
+      c.pressure_max_iterations < 0 || !(c.teapot.radius > 0))
+    return fail(PL3D_EINVAL, "invalid Config3D");
+  if (c.preconditioner != PL3D_PRECOND_MIC0 && c.preconditioner != PL3D_PRECOND_JACOBI)
+    return fail(PL3D_EINVAL, "preconditioner %d is not available", c.preconditioner);
+  if (c.transfer != PL3D_TRANSFER_FLIP)
+    return fail(PL3D_EINVAL, "transfer %d is not available", c.transfer);
+  auto *s = new pl3d_sim();
+  s->cfg = c;
+  s->pot = make_pot(c.teapot);
+  const double h = c.h;
+  const pl3d_teapot_config &t = c.teapot;
+  const double fillHeight = c.initial_volume / (PI * t.radius * t.radius);
+  // domain: bounding box of the pot over the tilt range (solver3d.ts:124-157)
+  const double pts[6][2] = {{-t.radius - t.wall, -t.wall},
+                            {-t.radius - t.wall, t.height + 0.02},
+                            {t.radius + t.wall, -t.wall},
+                            {t.radius + t.wall, t.height + 0.02},
+                            {s->pot.tipX + 0.02, s->pot.tipY + 0.02},
+                            {s->pot.tipX + 0.02, s->pot.tipY - 0.03}};
+  double X0 = INFINITY, X1 = -INFINITY, Y0 = INFINITY, Y1 = -INFINITY;
+  const double maxA = std::max(c.motion.max_angle_deg, c.motion.hold_angle_deg);
+  for (int a = 0; a <= maxA; a += 2) {
+    const double ph = (-a * PI) / 180;
+    const double cc = std::cos(ph), ss = std::sin(ph);
+    for (const auto &p : pts) {
+      const double rx = p[0] - c.pivot[0], ry = p[1] - c.pivot[1];
+      const double wx = cc * rx - ss * ry, wy = ss * rx + cc * ry;
+      X0 = std::min(X0, wx);
+      X1 = std::max(X1, wx);
+      Y0 = std::min(Y0, wy);
+      Y1 = std::max(Y1, wy);
+    }
+  }
+  const double mg = 3 * h;
+  const double pwx = -X0 + mg, pwy = -Y0 + mg;
+  const int nx = int(std::ceil((X1 - X0 + 2 * mg) / h));
+  const int ny = int(std::ceil((Y1 - Y0 + 2 * mg) / h));
+  const int nz = int(std::ceil((t.radius + t.wall + 2 * h) / h)) + 1;
+  if (nx < 3 || ny < 3 || nz < 3 || double(nx) * ny * nz > 2.0e8) {
+    delete s;
+    return fail(PL3D_EINVAL, "grid %dx%dx%d out of range", nx, ny, nz);
+  }
+  if (nx + ny + nz - 2 > PL3D_MAX_PLANES) {
+    delete s;
+    return fail(PL3D_ECAPACITY, "grid too large for the hyperplane schedule");
+  }
+  // seeding (solver3d.ts:185-200), pose phi = 0
+  Mulberry32 rng{c.seed};
+  const double sub = h / 2;
+  std::vector<double> xs;
+  for (int k = 0; k < nz * 2; k++)
+    for (int j = 0; j < ny * 2; j++)
+      for (int i = 0; i < nx * 2; i++) {
+        const double wx = (i + 0.5 + (rng.next() - 0.5) * 0.5) * sub;
+        const double wy = (j + 0.5 + (rng.next() - 0.5) * 0.5) * sub;
+        const double wz = (k + 0.5 + (rng.next() - 0.5) * 0.5) * sub;
+        const double rx = wx - pwx, ry = wy - pwy;
+        const double lx = 1.0 * rx + 0.0 * ry + c.pivot[0];
+        const double ly = -0.0 * rx + 1.0 * ry + c.pivot[1];
+        const double lz = wz - h;
+        if (lz < 0 || ly < 0 || ly > fillHeight) continue;
+        if (!cavity(s->pot, lx, ly, lz)) continue;
+        if (sdf(s->pot, lx, ly, lz) < 0.4 * h) continue;
+        xs.push_back(wx);
+        xs.push_back(wy);
+        xs.push_back(wz);
+      }
+  const int n = int(xs.size() / 3);
+  if (n == 0) {
+    delete s;
+    return fail(PL3D_EINVAL, "no parcels seeded");
+  }
+  std::vector<float> y0(n);
+  for (int p = 0; p < n; p++) y0[p] = float(xs[3 * p + 1] - pwy + c.pivot[1]);
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int a, int b) {  // f32 y0, ties by index
+    return y0[a] < y0[b] || (y0[a] == y0[b] && a < b);
+  });
+  s->y0 = y0;
+  std::vector<uint8_t> origin(n);
+  int bands[3] = {0, 0, 0};
+  for (int r = 0; r < n; r++) {
+    const int k = std::min(2, int(std::floor(double(r) * 3.0 / n)));
+    origin[order[r]] = uint8_t(k);
+    bands[k]++;
+  }
+  // device parameters
+  DevParams &P = s->P;
+  P.nx = nx;
+  P.ny = ny;
+  P.nz = nz;
+  P.n = n;
+  P.nc = nx * ny * nz;
+  P.nu = (nx + 1) * ny * nz;
+  P.nv = nx * (ny + 1) * nz;
+  P.nw = nx * ny * (nz + 1);
+  P.nfaces = P.nu + P.nv + P.nw;
+  P.scanBlocks = (P.nc + SCAN_TILE - 1) / SCAN_TILE;
+  P.h = float(h);
+  P.invh = float(1.0 / h);
+  P.hd = h;
+  P.radius = t.radius;
+  P.height = t.height;
+  P.wall = t.wall;
+  P.inletHeight = t.inlet_height;
+  P.boreRadius = t.bore_radius;
+  P.spoutLength = t.spout_length;
+  P.ax = s->pot.ax;
+  P.ay = s->pot.ay;
+  P.dx = s->pot.dx;
+  P.dy = s->pot.dy;
+  P.tipX = s->pot.tipX;
+  P.tipY = s->pot.tipY;
+  P.pivotX = c.pivot[0];
+  P.pivotY = c.pivot[1];
+  P.pivotWX = pwx;
+  P.pivotWY = pwy;
+  P.parcelVolume = c.initial_volume / (2 * n);
+  P.gravity = c.gravity;
+  P.flipRatio = c.flip_ratio;
+  P.densityCorrection = c.density_correction;
+  P.cfl = c.cfl;
+  P.maxSubstep = c.max_substep;
+  P.duration = c.duration;
+  P.pressureTolerance = c.pressure_tolerance;
+  P.pressureMaxIterations = c.pressure_max_iterations;
+  P.preconditioner = c.preconditioner;
+  {  // Math.hypot(nx, ny) * h
+    const double a = nx, b = ny, mx = std::max(a, b);
+    const double na = a / mx, nb = b / mx;
+    P.Rdiag = std::sqrt(na * na + nb * nb) * mx * h;
+  }
+  P.motionKind = c.motion.kind;
+  P.repeats = c.motion.repeats;
+  P.startDelay = c.motion.start_delay;
+  P.tiltDuration = c.motion.tilt_duration;
+  P.holdAngleDeg = c.motion.hold_angle_deg;
+  P.holdTiltRate = c.motion.hold_tilt_rate;
+  P.maxAngleDeg = c.motion.max_angle_deg;
+  P.returnDuration = c.motion.return_duration;
+  P.maxHold = c.motion.max_hold;
+  P.holdTime = c.motion.hold_time;
+  P.targetVolume = c.motion.target_volume;
+  P.anticipation = c.motion.anticipation;
+  P.repeatGap = c.motion.repeat_gap;
+  // info
+  pl3d_info &I = s->info;
+  I.n = n;
+  I.nx = nx;
+  I.ny = ny;
+  I.nz = nz;
+  I.h = h;
+  I.fill_height = fillHeight;
+  I.pivot_world[0] = pwx;
+  I.pivot_world[1] = pwy;
+  I.parcel_volume = P.parcelVolume;
+  for (int k = 0; k < 3; k++) I.band_counts[k] = bands[k];
+  I.max_liquid_cells = PL3D_SOLVE_MMAX;
+
+  // ---- device memory: one allocation carved into arrays
+  const size_t NC = size_t(P.nc), NF = size_t(P.nfaces), NP = size_t(n);
+  size_t off = 0;
+  struct Slot {
+    void **ptr;
+    size_t bytes;
+  };
+  DevBufs &B = s->B;
+  std::vector<Slot> slots = {
+      {(void **)&B.ctl, sizeof(DevCtl)},
+      {(void **)&B.x, NP * 4}, {(void **)&B.y, NP * 4}, {(void **)&B.z, NP * 4},
+      {(void **)&B.vx, NP * 4}, {(void **)&B.vy, NP * 4}, {(void **)&B.vz, NP * 4},
+      {(void **)&B.status, NP}, {(void **)&B.origin, NP},
+      {(void **)&B.cellOf, NP * 4}, {(void **)&B.slot, NP * 4}, {(void **)&B.dstep, NP * 4},
+      {(void **)&B.cellCount, NC * 4}, {(void **)&B.binStart, (NC + 1) * 4},
+      {(void **)&B.binPos, NP * 16}, {(void **)&B.binVel, NP * 16},
+      {(void **)&B.scanStatus, size_t(P.scanBlocks) * 8},
+      {(void **)&B.flag, NC}, {(void **)&B.lidx, NC * 4}, {(void **)&B.density, NC * 4},
+      {(void **)&B.u, NF * 4}, {(void **)&B.uPrev, NF * 4}, {(void **)&B.uE, NF * 4},
+      {(void **)&B.uc, NF * 4}, {(void **)&B.valid, NF},
+      {(void **)&B.lcell, NC * 4}, {(void **)&B.nbr, NC * 12},
+      {(void **)&B.diag, NC}, {(void **)&B.mic, NC}, {(void **)&B.plane, NC * 2},
+      {(void **)&B.bP, NC * 8}, {(void **)&B.bD, NC * 8}, {(void **)&B.qP, NC * 8},
+      {(void **)&B.qD, NC * 8},
+      {(void **)&B.porder, 2 * NC * 4}, {(void **)&B.pc2, 2 * NC * 4},
+      {(void **)&B.nbrPlane, 2 * 3 * NC * 4}, {(void **)&B.descs, 2 * sizeof(PcgDesc)},
+  };
+  for (auto &sl : slots) off += align256(sl.bytes);
+  s->memBytes = off;
+  auto cleanup = [&](int rc) {
+    if (s->mem) (void)hipFree(s->mem);
+    if (s->hctl) (void)hipHostFree(s->hctl);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return rc;
+  };
+  hipError_t e = hipMalloc(&s->mem, off);
+  if (e != hipSuccess)
+    return cleanup(fail(PL3D_EDEVICE, "hipMalloc(%zu): %s", off, hipGetErrorString(e)));
+  off = 0;
+  for (auto &sl : slots) {
+    *sl.ptr = static_cast<char *>(s->mem) + off;
+    off += align256(sl.bytes);
+  }
+  if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipHostMalloc((void **)&s->hctl, sizeof(DevCtl), hipHostMallocDefault)) != hipSuccess)
+    return cleanup(fail(PL3D_EDEVICE, "stream/pinned alloc: %s", hipGetErrorString(e)));
+  // initial state
+  std::vector<float> fx(NP), fy(NP), fz(NP), zero(NP, 0.f);
+  for (int p = 0; p < n; p++) {
+    fx[p] = float(xs[3 * p]);
+    fy[p] = float(xs[3 * p + 1]);
+    fz[p] = float(xs[3 * p + 2]);
+  }
+  std::vector<int> minus1(NP, -1);
+  std::vector<uint8_t> st0(NP, 0);
+  PcgDesc descs[2];
+  std::memset(descs, 0, sizeof descs);
+  descs[0].b = B.bP;
+  descs[0].q = B.qP;
+  descs[0].tol = float(c.pressure_tolerance);
+  descs[0].maxIter = c.pressure_max_iterations;
+  descs[0].iterations = &B.ctl->pressureIterations;
+  descs[0].converged = &B.ctl->pressureConverged;
+  descs[0].failures = &B.ctl->pressureFailures;
+  descs[0].residual = &B.ctl->pressureResidual;
+  descs[0].trueResidual = &B.ctl->pressureTrueResidual;
+  descs[0].bnorm = &B.ctl->pressureBnorm;
+  descs[0].porder = B.porder;
+  descs[0].pc2 = B.pc2;
+  descs[0].nbrPlane = B.nbrPlane;
+  descs[1].b = B.bD;  // densityCorrection solve: tol 1e-4, cap 80, failures not counted (:673)
+  descs[1].q = B.qD;
+  descs[1].tol = 1e-4f;
+  descs[1].maxIter = 80;
+  descs[1].iterations = &B.ctl->correctionIterations;
+  descs[1].converged = &B.ctl->pad0;
+  descs[1].failures = nullptr;
+  descs[1].porder = B.porder + NC;
+  descs[1].pc2 = B.pc2 + NC;
+  descs[1].nbrPlane = B.nbrPlane + 3 * NC;
+  DevCtl &H = *s->hctl;
+  std::memset(&H, 0, sizeof(DevCtl));
+  H.phase = c.motion.kind == PL3D_MOTION_STILL ? PL3D_PHASE_DONE
+            : c.motion.kind == PL3D_MOTION_MANUAL ? PL3D_PHASE_MANUAL
+                                                   : PL3D_PHASE_WAIT;
+  H.counts[0] = n;
+  H.active = 0;
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream);
+  };
+  if ((e = up(B.x, fx.data(), NP * 4)) || (e = up(B.y, fy.data(), NP * 4)) ||
+      (e = up(B.z, fz.data(), NP * 4)) || (e = up(B.vx, zero.data(), NP * 4)) ||
+      (e = up(B.vy, zero.data(), NP * 4)) || (e = up(B.vz, zero.data(), NP * 4)) ||
+      (e = up(B.status, st0.data(), NP)) || (e = up(B.origin, origin.data(), NP)) ||
+      (e = up(B.dstep, minus1.data(), NP * 4)) || (e = up(B.cellOf, minus1.data(), NP * 4)) ||
+      (e = hipMemsetAsync(B.cellCount, 0, NC * 4, s->stream)) ||
+      (e = hipMemsetAsync(B.u, 0, NF * 4, s->stream)) ||
+      (e = hipMemsetAsync(B.uE, 0, NF * 4, s->stream)) ||
+      (e = hipMemsetAsync(B.uc, 0, NF * 4, s->stream)) ||
+      (e = hipMemsetAsync(B.qP, 0, NC * 8, s->stream)) ||
+      (e = hipMemsetAsync(B.qD, 0, NC * 8, s->stream)) ||
+      (e = up(B.descs, descs, sizeof descs)) || (e = up(B.ctl, &H, sizeof(DevCtl))) ||
+      (e = hipStreamSynchronize(s->stream)))
+    return cleanup(fail(PL3D_EDEVICE, "upload: %s", hipGetErrorString(e)));
+  s->gridN = (n + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridC = (P.nc + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridF = (P.nfaces + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridFC = (P.nfaces + P.nc + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridExtrap = (((P.nfaces + 63) & ~63) + P.nc + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridScan = P.scanBlocks;
+  static const char *names[] = {"k_begin",   "k_count",  "k_classify_scan", "k_scatter",
+                                "k_p2g",     "k_liquid", "k_pcg",           "k_faces_post",
+                                "k_extrap",  "k_advect"};
+  for (const char *nm : names) s->timers.push_back(KernelTimer{nm});
+  if ((e = hipEventCreate(&s->ev0)) || (e = hipEventCreate(&s->ev1)))
+    return cleanup(fail(PL3D_EDEVICE, "events: %s", hipGetErrorString(e)));
+  s->lastDt = c.max_substep;
+  *out = s;
+  return PL3D_OK;
+}
+
+int pl3d_destroy(pl3d_sim *s) {
+  if (!s) return fail(PL3D_EINVAL, "null simulation handle");
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->mem) (void)hipFree(s->mem);
+  if (s->render) (void)hipFree(s->render);
+  if (s->hctl) (void)hipHostFree(s->hctl);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  s->destroyed = true;
+  delete s;
+  return PL3D_OK;
+}
+
+int pl3d_get_info(pl3d_sim *s, pl3d_info *out) {
+  if (int rc = check_sim(s)) return rc;
+  if (!out) return fail(PL3D_EINVAL, "null out");
+  *out = s->info;
+  return PL3D_OK;
+}
+
+int pl3d_step(pl3d_sim *s, double dt) {
+  if (int rc = check_sim(s)) return rc;
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(PL3D_EINVAL, "dt must be > 0");
+  int rc;
+  if ((rc = push_command(s, 1, 1, 0.0, dt))) return rc;
+  if ((rc = run_substeps(s, 1))) return rc;
+  if ((rc = sync_ctl(s))) return rc;
+  s->lastDt = dt;
+  return PL3D_OK;
+}
+
+int pl3d_advance(pl3d_sim *s, double seconds, pl3d_diagnostics *out) {
+  if (int rc = check_sim(s)) return rc;
+  if (!(seconds >= 0) || !std::isfinite(seconds)) return fail(PL3D_EINVAL, "bad duration");
+  int rc;
+  if ((rc = sync_ctl(s))) return rc;
+  const double end = s->hctl->time + seconds;
+  if ((rc = push_command(s, 0, 0, end, 0.0))) return rc;
+  for (int guard = 0; guard < 1000000; guard++) {
+    const double remain = end - s->hctl->time;
+    if (!(remain > 1e-12) || s->hctl->time >= s->cfg.duration - 1e-9) break;
+    const double dtEst = std::max(s->lastDt, 1e-5);
+    const int batch = std::max(1, std::min(64, int(std::ceil(remain / dtEst)) + 1));
+    if ((rc = run_substeps(s, batch))) return rc;
+    if ((rc = sync_ctl(s))) return rc;
+    if (s->hctl->dt > 0) s->lastDt = s->hctl->dt;
+    if (!s->hctl->active) break;
+  }
+  if (out) return pl3d_get_diagnostics(s, out);
+  return PL3D_OK;
+}
+
+int pl3d_stable_dt(pl3d_sim *s, double *out) {
+  if (int rc = check_sim(s)) return rc;
+  if (int rc = sync_ctl(s)) return rc;
+  const DevCtl &C = *s->hctl;
+  float m2;
+  std::memcpy(&m2, &C.maxSpeedSqBits, 4);
+  const double speed = std::sqrt(double(m2)) + std::sqrt(5 * s->P.hd * s->P.gravity) +
+                       std::fabs(C.omega) * s->P.Rdiag;
+  *out = std::min(s->P.maxSubstep, (s->P.cfl * s->P.hd) / speed);
+  return PL3D_OK;
+}
+
+int pl3d_finished(pl3d_sim *s, int32_t *out) {
+  if (int rc = check_sim(s)) return rc;
+  if (int rc = sync_ctl(s)) return rc;
+  *out = s->hctl->time >= s->cfg.duration - 1e-9 ? 1 : 0;
+  return PL3D_OK;
+}
+
+int pl3d_get_diagnostics(pl3d_sim *s, pl3d_diagnostics *d) {
+  if (int rc = check_sim(s)) return rc;
+  if (!d) return fail(PL3D_EINVAL, "null out");
+  if (int rc = sync_ctl(s)) return rc;
+  const DevCtl &C = *s->hctl;
+  const double v = s->P.parcelVolume * 2 * 1e6;
+  std::memset(d, 0, sizeof(*d));
+  d->time = C.time;
+  d->steps = C.steps;
+  d->angle_deg = C.angle / (PI / 180);
+  d->phase = C.phase;
+  d->particles = s->P.n;
+  d->grid[0] = s->P.nx;
+  d->grid[1] = s->P.ny;
+  d->grid[2] = s->P.nz;
+  d->discharged_ml = C.counts[1] * v;
+  d->spilled_ml = C.counts[2] * v;
+  d->lost_ml = C.counts[3] * v;
+  d->in_pot_ml = C.counts[0] * v;
+  d->pressure_iterations = C.pressureIterations;
+  d->pressure_failures = C.pressureFailures;
+  d->correction_iterations = C.correctionIterations;
+  float md;
+  std::memcpy(&md, &C.maxDivBits, 4);
+  d->max_divergence = double(md) / s->P.hd;
+  d->interior_density =
+      C.interiorCount > 0 ? double(C.interiorSumFixed) / 4294967296.0 / C.interiorCount : 1.0;
+  return PL3D_OK;
+}
+
+int pl3d_discharged_fractions(pl3d_sim *s, double out[3], int32_t *has) {
+  if (int rc = check_sim(s)) return rc;
+  if (int rc = sync_ctl(s)) return rc;
+  const DevCtl &C = *s->hctl;
+  const int total = C.byOrigin[0] + C.byOrigin[1] + C.byOrigin[2];
+  if (has) *has = total > 0;
+  for (int k = 0; k < 3; k++) out[k] = total > 0 ? double(C.byOrigin[k]) / total : 0.0;
+  return PL3D_OK;
+}
+
+int pl3d_stats(pl3d_sim *s, int32_t out[PL3D_STAT_SLOTS]) {
+  if (int rc = check_sim(s)) return rc;
+  if (int rc = sync_ctl(s)) return rc;
+  const DevCtl &C = *s->hctl;
+  std::memset(out, 0, sizeof(int32_t) * PL3D_STAT_SLOTS);
+  for (int k = 0; k < 4; k++) out[k] = C.counts[k];
+  for (int k = 0; k < 3; k++) out[4 + k] = C.byOrigin[k];
+  out[PL3D_STAT_PRESSURE_FAILURES] = C.pressureFailures;
+  std::memcpy(&out[PL3D_STAT_MAX_SPEED_SQ], &C.maxSpeedSqBits, 4);
+  float md;
+  std::memcpy(&md, &C.maxDivBits, 4);
+  md = float(double(md) / s->P.hd);
+  std::memcpy(&out[PL3D_STAT_MAX_DIVERGENCE], &md, 4);
+  out[PL3D_STAT_INTERIOR_COUNT] = C.interiorCount;
+  out[PL3D_STAT_INTERIOR_SUM] = int32_t(double(C.interiorSumFixed) / 4294967296.0 * 65536.0 + 0.5);
+  out[PL3D_STAT_PRESSURE_ITERATIONS] = C.pressureIterations;
+  out[PL3D_STAT_CORRECTION_ITERATIONS] = C.correctionIterations;
+  out[14] = C.lastM;
+  out[15] = C.overflow;
+  return PL3D_OK;
+}
+
+int pl3d_read_state(pl3d_sim *s, double *x, double *y, double *z, double *vx, double *vy,
+                    double *vz, uint8_t *status, uint8_t *origin, float *y0) {
+  if (int rc = check_sim(s)) return rc;
+  const size_t n = size_t(s->P.n);
+  std::vector<float> tmp(n);
+  auto rd = [&](const float *src, double *dst) -> int {
+    if (!dst) return PL3D_OK;
+    HIP_OK(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < n; i++) dst[i] = tmp[i];
+    return PL3D_OK;
+  };
+  int rc;
+  if ((rc = rd(s->B.x, x)) || (rc = rd(s->B.y, y)) || (rc = rd(s->B.z, z)) ||
+      (rc = rd(s->B.vx, vx)) || (rc = rd(s->B.vy, vy)) || (rc = rd(s->B.vz, vz)))
+    return rc;
+  if (status) {
+    HIP_OK(hipMemcpyAsync(status, s->B.status, n, hipMemcpyDeviceToHost, s->stream));
+  }
+  if (origin) {
+    HIP_OK(hipMemcpyAsync(origin, s->B.origin, n, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  if (y0) std::copy(s->y0.begin(), s->y0.end(), y0);
+  return PL3D_OK;
+}
+
+int pl3d_discharge_order(pl3d_sim *s, int32_t *out, int32_t cap, int32_t *count) {
+  if (int rc = check_sim(s)) return rc;
+  const size_t n = size_t(s->P.n);
+  std::vector<int> ds(n);
+  std::vector<uint8_t> st(n);
+  HIP_OK(hipMemcpyAsync(ds.data(), s->B.dstep, n * 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(st.data(), s->B.status, n, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  std::vector<int> idx;
+  for (size_t p = 0; p < n; p++)
+    if (st[p] == PL3D_DISCHARGED && ds[p] >= 0) idx.push_back(int(p));
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ds[a] < ds[b]; });
+  if (count) *count = int32_t(idx.size());
+  for (int32_t i = 0; i < cap && i < int32_t(idx.size()); i++) out[i] = idx[i];
+  return PL3D_OK;
+}
+
+int pl3d_read_particles(pl3d_sim *s, float *pos4, float *vel4, uint32_t *status) {
+  if (int rc = check_sim(s)) return rc;
+  const size_t n = size_t(s->P.n);
+  std::vector<float> a(n), b(n), c(n);
+  auto rd3 = [&](const float *x, const float *y, const float *z, float *dst) -> int {
+    if (!dst) return PL3D_OK;
+    HIP_OK(hipMemcpyAsync(a.data(), x, n * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(b.data(), y, n * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(c.data(), z, n * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < n; i++) {
+      dst[4 * i] = a[i];
+      dst[4 * i + 1] = b[i];
+      dst[4 * i + 2] = c[i];
+      dst[4 * i + 3] = 0.f;
+    }
+    return PL3D_OK;
+  };
+  int rc;
+  if ((rc = rd3(s->B.x, s->B.y, s->B.z, pos4))) return rc;
+  if ((rc = rd3(s->B.vx, s->B.vy, s->B.vz, vel4))) return rc;
+  if (status) {
+    std::vector<uint8_t> st(n);
+    HIP_OK(hipMemcpyAsync(st.data(), s->B.status, n, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    for (size_t i = 0; i < n; i++) status[i] = st[i];
+  }
+  return PL3D_OK;
+}
+
+int pl3d_read_render_positions(pl3d_sim *s, float *out) {
+  if (int rc = check_sim(s)) return rc;
+  if (!out) return fail(PL3D_EINVAL, "null out");
+  const size_t n = size_t(s->P.n);
+  if (!s->render) HIP_OK(hipMalloc((void **)&s->render, n * 24));
+  hipLaunchKernelGGL(k_render_positions, dim3(s->gridN), dim3(STEP_THREADS), 0, s->stream, s->B,
+                     s->P, s->render);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, s->render, n * 24, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return PL3D_OK;
+}
+
+int pl3d_set_target(pl3d_sim *s, double angle_deg) {
+  if (int rc = check_sim(s)) return rc;
+  if (int rc = sync_ctl(s)) return rc;
+  const double a = std::max(0.0, std::min(s->cfg.motion.max_angle_deg, angle_deg));
+  s->hctl->target = a * (PI / 180);
+  const size_t off = offsetof(DevCtl, target);
+  HIP_OK(hipMemcpyAsync(reinterpret_cast<char *>(s->B.ctl) + off,
+                        reinterpret_cast<char *>(s->hctl) + off, sizeof(double),
+                        hipMemcpyHostToDevice, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  return PL3D_OK;
+}
+
+int pl3d_set_timing(pl3d_sim *s, int32_t enabled) {
+  if (int rc = check_sim(s)) return rc;
+  s->timing = enabled != 0;
+  for (auto &t : s->timers) {
+    t.ms = 0;
+    t.launches = 0;
+  }
+  return PL3D_OK;
+}
+
+int pl3d_kernel_times(pl3d_sim *s, int32_t cap, const char **names, double *total_ms,
+                      int64_t *launches, int32_t *count) {
+  if (int rc = check_sim(s)) return rc;
+  const int32_t nt = int32_t(s->timers.size());
+  if (count) *count = nt;
+  for (int32_t i = 0; i < nt && i < cap; i++) {
+    if (names) names[i] = s->timers[i].name;
+    if (total_ms) total_ms[i] = s->timers[i].ms;
+    if (launches) launches[i] = s->timers[i].launches;
+  }
+  return PL3D_OK;
+}
+
+}  // extern "C"

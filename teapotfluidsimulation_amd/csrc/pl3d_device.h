@@ -1,0 +1,187 @@
+// pl3d_device.h — device-side layout and helpers shared by the gfx950 kernels of libpourlab3d.
+//
+// Data layout in HBM (one allocation per simulation, carved into arrays):
+//   parcels  SoA float x,y,z,vx,vy,vz; u8 status, origin; i32 cellOf, slot, dstep
+//   bins     i32 cellCount[C] (zeroed again by the scan), binStart[C+1], float4 binPos/binVel[n]
+//   cells    u8 flag[C]; i32 lidx[C] (liquid index or -1); float density[C]
+//   faces    float u/v/w[F], uPrev[F], uE[F] (extrapolated), uc[F]; u8 valid[F]
+//   liquid   (list order t = rank of the FLUID cell in index order) i32 lcell[m], u32 nbr[3][m]
+//            (u16 pairs -/+), u8 diag[m], u8 mic[m], u16 plane[m], double bP/bD/qP/qD[m]
+// Grid indexing follows the reference: cell c = i + nx*(j + ny*k); u face i + (nx+1)*(j + ny*k);
+// v face i + nx*(j + (ny+1)*k); w face i + nx*(j + ny*k) (solver3d.ts:159-163).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PL3D_AIR_ 0
+#define PL3D_FLUID_ 1
+#define PL3D_SOLID_ 2
+#define ST_IN_POT 0
+#define ST_DISCHARGED 1
+#define ST_SPILLED 2
+#define ST_LOST 3
+
+// Fixed-point scales of the deterministic particle-to-grid sums (64-bit integer accumulation is
+// associative, so the result does not depend on the order in which parcels are visited).
+#define P2G_SHIFT 40
+#define P2G_SCALE 1099511627776.0  // 2^40
+#define RATIO_SCALE 65536.0        // STAT interiorSum unit (gpu3d/layout.ts:17)
+
+#define NO_NBR 0xFFFFu
+
+// Device controller and per-substep state.  FP64 scalars: time, controller, pose.
+struct DevCtl {
+  // --- simulation clock (solver3d.ts:86-89) ---
+  double time;
+  double phi;
+  double omega;
+  long long steps;
+  // --- PourController (motion.ts:21-37) ---
+  int phase;
+  int pour;
+  double angle;
+  double phaseStart;
+  double startAngle;
+  double startRate;
+  double lastRate;
+  double flowRate;
+  double lastDischarged;
+  double target;
+  double manualRate;
+  // --- command from the host for the next substeps ---
+  int mode;        // 0 = advance to endTime with stable substeps, 1 = one explicit substep
+  int pending;     // explicit substeps left
+  double endTime;  // advance: run while time < endTime - 1e-12 && !finished
+  double dtExplicit;
+  // --- current substep (written by k_begin) ---
+  int active;
+  int pad0;
+  double dt;
+  double phi1;
+  float fdt, fomega;
+  float c0, s0, c1, s1;  // cos/sin of the old and new pose
+  double dc0, ds0;       // FP64 old pose for the classification
+  // --- counters (STAT) ---
+  int counts[4];
+  int byOrigin[3];
+  int pressureFailures;
+  unsigned int maxSpeedSqBits;  // of the last completed substep (input of stableDt)
+  unsigned int stepMaxSpeedSqBits;
+  unsigned int maxDivBits;
+  int interiorCount;
+  unsigned long long interiorSumFixed;  // sum of density ratios * 2^32
+  int pressureIterations;
+  int correctionIterations;
+  int pressureConverged;
+  int m;           // liquid cells of the current substep
+  int overflow;    // liquid cells beyond the solver capacity (sticky)
+  int lastM;
+  double pressureResidual;      // solver residual (max norm) of the last pressure solve
+  double pressureTrueResidual;  // FP64 max|b - A q| of the last pressure solve
+  double pressureBnorm;
+  // --- scan bookkeeping ---
+  unsigned int scanTicket;
+  unsigned int doneTicket;
+};
+
+// Fixed configuration, uploaded once.
+struct DevParams {
+  int nx, ny, nz;
+  int n;
+  int nc, nu, nv, nw;
+  int nfaces;
+  int scanBlocks;
+  float h, invh;
+  double hd;
+  // geometry (Teapot3D, vessel3d.ts:34-59)
+  double radius, height, wall, inletHeight, boreRadius, spoutLength;
+  double ax, ay, dx, dy, tipX, tipY;
+  // frames (solver3d.ts:71-72)
+  double pivotX, pivotY;        // cfg.pivot (pot-local)
+  double pivotWX, pivotWY;      // pivotWorld
+  double parcelVolume;
+  double gravity, flipRatio, densityCorrection, cfl, maxSubstep, duration;
+  double pressureTolerance;
+  int pressureMaxIterations;
+  int preconditioner;
+  double Rdiag;  // hypot(nx, ny) * h (solver3d.ts:275)
+  // motion config (types.ts:55-84)
+  int motionKind, repeats;
+  double startDelay, tiltDuration, holdAngleDeg, holdTiltRate, maxAngleDeg, returnDuration,
+      maxHold, holdTime, targetVolume, anticipation, repeatGap;
+};
+
+// ---------------------------------------------------------------------------- geometry
+// Teapot signed distance (vessel3d.ts:62-95), templated on the arithmetic type.
+template <class T>
+__device__ __forceinline__ T teapot_sdf(const DevParams &P, T x, T y, T z) {
+  const T radius = T(P.radius), wall = T(P.wall), height = T(P.height);
+  const T hw = wall / T(2);
+  const T r = sqrt(x * x + z * z);
+  const T ring = fabs(r - (radius + hw)) - hw;
+  const T yIn = fmax(-(y + wall), y - height);
+  const T side = fmax(ring, yIn);
+  const T floor_ = fmax(fabs(y + hw) - hw, r - (radius + wall));
+  T solid = fmin(side, floor_);
+  const T px = x - T(P.ax), py = y - T(P.ay);
+  const T s = px * T(P.dx) + py * T(P.dy);
+  const T L = T(P.spoutLength);
+  const T sc = s < T(0) ? T(0) : (s > L ? L : s);
+  const T qx = px - sc * T(P.dx), qy = py - sc * T(P.dy);
+  const T dist = sqrt(qx * qx + qy * qy + z * z);
+  const T along = fmax(-s, s - L);
+  const T tube = fmax(fabs(dist - (T(P.boreRadius) + hw)) - hw, along);
+  solid = fmin(solid, fmax(tube, radius - r));
+  const T bore = fmax(fmax(dist - T(P.boreRadius), -s - wall), s - L);
+  return fmax(solid, -bore);
+}
+
+// ---------------------------------------------------------------------------- stencils
+// Clamped trilinear stencil of one staggered component (solver3d.ts:339-362): base index per
+// dimension and fractions, computed from a position in cell units g = x/h - offset.
+struct Stencil1 {
+  int i;
+  float f;
+};
+__device__ __forceinline__ Stencil1 stencil1(float g, int s) {
+  float fl = floorf(g);
+  float fi = fminf(fmaxf(fl, 0.0f), float(s - 2));
+  Stencil1 r;
+  r.i = int(fi);
+  r.f = fminf(fmaxf(g - fi, 0.0f), 1.0f);
+  return r;
+}
+
+// Sample a staggered field with the reference's 8-corner order (solver3d.ts:366-377, :386-408).
+// valid == nullptr: plain trilinear value.  Otherwise masked: sum over valid corners divided by
+// their weight, falling back to the unmasked value when that weight is <= 1e-9.
+__device__ __forceinline__ float sample_field(const float *__restrict__ f,
+                                              const uint8_t *__restrict__ valid, float gx,
+                                              float gy, float gz, int sx, int sy, int sz) {
+  Stencil1 a = stencil1(gx, sx), b = stencil1(gy, sy), c = stencil1(gz, sz);
+  const int sxy = sx * sy;
+  const int base = a.i + sx * (b.i + sy * c.i);
+  const float fx = a.f, fy = b.f, fz = c.f;
+  const float w[8] = {(1 - fx) * (1 - fy) * (1 - fz), fx * (1 - fy) * (1 - fz),
+                      (1 - fx) * fy * (1 - fz),       fx * fy * (1 - fz),
+                      (1 - fx) * (1 - fy) * fz,       fx * (1 - fy) * fz,
+                      (1 - fx) * fy * fz,             fx * fy * fz};
+  const int off[8] = {0, 1, sx, sx + 1, sxy, sxy + 1, sxy + sx, sxy + sx + 1};
+  float all = 0.f, acc = 0.f, ws = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    float v = f[base + off[q]];
+    all += w[q] * v;
+    if (valid != nullptr && valid[base + off[q]]) {
+      acc += w[q] * v;
+      ws += w[q];
+    }
+  }
+  if (valid == nullptr) return all;
+  return ws > 1e-9f ? acc / ws : all;
+}
+
+__device__ __forceinline__ unsigned int float_as_ordered(float v) {  // v >= 0
+  return __float_as_uint(v);
+}

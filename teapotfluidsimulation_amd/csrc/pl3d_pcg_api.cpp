@@ -1,0 +1,165 @@
+// pl3d_pcg_api.cpp — pl3d_pcg_solve: the standalone Pcg3D.solve (solver3d.ts:858-937) on the
+// device.  The host lists the FLUID cells in index order, builds the 7-point structure (neighbour
+// liquid indices, non-solid neighbour count, MIC(0) couplings, hyperplane index), and one
+// workgroup of k_pcg solves.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pourlab3d.h"
+#include "pl3d_solver.h"
+
+#define NO_NBR_H 0xFFFFu
+
+extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell,
+                              const double *b, double *q, double tol, int32_t max_iter,
+                              int32_t precond, pl3d_solve_result *out);
+
+// error text goes to the thread-local slot read by pl3d_last_error (pl3d_api.cpp)
+int pl3d_set_error(int code, const std::string &msg);
+
+static int perr(int code, const std::string &msg) { return pl3d_set_error(code, msg); }
+
+extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell,
+                              const double *b, double *q, double tol, int32_t max_iter,
+                              int32_t precond, pl3d_solve_result *out) {
+  pl3d_set_error(PL3D_OK, "");
+  if (!cell || !b || !q || nx < 3 || ny < 3 || nz < 3 || max_iter < 0)
+    return perr(PL3D_EINVAL, "bad arguments");
+  if (precond != PL3D_PRECOND_MIC0 && precond != PL3D_PRECOND_JACOBI)
+    return perr(PL3D_EINVAL, "preconditioner not available");
+  const size_t nc = size_t(nx) * ny * nz;
+  const int nxy = nx * ny;
+  std::vector<int> lidx(nc, -1), lcell;
+  for (size_t c = 0; c < nc; c++)
+    if (cell[c] == PL3D_FLUID) {
+      const int i = int(c % nx), j = int((c / nx) % ny), k = int(c / nxy);
+      if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1)
+        return perr(PL3D_EINVAL, "FLUID cell on the grid border");
+      lidx[c] = int(lcell.size());
+      lcell.push_back(int(c));
+    }
+  const int m = int(lcell.size());
+  for (size_t c = 0; c < nc; c++) q[c] = 0.0;
+  if (m > PL3D_SOLVE_MMAX)
+    return perr(PL3D_ECAPACITY, "liquid cells " + std::to_string(m) + " exceed capacity " +
+                                    std::to_string(PL3D_SOLVE_MMAX));
+  if (nx + ny + nz - 2 > PL3D_MAX_PLANES) return perr(PL3D_ECAPACITY, "too many hyperplanes");
+  const int stride = std::max(m, 1);
+  std::vector<uint32_t> nbr(3 * size_t(stride), 0);
+  std::vector<uint8_t> diag(stride, 0), mic(stride, 0);
+  std::vector<uint16_t> plane(stride, 0);
+  std::vector<double> bl(stride, 0.0);
+  const long offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
+  auto fl = [&](long c) -> unsigned { return cell[c] == PL3D_FLUID ? 1u : 0u; };
+  for (int t = 0; t < m; t++) {
+    const long c = lcell[t];
+    uint32_t nb[6];
+    int dg = 0;
+    for (int d = 0; d < 6; d++) {
+      const uint8_t f = cell[c + offs[d]];
+      if (f != PL3D_SOLID) dg++;
+      nb[d] = f == PL3D_FLUID ? uint32_t(lidx[c + offs[d]]) : NO_NBR_H;
+    }
+    nbr[t] = nb[0] | (nb[1] << 16);
+    nbr[t + stride] = nb[2] | (nb[3] << 16);
+    nbr[t + 2 * size_t(stride)] = nb[4] | (nb[5] << 16);
+    diag[t] = uint8_t(dg);
+    unsigned mc = 0;
+    if (fl(c - 1)) mc |= fl(c - 1 + nx) + fl(c - 1 + nxy);
+    if (fl(c - nx)) mc |= (fl(c - nx + 1) + fl(c - nx + nxy)) << 2;
+    if (fl(c - nxy)) mc |= (fl(c - nxy + 1) + fl(c - nxy + nx)) << 4;
+    mic[t] = uint8_t(mc);
+    plane[t] = uint16_t(c % nx + (c / nx) % ny + c / nxy);
+    bl[t] = b[c];
+  }
+  // device buffers
+  struct Res {
+    int m, iterations, converged, pad;
+    double residual, trueResidual, bnorm;
+  };
+  size_t bytes = 0;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t S = size_t(stride);
+  const size_t o_res = bytes; bytes += al(sizeof(Res));
+  const size_t o_nbr = bytes; bytes += al(12 * S);
+  const size_t o_diag = bytes; bytes += al(S);
+  const size_t o_mic = bytes; bytes += al(S);
+  const size_t o_plane = bytes; bytes += al(2 * S);
+  const size_t o_b = bytes; bytes += al(8 * S);
+  const size_t o_q = bytes; bytes += al(8 * S);
+  const size_t o_porder = bytes; bytes += al(4 * S);
+  const size_t o_pc2 = bytes; bytes += al(4 * S);
+  const size_t o_nbp = bytes; bytes += al(12 * S);
+  const size_t o_desc = bytes; bytes += al(sizeof(PcgDesc));
+  char *mem = nullptr;
+  hipStream_t st = nullptr;
+  hipError_t e;
+  auto done = [&](int rc) {
+    if (st) (void)hipStreamSynchronize(st);
+    if (mem) (void)hipFree(mem);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+  };
+  if ((e = hipMalloc((void **)&mem, bytes)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess)
+    return done(perr(PL3D_EDEVICE, std::string("alloc: ") + hipGetErrorString(e)));
+  Res hres{};
+  hres.m = m;
+  PcgDesc D;
+  std::memset(&D, 0, sizeof D);
+  Res *dres = reinterpret_cast<Res *>(mem + o_res);
+  D.b = reinterpret_cast<double *>(mem + o_b);
+  D.q = reinterpret_cast<double *>(mem + o_q);
+  D.tol = float(tol);
+  D.maxIter = max_iter;
+  D.iterations = &dres->iterations;
+  D.converged = &dres->converged;
+  D.failures = nullptr;
+  D.residual = &dres->residual;
+  D.trueResidual = &dres->trueResidual;
+  D.bnorm = &dres->bnorm;
+  D.porder = reinterpret_cast<int *>(mem + o_porder);
+  D.pc2 = reinterpret_cast<float *>(mem + o_pc2);
+  D.nbrPlane = reinterpret_cast<uint32_t *>(mem + o_nbp);
+  PcgShared A;
+  A.m = &dres->m;
+  A.nbr = reinterpret_cast<uint32_t *>(mem + o_nbr);
+  A.diag = reinterpret_cast<uint8_t *>(mem + o_diag);
+  A.mic = reinterpret_cast<uint8_t *>(mem + o_mic);
+  A.plane = reinterpret_cast<uint16_t *>(mem + o_plane);
+  A.stride = stride;
+  A.planes = nx + ny + nz - 2;
+  A.precond = precond == PL3D_PRECOND_JACOBI ? 2 : 0;
+  A.overflow = nullptr;
+  auto up = [&](size_t o, const void *src, size_t n) {
+    return hipMemcpyAsync(mem + o, src, n, hipMemcpyHostToDevice, st);
+  };
+  if ((e = up(o_res, &hres, sizeof hres)) || (e = up(o_nbr, nbr.data(), 12 * S)) ||
+      (e = up(o_diag, diag.data(), S)) || (e = up(o_mic, mic.data(), S)) ||
+      (e = up(o_plane, plane.data(), 2 * S)) || (e = up(o_b, bl.data(), 8 * S)) ||
+      (e = up(o_desc, &D, sizeof D)))
+    return done(perr(PL3D_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
+  hipLaunchKernelGGL(k_pcg, dim3(1), dim3(PL3D_SOLVE_THREADS), 0, st, A,
+                     reinterpret_cast<const PcgDesc *>(mem + o_desc), nullptr);
+  if ((e = hipGetLastError()) != hipSuccess)
+    return done(perr(PL3D_EDEVICE, std::string("launch: ") + hipGetErrorString(e)));
+  std::vector<double> ql(S);
+  if ((e = hipMemcpyAsync(ql.data(), mem + o_q, 8 * S, hipMemcpyDeviceToHost, st)) ||
+      (e = hipMemcpyAsync(&hres, mem + o_res, sizeof hres, hipMemcpyDeviceToHost, st)) ||
+      (e = hipStreamSynchronize(st)))
+    return done(perr(PL3D_EDEVICE, std::string("solve: ") + hipGetErrorString(e)));
+  for (int t = 0; t < m; t++) q[lcell[t]] = ql[t];
+  if (out) {
+    out->iterations = m == 0 ? 0 : hres.iterations;
+    out->converged = m == 0 ? 1 : hres.converged;
+    out->residual = m == 0 ? 0.0 : hres.residual;
+    out->true_residual = m == 0 ? 0.0 : hres.trueResidual;
+    out->liquid_cells = m;
+    out->pad = 0;
+  }
+  return done(PL3D_OK);
+}

@@ -1,0 +1,51 @@
+// pl3d_step.h — device buffers of one simulation and the substep kernels (pl3d_step.hip).
+#pragma once
+
+#include "pl3d_device.h"
+#include "pl3d_solver.h"
+
+#define STEP_THREADS 256
+#define SCAN_ITEMS 4
+#define SCAN_TILE (STEP_THREADS * SCAN_ITEMS)
+
+struct DevBufs {
+  DevCtl *ctl;
+  // parcels
+  float *x, *y, *z, *vx, *vy, *vz;
+  uint8_t *status, *origin;
+  int *cellOf, *slot, *dstep;
+  // bins
+  int *cellCount;  // zero between substeps
+  int *binStart;   // C + 1
+  float4 *binPos, *binVel;
+  unsigned long long *scanStatus;  // one word per scan tile
+  // cells
+  uint8_t *flag;
+  int *lidx;
+  float *density;
+  // faces: [u | v | w] concatenated, F = nu + nv + nw
+  float *u, *uPrev, *uE, *uc;
+  uint8_t *valid;
+  // liquid list (capacity C)
+  int *lcell;
+  uint32_t *nbr;  // 3 x C
+  uint8_t *diag, *mic;
+  uint16_t *plane;
+  double *bP, *bD, *qP, *qD;
+  // solver scratch (two solves)
+  int *porder;     // 2 x C
+  float *pc2;      // 2 x C
+  uint32_t *nbrPlane;  // 2 x 3 x C
+  PcgDesc *descs;  // 2
+};
+
+__global__ void k_begin(DevBufs B, DevParams P);
+__global__ void k_count(DevBufs B, DevParams P);
+__global__ void k_classify_scan(DevBufs B, DevParams P);
+__global__ void k_scatter(DevBufs B, DevParams P);
+__global__ void k_p2g(DevBufs B, DevParams P);
+__global__ void k_liquid(DevBufs B, DevParams P);
+__global__ void k_faces_post(DevBufs B, DevParams P);
+__global__ void k_extrap(DevBufs B, DevParams P);
+__global__ void k_advect(DevBufs B, DevParams P);
+__global__ void k_render_positions(DevBufs B, DevParams P, float *out);

@@ -1,0 +1,817 @@
+// pl3d_step.hip — the substep of the 3D pour (Simulation3D.step, solver3d.ts:288-309) as gfx950
+// kernels.  One substep = k_begin, k_count, k_classify_scan, k_scatter, k_p2g, k_liquid, k_pcg
+// (pressure + density correction, one workgroup each), k_faces_post, k_extrap, k_advect.  Every
+// kernel returns at once when k_begin found nothing to do (ctl->active == 0), so a fixed batch of
+// captured substeps can be replayed without a host round trip per substep.
+//
+// Determinism: the only atomics are integer ones (bin counts and slots, status counters) and
+// order-independent float max; particle-to-grid sums are gathered per face from cell bins and
+// accumulated in 64-bit fixed point, so the order in which parcels sit in a bin does not change
+// any bit of the result.
+#include "pl3d_step.h"
+
+namespace {
+
+constexpr double PI_D = 3.14159265358979323846;
+constexpr double DEG = PI_D / 180.0;
+
+__device__ __forceinline__ int cidx(const DevParams &P, int i, int j, int k) {
+  return i + P.nx * (j + P.ny * k);
+}
+
+// ----------------------------------------------------------------------------- controller
+// PourController.advance (motion.ts:74-143) in FP64 on the device.
+__device__ bool should_return(const DevCtl &C, const DevParams &P, double t, double discharged) {
+  if (P.motionKind == 2) return C.phase == 2 && t - C.phaseStart >= P.holdTime;  // 'tilt'
+  if (C.phase == 2 && t - C.phaseStart >= P.maxHold) return true;
+  if (P.targetVolume <= 0) return false;
+  const double inTransit = fmax(0.0, C.flowRate) * P.anticipation * P.returnDuration;
+  const int reps = P.repeats > 1 ? P.repeats : 1;
+  const double target = (P.targetVolume * (C.pour + 1)) / reps;
+  return discharged + inTransit >= target;
+}
+
+__device__ void enter(DevCtl &C, int phase, double t) {
+  C.phase = phase;
+  C.phaseStart = t;
+  C.startAngle = C.angle;
+  C.startRate = C.lastRate;
+}
+
+__device__ double controller_advance(DevCtl &C, const DevParams &P, double t, double dt,
+                                     double discharged) {
+  if (dt > 0) {
+    const double inst = (discharged - C.lastDischarged) / dt;
+    const double k = fmin(1.0, dt / 0.25);
+    C.flowRate += (inst - C.flowRate) * k;
+  }
+  C.lastDischarged = discharged;
+  const double prev = C.angle;
+  if (C.phase == 6) {  // manual
+    if (dt > 0) {
+      const double desired = fmax(-90 * DEG, fmin(90 * DEG, (C.target - C.angle) / 0.15));
+      const double maxChange = 600 * DEG * dt;
+      C.manualRate += fmax(-maxChange, fmin(maxChange, desired - C.manualRate));
+      C.angle += C.manualRate * dt;
+      C.lastRate = (C.angle - prev) / dt;
+    }
+    return C.angle;
+  }
+  const double hold = P.holdAngleDeg * DEG;
+  const double maxA = P.maxAngleDeg * DEG;
+  const int reps = P.repeats > 1 ? P.repeats : 1;
+  for (int guard = 0; guard < 4; guard++) {
+    const double el = t - C.phaseStart;
+    if (C.phase == 0 && t >= P.startDelay) {
+      enter(C, 1, P.startDelay);
+    } else if ((C.phase == 1 || C.phase == 2) && should_return(C, P, t, discharged)) {
+      enter(C, 3, t);
+    } else if (C.phase == 1 && el >= P.tiltDuration) {
+      C.angle = hold;
+      enter(C, 2, C.phaseStart + P.tiltDuration);
+    } else if (C.phase == 3 && el >= P.returnDuration) {
+      C.angle = 0;
+      C.pour++;
+      enter(C, 4, C.phaseStart + P.returnDuration);
+    } else if (C.phase == 4 && C.pour < reps && el >= P.repeatGap) {
+      enter(C, 1, C.phaseStart + P.repeatGap);
+    } else if (C.phase == 4 && C.pour >= reps) {
+      enter(C, 5, t);
+    } else {
+      break;
+    }
+  }
+  const double el = t - C.phaseStart;
+  switch (C.phase) {
+    case 1: {
+      const double s = fmin(1.0, el / P.tiltDuration);
+      C.angle = C.startAngle + (hold - C.startAngle) * 0.5 * (1 - cos(PI_D * s));
+      break;
+    }
+    case 2:
+      C.angle = fmin(maxA, C.startAngle + P.holdTiltRate * DEG * el);
+      break;
+    case 3: {
+      const double T = P.returnDuration;
+      const double s = fmin(1.0, el / T);
+      const double h00 = 2 * s * s * s - 3 * (s * s) + 1;
+      const double h10 = s * s * s - 2 * (s * s) + s;
+      C.angle = h00 * C.startAngle + h10 * T * C.startRate;
+      break;
+    }
+    default:
+      C.angle = 0;
+  }
+  if (dt > 0) C.lastRate = (C.angle - prev) / dt;
+  return C.angle;
+}
+
+__device__ __forceinline__ double stable_dt(const DevCtl &C, const DevParams &P) {
+  // solver3d.ts:268-278, max speed^2 from the last completed substep
+  const double m = double(__uint_as_float(C.maxSpeedSqBits));
+  const double speed = sqrt(m) + sqrt(5 * P.hd * P.gravity) + fabs(C.omega) * P.Rdiag;
+  return fmin(P.maxSubstep, (P.cfl * P.hd) / speed);
+}
+
+__device__ __forceinline__ bool is_active(const DevBufs &B) {
+  return *(volatile int *)&B.ctl->active != 0;
+}
+
+// fixed-point conversion of a (small) float term
+__device__ __forceinline__ long long fx40(float v) { return __double2ll_rn(double(v) * P2G_SCALE); }
+
+}  // namespace
+
+// ============================================================================= k_begin
+// One substep's scalar work: the substep length (stableDt or the caller's dt), the controller
+// (motion.ts:74), omega and the two poses; clears the per-substep counters.
+__global__ void k_begin(DevBufs B, DevParams P) {
+  for (int i = threadIdx.x; i < P.scanBlocks; i += blockDim.x) B.scanStatus[i] = 0ull;
+  if (threadIdx.x != 0) return;
+  DevCtl &C = *B.ctl;
+  bool run;
+  double dt = 0.0;
+  if (C.mode == 1) {
+    run = C.pending > 0;
+    dt = C.dtExplicit;
+    if (run) C.pending--;
+  } else {
+    run = C.time < C.endTime - 1e-12 && !(C.time >= P.duration - 1e-9);  // solver3d.ts:264, :283
+    if (run) dt = fmin(stable_dt(C, P), P.duration - C.time);
+  }
+  C.active = run ? 1 : 0;
+  if (!run) return;
+  const double discharged = C.counts[ST_DISCHARGED] * P.parcelVolume * 2;
+  const double tilt1 = controller_advance(C, P, C.time + dt, dt, discharged);
+  const double phi1 = -tilt1;
+  C.omega = (phi1 - C.phi) / dt;
+  C.dt = dt;
+  C.phi1 = phi1;
+  C.fdt = float(dt);
+  C.fomega = float(C.omega);
+  C.dc0 = cos(C.phi);
+  C.ds0 = sin(C.phi);
+  C.c0 = float(C.dc0);
+  C.s0 = float(C.ds0);
+  C.c1 = float(cos(phi1));
+  C.s1 = float(sin(phi1));
+  C.stepMaxSpeedSqBits = 0u;
+  C.maxDivBits = 0u;
+  C.interiorCount = 0;
+  C.interiorSumFixed = 0ull;
+  C.scanTicket = 0u;
+  C.doneTicket = 0u;
+  __threadfence();
+}
+
+// ============================================================================= k_count
+// Bin IN_POT parcels by cell (solver3d.ts:327-335 uses the same cell to mark FLUID).
+__global__ void k_count(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  if (B.status[p] != ST_IN_POT) {
+    B.cellOf[p] = -1;
+    return;
+  }
+  const int i = min(max(int(floorf(B.x[p] * P.invh)), 0), P.nx - 1);
+  const int j = min(max(int(floorf(B.y[p] * P.invh)), 0), P.ny - 1);
+  const int k = min(max(int(floorf(B.z[p] * P.invh)), 0), P.nz - 1);
+  const int c = cidx(P, i, j, k);
+  B.cellOf[p] = c;
+  B.slot[p] = atomicAdd(&B.cellCount[c], 1);
+}
+
+// ============================================================================= k_classify_scan
+// classify (solver3d.ts:313-336): SOLID border or sdf < 0 at the old pose (FP64), FLUID for an
+// AIR cell holding an IN_POT parcel, else AIR.  In the same pass, a single-pass decoupled
+// look-back scan over cells of (parcel count, FLUID) gives bin offsets and the liquid list in
+// grid index order (the order of Pcg3D's list, solver3d.ts:869).
+__global__ __launch_bounds__(STEP_THREADS) void k_classify_scan(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  __shared__ unsigned int tileId;
+  __shared__ unsigned long long wsum[STEP_THREADS / 64];
+  __shared__ unsigned long long prefixS;
+  DevCtl &C = *B.ctl;
+  if (threadIdx.x == 0) tileId = atomicAdd(&C.scanTicket, 1u);
+  __syncthreads();
+  const int tile = int(tileId);
+  const int c0 = tile * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  const double cph = C.dc0, sph = C.ds0;
+  unsigned int cnt[SCAN_ITEMS];
+  unsigned int fl[SCAN_ITEMS];
+  unsigned long long local = 0ull;  // (fluid << 32) | count
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    const int c = c0 + q;
+    cnt[q] = 0;
+    fl[q] = 0;
+    if (c < P.nc) {
+      const int i = c % P.nx;
+      const int j = (c / P.nx) % P.ny;
+      const int k = c / (P.nx * P.ny);
+      uint8_t f;
+      const int count = B.cellCount[c];
+      B.cellCount[c] = 0;  // ready for the next substep's k_count
+      if (i == 0 || j == 0 || k == 0 || i == P.nx - 1 || j == P.ny - 1 || k == P.nz - 1) {
+        f = PL3D_SOLID_;
+      } else {
+        const double rx = (i + 0.5) * P.hd - P.pivotWX;
+        const double ry = (j + 0.5) * P.hd - P.pivotWY;
+        const double lx = cph * rx + sph * ry + P.pivotX;
+        const double ly = -sph * rx + cph * ry + P.pivotY;
+        const double lz = (k + 0.5) * P.hd - P.hd;
+        f = teapot_sdf<double>(P, lx, ly, lz) < 0.0 ? PL3D_SOLID_ : PL3D_AIR_;
+        if (f == PL3D_AIR_ && count > 0) f = PL3D_FLUID_;
+      }
+      B.flag[c] = f;
+      cnt[q] = unsigned(count);
+      fl[q] = f == PL3D_FLUID_ ? 1u : 0u;
+      local += (unsigned long long)cnt[q] | ((unsigned long long)fl[q] << 32);
+    }
+  }
+  // block-exclusive scan of `local` (both halves add independently; no carries cross 32 bits)
+  unsigned long long incl = local;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned long long v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  unsigned long long waveOff = 0ull, blockTotal = 0ull;
+#pragma unroll
+  for (int w = 0; w < STEP_THREADS / 64; w++) {
+    if (w < wave) waveOff += wsum[w];
+    blockTotal += wsum[w];
+  }
+  // look-back: status word = flag(2 bits) << 62 | fluid(31) << 31 | count(31)
+  if (threadIdx.x == 0) {
+    const unsigned long long aggC = blockTotal & 0xFFFFFFFFull, aggF = blockTotal >> 32;
+    const unsigned long long A_FLAG = 1ull << 62, P_FLAG = 2ull << 62;
+    unsigned long long exC = 0ull, exF = 0ull;
+    if (tile == 0) {
+      __hip_atomic_store(&B.scanStatus[0], P_FLAG | (aggF << 31) | aggC, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&B.scanStatus[tile], A_FLAG | (aggF << 31) | aggC, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      int j = tile - 1;
+      while (true) {
+        unsigned long long s;
+        do {
+          s = __hip_atomic_load(&B.scanStatus[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        } while ((s >> 62) == 0ull);
+        exC += s & 0x7FFFFFFFull;
+        exF += (s >> 31) & 0x7FFFFFFFull;
+        if ((s >> 62) == 2ull) break;
+        j--;
+      }
+      __hip_atomic_store(&B.scanStatus[tile],
+                         P_FLAG | ((exF + aggF) << 31) | (exC + aggC), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    prefixS = exC | (exF << 32);
+    if (tile == P.scanBlocks - 1) {
+      B.binStart[P.nc] = int(exC + aggC);
+      C.m = int(exF + aggF);
+    }
+  }
+  __syncthreads();
+  unsigned long long run = prefixS + waveOff + (incl - local);
+  unsigned int runC = unsigned(run & 0xFFFFFFFFull), runF = unsigned(run >> 32);
+#pragma unroll
+  for (int q = 0; q < SCAN_ITEMS; q++) {
+    const int c = c0 + q;
+    if (c < P.nc) {
+      B.binStart[c] = int(runC);
+      if (fl[q]) {
+        B.lidx[c] = int(runF);
+        B.lcell[runF] = c;
+      } else {
+        B.lidx[c] = -1;
+      }
+      runC += cnt[q];
+      runF += fl[q];
+    }
+  }
+}
+
+// ============================================================================= k_scatter
+__global__ void k_scatter(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  const int c = B.cellOf[p];
+  if (c < 0) return;
+  const int dst = B.binStart[c] + B.slot[p];
+  B.binPos[dst] = make_float4(B.x[p], B.y[p], B.z[p], 0.f);
+  B.binVel[dst] = make_float4(B.vx[p], B.vy[p], B.vz[p], 0.f);
+}
+
+// ============================================================================= k_p2g
+// particlesToGrid (solver3d.ts:410-431) as a gather: each face sums the parcels of the cells whose
+// clamped stencils can reach it, in 64-bit fixed point; then uPrev = u (:295), gravity on v faces
+// next to FLUID (:433-443), wall velocity on faces next to SOLID (:446-471) and the validity
+// mask (:545-562).  Extra threads after the faces compute the cell-centre parcel density of FLUID
+// cells (densityCorrection, :635-642).
+__device__ __forceinline__ bool corner_w(float g, int s, int f, float &w) {
+  const Stencil1 st = stencil1(g, s);
+  if (st.i == f) {
+    w = 1.f - st.f;
+    return true;
+  }
+  if (st.i + 1 == f) {
+    w = st.f;
+    return true;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(STEP_THREADS) void k_p2g(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  const DevCtl &C = *B.ctl;
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  if (id < P.nfaces) {
+    int comp, i, j, k, sx, sy, sz;
+    float ox, oy, oz;
+    int f = id;
+    if (f < P.nu) {
+      comp = 0; sx = nx + 1; sy = ny; sz = nz; ox = 0.f; oy = 0.5f; oz = 0.5f;
+    } else if (f < P.nu + P.nv) {
+      f -= P.nu;
+      comp = 1; sx = nx; sy = ny + 1; sz = nz; ox = 0.5f; oy = 0.f; oz = 0.5f;
+    } else {
+      f -= P.nu + P.nv;
+      comp = 2; sx = nx; sy = ny; sz = nz + 1; ox = 0.5f; oy = 0.5f; oz = 0.f;
+    }
+    i = f % sx;
+    j = (f / sx) % sy;
+    k = f / (sx * sy);
+    // candidate cell ranges per dimension: offset 0 -> {f-1, f}, offset 1/2 -> {f-1, f, f+1}
+    const int xa = max(i - 1, 0), xb = min(comp == 0 ? i : i + 1, nx - 1);
+    const int ya = max(j - 1, 0), yb = min(comp == 1 ? j : j + 1, ny - 1);
+    const int za = max(k - 1, 0), zb = min(comp == 2 ? k : k + 1, nz - 1);
+    long long sw = 0, swv = 0;
+    for (int cz = za; cz <= zb; cz++)
+      for (int cy = ya; cy <= yb; cy++) {
+        const int r0 = B.binStart[cidx(P, xa, cy, cz)];
+        const int r1 = B.binStart[cidx(P, xb, cy, cz) + 1];
+        for (int q = r0; q < r1; q++) {
+          const float4 pp = B.binPos[q];
+          float wx, wy, wz;
+          if (!corner_w(pp.x * P.invh - ox, sx, i, wx)) continue;
+          if (!corner_w(pp.y * P.invh - oy, sy, j, wy)) continue;
+          if (!corner_w(pp.z * P.invh - oz, sz, k, wz)) continue;
+          const float4 vv = B.binVel[q];
+          const float vel = comp == 0 ? vv.x : (comp == 1 ? vv.y : vv.z);
+          const float w = wx * wy * wz;
+          sw += fx40(w);
+          swv += fx40(w * vel);
+        }
+      }
+    float val = sw > (long long)(1e-9 * P2G_SCALE) ? float(double(swv) / double(sw)) : 0.f;
+    B.uPrev[id] = val;
+    // neighbouring cells across the face (outside the grid counts as SOLID)
+    int ia = i, ja = j, ka = k;
+    if (comp == 0) ia--; else if (comp == 1) ja--; else ka--;
+    const uint8_t a = (ia >= 0 && ja >= 0 && ka >= 0) ? B.flag[cidx(P, ia, ja, ka)] : PL3D_SOLID_;
+    const uint8_t b = (i < nx && j < ny && k < nz) ? B.flag[cidx(P, i, j, k)] : PL3D_SOLID_;
+    if (comp == 1 && j >= 1 && j <= ny - 1 && (a == PL3D_FLUID_ || b == PL3D_FLUID_))
+      val -= float(P.gravity * C.dt);
+    if (a == PL3D_SOLID_ || b == PL3D_SOLID_) {
+      if (comp == 0) val = float(-C.omega * ((j + 0.5) * P.hd - P.pivotWY));
+      else if (comp == 1) val = float(C.omega * ((i + 0.5) * P.hd - P.pivotWX));
+      else val = 0.f;
+    }
+    B.u[id] = val;
+    B.valid[id] = (a == PL3D_SOLID_ || b == PL3D_SOLID_ || a == PL3D_FLUID_ || b == PL3D_FLUID_)
+                      ? 1 : 0;
+    return;
+  }
+  id -= P.nfaces;
+  if (id >= P.nc) return;
+  if (B.flag[id] != PL3D_FLUID_) return;
+  const int i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
+  const int xa = max(i - 1, 0), xb = min(i + 1, nx - 1);
+  const int ya = max(j - 1, 0), yb = min(j + 1, ny - 1);
+  const int za = max(k - 1, 0), zb = min(k + 1, nz - 1);
+  long long sw = 0;
+  for (int cz = za; cz <= zb; cz++)
+    for (int cy = ya; cy <= yb; cy++) {
+      const int r0 = B.binStart[cidx(P, xa, cy, cz)];
+      const int r1 = B.binStart[cidx(P, xb, cy, cz) + 1];
+      for (int q = r0; q < r1; q++) {
+        const float4 pp = B.binPos[q];
+        float wx, wy, wz;
+        if (!corner_w(pp.x * P.invh - 0.5f, nx, i, wx)) continue;
+        if (!corner_w(pp.y * P.invh - 0.5f, ny, j, wy)) continue;
+        if (!corner_w(pp.z * P.invh - 0.5f, nz, k, wz)) continue;
+        sw += fx40(wx * wy * wz);
+      }
+    }
+  B.density[id] = float(double(sw) / P2G_SCALE);
+}
+
+// ============================================================================= k_liquid
+// Per liquid cell: pressure rhs = -div u (divergenceRhs, solver3d.ts:473-493), the 7-point
+// operator's structure (Pcg3D.solve setup, :872-881), the density-correction rhs and the
+// interior-density statistic (densityCorrection, :643-670).
+__global__ __launch_bounds__(STEP_THREADS) void k_liquid(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  DevCtl &C = *B.ctl;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = C.m;
+  if (blockIdx.x * blockDim.x >= m) return;  // whole block past the list
+  unsigned long long ratioFixed = 0ull;
+  unsigned int interiorOne = 0u;
+  if (t < m) {
+    const int nx = P.nx, ny = P.ny, nxy = nx * ny;
+    const int c = B.lcell[t];
+    const int i = c % nx, j = (c / nx) % ny, k = c / nxy;
+    const float *u = B.u, *v = B.u + P.nu, *w = B.u + P.nu + P.nv;
+    const float d = u[i + 1 + (nx + 1) * (j + ny * k)] - u[i + (nx + 1) * (j + ny * k)] +
+                    v[i + nx * (j + 1 + (ny + 1) * k)] - v[i + nx * (j + (ny + 1) * k)] +
+                    w[i + nx * (j + ny * (k + 1))] - w[i + nx * (j + ny * k)];
+    B.bP[t] = double(-d);
+    const int offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
+    uint8_t f6[6];
+    int dg = 0;
+    bool interior = true;
+    uint32_t nb[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const uint8_t fq = B.flag[c + offs[q]];
+      f6[q] = fq;
+      if (fq != PL3D_SOLID_) dg++;
+      if (fq == PL3D_AIR_) interior = false;
+      nb[q] = fq == PL3D_FLUID_ ? uint32_t(B.lidx[c + offs[q]]) : NO_NBR;
+    }
+    B.nbr[t] = nb[0] | (nb[1] << 16);
+    B.nbr[t + P.nc] = nb[2] | (nb[3] << 16);
+    B.nbr[t + 2 * P.nc] = nb[4] | (nb[5] << 16);
+    B.diag[t] = uint8_t(dg);
+    // MIC(0) couplings of the -x/-y/-z neighbours: count of that neighbour's FLUID neighbours in
+    // the two other + directions (the a'+a'' terms of solver3d.ts:894-897)
+    auto fluid = [&](int cc) -> unsigned { return B.flag[cc] == PL3D_FLUID_ ? 1u : 0u; };
+    unsigned mc = 0;
+    if (f6[0] == PL3D_FLUID_) mc |= (fluid(c - 1 + nx) + fluid(c - 1 + nxy));
+    if (f6[2] == PL3D_FLUID_) mc |= (fluid(c - nx + 1) + fluid(c - nx + nxy)) << 2;
+    if (f6[4] == PL3D_FLUID_) mc |= (fluid(c - nxy + 1) + fluid(c - nxy + nx)) << 4;
+    B.mic[t] = uint8_t(mc);
+    B.plane[t] = uint16_t(i + j + k);
+    // density correction
+    const double ratio = double(B.density[c]) / 8;
+    if (interior) {
+      interiorOne = 1u;
+      ratioFixed = (unsigned long long)(ratio * 4294967296.0 + 0.5);
+    }
+    double rd = 0.0;
+    if (P.densityCorrection > 0 && (ratio > 1 || interior))
+      rd = (P.densityCorrection * (ratio - 1) * P.hd) / C.dt;
+    B.bD[t] = rd;
+  }
+  // one atomic per wave for the interior-density statistic (integer: order-independent)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    ratioFixed += __shfl_xor(ratioFixed, o, 64);
+    interiorOne += __shfl_xor(interiorOne, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && interiorOne > 0) {
+    atomicAdd(&C.interiorCount, int(interiorOne));
+    atomicAdd(&C.interiorSumFixed, ratioFixed);
+  }
+}
+
+// ============================================================================= k_faces_post
+// applyGradient (solver3d.ts:496-528): u -= q_b - q_a on faces next to liquid with no solid side
+// (pressure, in FP64 differences), and uc = -(qc_b - qc_a) (density correction, :674).
+__global__ void k_faces_post(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= P.nfaces) return;
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  int comp, i, j, k, f = id, sx, sy;
+  if (f < P.nu) {
+    comp = 0; sx = nx + 1; sy = ny;
+  } else if (f < P.nu + P.nv) {
+    f -= P.nu; comp = 1; sx = nx; sy = ny + 1;
+  } else {
+    f -= P.nu + P.nv; comp = 2; sx = nx; sy = ny;
+  }
+  i = f % sx;
+  j = (f / sx) % sy;
+  k = f / (sx * sy);
+  float ucv = 0.f;
+  const bool inner = comp == 0 ? (i >= 1 && i <= nx - 1)
+                               : (comp == 1 ? (j >= 1 && j <= ny - 1) : (k >= 1 && k <= nz - 1));
+  if (inner) {
+    const int cb = cidx(P, i, j, k);
+    const int ca = comp == 0 ? cb - 1 : (comp == 1 ? cb - nx : cb - nx * ny);
+    const uint8_t fa = B.flag[ca], fb = B.flag[cb];
+    if (!(fa == PL3D_SOLID_ || fb == PL3D_SOLID_ || (fa != PL3D_FLUID_ && fb != PL3D_FLUID_))) {
+      const int la = fa == PL3D_FLUID_ ? B.lidx[ca] : -1;
+      const int lb = fb == PL3D_FLUID_ ? B.lidx[cb] : -1;
+      const double qa = la >= 0 ? B.qP[la] : 0.0, qb = lb >= 0 ? B.qP[lb] : 0.0;
+      const double da = la >= 0 ? B.qD[la] : 0.0, db = lb >= 0 ? B.qD[lb] : 0.0;
+      B.u[id] = B.u[id] - float(qb - qa);
+      ucv = -float(db - da);
+    }
+  }
+  B.uc[id] = ucv;
+}
+
+// ============================================================================= k_extrap
+// extrapolate (solver3d.ts:565-605): two Jacobi layers of 6-neighbour averaging into faces not
+// valid, fused: a face in layer 2 recomputes its neighbours' layer-1 values.  Written to uE so the
+// fields stay readable while other faces are processed.  Extra threads: post-projection max |div|
+// over FLUID cells (divergenceRhs, :542).
+__device__ __forceinline__ int layer1(const float *u, const uint8_t *vm, int c, int i, int j, int k,
+                                      int sx, int sy, int sz, float &sum) {
+  const int sxy = sx * sy;
+  int n = 0;
+  float s = 0.f;
+  if (i > 0 && vm[c - 1]) { s += u[c - 1]; n++; }
+  if (i < sx - 1 && vm[c + 1]) { s += u[c + 1]; n++; }
+  if (j > 0 && vm[c - sx]) { s += u[c - sx]; n++; }
+  if (j < sy - 1 && vm[c + sx]) { s += u[c + sx]; n++; }
+  if (k > 0 && vm[c - sxy]) { s += u[c - sxy]; n++; }
+  if (k < sz - 1 && vm[c + sxy]) { s += u[c + sxy]; n++; }
+  sum = s;
+  return n;
+}
+
+__global__ void k_extrap(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const int facesPadded = (P.nfaces + 63) & ~63;  // cell threads start on a wave boundary
+  if (id < facesPadded) {
+    if (id >= P.nfaces) return;
+    int f = id, base = 0, sx, sy, sz;
+    if (f < P.nu) {
+      sx = nx + 1; sy = ny; sz = nz;
+    } else if (f < P.nu + P.nv) {
+      base = P.nu; f -= base; sx = nx; sy = ny + 1; sz = nz;
+    } else {
+      base = P.nu + P.nv; f -= base; sx = nx; sy = ny; sz = nz + 1;
+    }
+    const float *u = B.u + base;
+    const uint8_t *vm = B.valid + base;
+    if (vm[f]) {
+      B.uE[id] = u[f];
+      return;
+    }
+    const int i = f % sx, j = (f / sx) % sy, k = f / (sx * sy);
+    float s;
+    int n = layer1(u, vm, f, i, j, k, sx, sy, sz, s);
+    if (n > 0) {
+      B.uE[id] = s / float(n);
+      return;
+    }
+    // layer 2: neighbours valid after layer 1, with their layer-1 values
+    const int sxy = sx * sy;
+    float acc = 0.f;
+    int cnt = 0;
+    auto take = [&](int g, int gi, int gj, int gk) {
+      if (vm[g]) {
+        acc += u[g];
+        cnt++;
+        return;
+      }
+      float sg;
+      const int ng = layer1(u, vm, g, gi, gj, gk, sx, sy, sz, sg);
+      if (ng > 0) {
+        acc += sg / float(ng);
+        cnt++;
+      }
+    };
+    if (i > 0) take(f - 1, i - 1, j, k);
+    if (i < sx - 1) take(f + 1, i + 1, j, k);
+    if (j > 0) take(f - sx, i, j - 1, k);
+    if (j < sy - 1) take(f + sx, i, j + 1, k);
+    if (k > 0) take(f - sxy, i, j, k - 1);
+    if (k < sz - 1) take(f + sxy, i, j, k + 1);
+    B.uE[id] = cnt > 0 ? acc / float(cnt) : u[f];
+    return;
+  }
+  id -= facesPadded;
+  float ad = 0.f;
+  if (id < P.nc && B.flag[id] == PL3D_FLUID_) {
+    const int i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
+    if (i >= 1 && j >= 1 && k >= 1 && i <= nx - 2 && j <= ny - 2 && k <= nz - 2) {
+      const float *u = B.u, *v = B.u + P.nu, *w = B.u + P.nu + P.nv;
+      const float d = u[i + 1 + (nx + 1) * (j + ny * k)] - u[i + (nx + 1) * (j + ny * k)] +
+                      v[i + nx * (j + 1 + (ny + 1) * k)] - v[i + nx * (j + (ny + 1) * k)] +
+                      w[i + nx * (j + ny * (k + 1))] - w[i + nx * (j + ny * k)];
+      ad = fabsf(d);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) ad = fmaxf(ad, __shfl_xor(ad, o, 64));
+  if ((threadIdx.x & 63) == 0 && ad > 0.f) atomicMax(&B.ctl->maxDivBits, __float_as_uint(ad));
+}
+
+// ============================================================================= k_advect
+// gridToParticles (FLIP/PIC blend, solver3d.ts:607-624) fused with advect (:677-788): RK2
+// midpoint on the extrapolated field, density-correction displacement, SDF collision at the new
+// pose with the moving-wall normal velocity removed, symmetry reflection, LOST / DISCHARGED /
+// SPILLED.  The last block to finish closes the substep (time, steps, pose, max speed).
+__global__ __launch_bounds__(STEP_THREADS) void k_advect(DevBufs B, DevParams P) {
+  if (!is_active(B)) return;
+  DevCtl &C = *B.ctl;
+  __shared__ float smax[STEP_THREADS / 64];
+  __shared__ bool lastBlock;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const float h = P.h, invh = P.invh;
+  const float dt = C.fdt;
+  float speed2 = 0.f;
+  if (p < P.n) {
+    const uint8_t st = B.status[p];
+    if (st == ST_DISCHARGED || st == ST_SPILLED) {
+      if (B.y[p] > -1.f) {  // ballistic continuation (:693-701)
+        const float vy = B.vy[p] - float(P.gravity) * dt;
+        B.vy[p] = vy;
+        B.x[p] += B.vx[p] * dt;
+        B.y[p] += vy * dt;
+        B.z[p] += B.vz[p] * dt;
+      }
+    } else if (st == ST_IN_POT) {
+      const float *uE = B.uE, *vE = B.uE + P.nu, *wE = B.uE + P.nu + P.nv;
+      const float *uP = B.uPrev, *vP = B.uPrev + P.nu, *wP = B.uPrev + P.nu + P.nv;
+      const float *ucf = B.uc, *vcf = B.uc + P.nu, *wcf = B.uc + P.nu + P.nv;
+      const uint8_t *uV = B.valid, *vV = B.valid + P.nu, *wV = B.valid + P.nu + P.nv;
+      const float x0 = B.x[p], y0 = B.y[p], z0 = B.z[p];
+      const float gx = x0 * invh, gy = y0 * invh, gz = z0 * invh;
+      // G2P, FLIP blend with the pre-extrapolation mask (:614-622)
+      const float a = float(P.flipRatio);
+      const float un = sample_field(uE, uV, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz);
+      const float vn = sample_field(vE, vV, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz);
+      const float wn = sample_field(wE, wV, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1);
+      const float uo = sample_field(uP, uV, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz);
+      const float vo = sample_field(vP, vV, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz);
+      const float wo = sample_field(wP, wV, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1);
+      float pvx = (1 - a) * un + a * (B.vx[p] + un - uo);
+      float pvy = (1 - a) * vn + a * (B.vy[p] + vn - vo);
+      float pvz = (1 - a) * wn + a * (B.vz[p] + wn - wo);
+      // advect
+      const float c0 = C.c0, s0 = C.s0, c1 = C.c1, s1 = C.s1;
+      const float pwx = float(P.pivotWX), pwy = float(P.pivotWY);
+      const float pvtx = float(P.pivotX), pvty = float(P.pivotY);
+      float l0x, l0y;
+      {
+        const float rx = x0 - pwx, ry = y0 - pwy;
+        l0x = c0 * rx + s0 * ry + pvtx;
+        l0y = -s0 * rx + c0 * ry + pvty;
+      }
+      const float u1 = sample_field(uE, nullptr, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz);
+      const float v1 = sample_field(vE, nullptr, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz);
+      const float w1 = sample_field(wE, nullptr, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1);
+      const float xm = x0 + 0.5f * dt * u1, ym = y0 + 0.5f * dt * v1, zm = z0 + 0.5f * dt * w1;
+      const float gxm = xm * invh, gym = ym * invh, gzm = zm * invh;
+      float x1 = x0 + dt * sample_field(uE, nullptr, gxm, gym - 0.5f, gzm - 0.5f, nx + 1, ny, nz);
+      float y1 = y0 + dt * sample_field(vE, nullptr, gxm - 0.5f, gym, gzm - 0.5f, nx, ny + 1, nz);
+      float z1 = z0 + dt * sample_field(wE, nullptr, gxm - 0.5f, gym - 0.5f, gzm, nx, ny, nz + 1);
+      if (P.densityCorrection > 0) {
+        x1 += dt * sample_field(ucf, nullptr, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz);
+        y1 += dt * sample_field(vcf, nullptr, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz);
+        z1 += dt * sample_field(wcf, nullptr, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1);
+      }
+      // collision at the new pose (:722-755)
+      const float margin = 0.1f * h;
+      const float eps = 1e-4f;
+      float lx, ly, lz;
+      {
+        const float rx = x1 - pwx, ry = y1 - pwy;
+        lx = c1 * rx + s1 * ry + pvtx;
+        ly = -s1 * rx + c1 * ry + pvty;
+        lz = z1 - h;
+      }
+      const float d = teapot_sdf<float>(P, lx, ly, lz);
+      if (d < margin) {
+        const float gxs = teapot_sdf<float>(P, lx + eps, ly, lz) - teapot_sdf<float>(P, lx - eps, ly, lz);
+        const float gys = teapot_sdf<float>(P, lx, ly + eps, lz) - teapot_sdf<float>(P, lx, ly - eps, lz);
+        const float gzs = teapot_sdf<float>(P, lx, ly, lz + eps) - teapot_sdf<float>(P, lx, ly, lz - eps);
+        float gl = sqrtf(gxs * gxs + gys * gys + gzs * gzs);
+        if (!(gl > 0.f)) gl = 1.f;  // `|| 1`
+        const float push = margin - d;
+        lx += (gxs / gl) * push;
+        ly += (gys / gl) * push;
+        lz += (gzs / gl) * push;
+        {
+          const float rx = lx - pvtx, ry = ly - pvty;
+          x1 = c1 * rx - s1 * ry + pwx;
+          y1 = s1 * rx + c1 * ry + pwy;
+          z1 = lz + h;
+        }
+        const float nwx = (c1 * gxs - s1 * gys) / gl;
+        const float nwy = (s1 * gxs + c1 * gys) / gl;
+        const float nwz = gzs / gl;
+        const float om = C.fomega;
+        const float wvx = -om * (y1 - pwy);
+        const float wvy = om * (x1 - pwx);
+        const float rn = (pvx - wvx) * nwx + (pvy - wvy) * nwy + pvz * nwz;
+        if (rn < 0.f) {
+          pvx -= rn * nwx;
+          pvy -= rn * nwy;
+          pvz -= rn * nwz;
+        }
+      }
+      if (z1 < h) {  // symmetry plane (:757-760)
+        z1 = 2 * h - z1;
+        pvz = fabsf(pvz);
+      }
+      B.vx[p] = pvx;
+      B.vy[p] = pvy;
+      B.vz[p] = pvz;
+      uint8_t ns = ST_IN_POT;
+      if (!isfinite(x1 + y1 + z1)) {
+        ns = ST_LOST;
+      } else {
+        B.x[p] = x1;
+        B.y[p] = y1;
+        B.z[p] = z1;
+        {
+          const float rx = x1 - pwx, ry = y1 - pwy;
+          lx = c1 * rx + s1 * ry + pvtx;
+          ly = -s1 * rx + c1 * ry + pvty;
+          lz = z1 - h;
+        }
+        const float tipX = float(P.tipX), tipY = float(P.tipY), ddx = float(P.dx), ddy = float(P.dy);
+        const float sd0 = (l0x - tipX) * ddx + (l0y - tipY) * ddy;
+        const float sd1 = (lx - tipX) * ddx + (ly - tipY) * ddy;
+        bool discharged = false;
+        if (sd0 < 0.f && sd1 >= 0.f) {
+          const float rx = lx - tipX - sd1 * ddx, ry = ly - tipY - sd1 * ddy;
+          if (sqrtf(rx * rx + ry * ry + lz * lz) < float(P.boreRadius + 3 * P.wall)) {
+            ns = ST_DISCHARGED;
+            discharged = true;
+          }
+        }
+        if (!discharged) {
+          const float r = sqrtf(lx * lx + lz * lz);
+          const bool insideBody = r < float(P.radius + 2 * P.wall) && ly > float(-2 * P.wall);
+          const bool nearSpout = lx > 0.f && lx < float(P.tipX + 0.03);
+          const float lo = h * 1.01f;
+          const bool atEdge = x1 <= lo || y1 <= lo || x1 >= (nx - 1) * h || y1 >= (ny - 1) * h ||
+                              z1 >= (nz - 1) * h;
+          if ((!insideBody && !nearSpout) || atEdge) ns = ST_SPILLED;
+        }
+      }
+      if (ns != ST_IN_POT) {
+        B.status[p] = ns;
+        atomicSub(&C.counts[ST_IN_POT], 1);
+        atomicAdd(&C.counts[ns], 1);
+        if (ns == ST_DISCHARGED) {
+          atomicAdd(&C.byOrigin[B.origin[p]], 1);
+          B.dstep[p] = int(C.steps);
+        }
+      } else {
+        speed2 = pvx * pvx + pvy * pvy + pvz * pvz;
+      }
+    }
+  }
+  // block max of speed^2 over parcels still IN_POT (input of the next stableDt)
+  float m2 = speed2;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, o, 64));
+  if ((threadIdx.x & 63) == 0) smax[threadIdx.x >> 6] = m2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float bm = 0.f;
+    for (int w = 0; w < STEP_THREADS / 64; w++) bm = fmaxf(bm, smax[w]);
+    atomicMax(&C.stepMaxSpeedSqBits, __float_as_uint(bm));
+    __threadfence();
+    const unsigned int ticket = atomicAdd(&C.doneTicket, 1u);
+    lastBlock = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (lastBlock && threadIdx.x == 0) {  // close the substep (solver3d.ts:306-308)
+    __threadfence();
+    C.maxSpeedSqBits = atomicAdd(&C.stepMaxSpeedSqBits, 0u);
+    C.time += C.dt;
+    C.steps += 1;
+    C.phi = C.phi1;
+    C.lastM = C.m;
+    __threadfence();
+  }
+}
+
+// ============================================================================= render positions
+// writePositions (gpu3d/shaders.ts:779-801): xyz of the simulated half, then its mirror image
+// about the symmetry plane z = h.
+__global__ void k_render_positions(DevBufs B, DevParams P, float *out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P.n) return;
+  const float x = B.x[p], y = B.y[p], z = B.z[p];
+  out[3 * p] = x;
+  out[3 * p + 1] = y;
+  out[3 * p + 2] = z;
+  const int q = P.n + p;
+  out[3 * q] = x;
+  out[3 * q + 1] = y;
+  out[3 * q + 2] = 2 * P.h - z;
+}

@@ -1,0 +1,67 @@
+"""Full pour on the GPU, as benchmarks/run3d.ts:9-38 drives the CPU solver.
+
+Usage: python -m teapotfluidsimulation_amd.run3d [cellMm] [seconds] [--precond mic0|jacobi]
+Prints run3d.json-shaped JSON (result, dischargedFractions, wallSeconds, history).
+"""
+import argparse
+import json
+import sys
+import time
+
+from .sim3d import Simulation3D, gentlePour3D
+
+
+def run_pour(cell_mm=4.0, seconds=20.0, precond="mic0", progress=False, chunk=0.5):
+    cfg = gentlePour3D(cell_mm, seconds)
+    cfg["preconditioner"] = precond
+    sim = Simulation3D(cfg)
+    rows = []
+    done_at = -1.0
+    max_it = 0
+    t0 = time.perf_counter()
+    while not sim.finished:
+        d = sim.advance(chunk)
+        rows.append(d)
+        max_it = max(max_it, d["pressureIterations"])
+        if progress:
+            sys.stderr.write(
+                f"t={d['time']:.1f} {d['phase']} tilt={d['angleDeg']:.1f} out={d['dischargedMl']:.0f} "
+                f"spill={d['spilledMl']:.0f} lost={d['lostMl']:.0f} it={d['pressureIterations']} "
+                f"rho={d['interiorDensity']:.3f} wall={time.perf_counter() - t0:.1f}s\n")
+        if d["phase"] == "done" and done_at < 0:
+            done_at = d["time"]
+        if done_at >= 0 and d["time"] > done_at + 1.5:
+            break
+    wall = time.perf_counter() - t0
+    d = sim.diagnostics()
+    st = sim.stats()
+    out = {
+        "cellMm": cell_mm,
+        "parcelsHalfPot": sim.n,
+        "grid": d["grid"],
+        "simulatedSeconds": d["time"],
+        "steps": d["steps"],
+        "result": d,
+        "dischargedFractions": sim.dischargedFractions(),
+        "dischargedByOrigin": st["dischargedByOrigin"],
+        "maxPressureIterations": max_it,
+        "solverOverflow": st["solverOverflow"],
+        "wallSeconds": wall,
+        "realTimeFactor": d["time"] / wall,
+        "history": rows,
+    }
+    sim.destroy()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cell", nargs="?", type=float, default=4.0)
+    ap.add_argument("seconds", nargs="?", type=float, default=20.0)
+    ap.add_argument("--precond", default="mic0")
+    a = ap.parse_args()
+    print(json.dumps(run_pour(a.cell, a.seconds, a.precond, progress=True), indent=1))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,237 @@
+"""Host mirror of the reference's 3D solver API over libpourlab3d.so.
+
+Names, argument meanings and error behaviour follow src/sim3d/solver3d.ts, src/sim3d/presets3d.ts
+and src/gpu3d/solver.ts, so a driver written like benchmarks/run3d.ts reads the same:
+
+    sim = Simulation3D(gentlePour3D(4, 20))
+    while not sim.finished:
+        d = sim.advance(0.5)
+
+Every call runs on the GPU through the C ABI; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import check, load
+
+IN_POT, DISCHARGED, SPILLED, LOST = abi.IN_POT, abi.DISCHARGED, abi.SPILLED, abi.LOST
+AIR, FLUID, SOLID = abi.AIR, abi.FLUID, abi.SOLID
+
+
+def gentlePour3D(cellMm: float, seconds: float) -> dict:
+    """gentlePour3D (presets3d.ts:9-26) as a Config3D dict (camelCase keys)."""
+    c = abi.Config()
+    check(load().pl3d_gentle_pour(float(cellMm), float(seconds), C.byref(c)))
+    return abi.config_to_dict(c)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Simulation3D:
+    """Simulation3D (solver3d.ts:62-824) / GpuSimulation3D (gpu3d/solver.ts:89-490)."""
+
+    def __init__(self, cfg: dict):
+        self.cfg = dict(cfg)
+        self._c = abi.config_from_dict(cfg)
+        h = C.c_void_p()
+        check(load().pl3d_create(C.byref(self._c), C.byref(h)))
+        self._h = h
+        info = abi.Info()
+        check(load().pl3d_get_info(self._h, C.byref(info)))
+        self.info = info
+        self.n = info.n
+        self.nx, self.ny, self.nz = info.nx, info.ny, info.nz
+        self.h = info.h
+        self.fillHeight = info.fill_height
+        self.pivotWorld = [info.pivot_world[0], info.pivot_world[1]]
+        self.parcelVolume = info.parcel_volume
+        self.bandCounts = list(info.band_counts)
+
+    # ------------------------------------------------------------------ lifecycle
+    def destroy(self):
+        """Free the device memory (GpuSimulation3D.destroy)."""
+        if getattr(self, "_h", None):
+            check(load().pl3d_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _handle(self):
+        if not getattr(self, "_h", None):
+            raise RuntimeError("simulation was destroyed")
+        return self._h
+
+    # ------------------------------------------------------------------ stepping
+    @property
+    def finished(self) -> bool:
+        v = C.c_int32()
+        check(load().pl3d_finished(self._handle(), C.byref(v)))
+        return bool(v.value)
+
+    def stableDt(self) -> float:
+        v = C.c_double()
+        check(load().pl3d_stable_dt(self._handle(), C.byref(v)))
+        return v.value
+
+    def step(self, dt: float):
+        check(load().pl3d_step(self._handle(), float(dt)))
+
+    def advance(self, duration: float) -> dict:
+        d = abi.Diagnostics()
+        check(load().pl3d_advance(self._handle(), float(duration), C.byref(d)))
+        return abi.diagnostics_dict(d)
+
+    def diagnostics(self) -> dict:
+        return abi.diagnostics_dict(self.raw_diagnostics())
+
+    def raw_diagnostics(self) -> abi.Diagnostics:
+        d = abi.Diagnostics()
+        check(load().pl3d_get_diagnostics(self._handle(), C.byref(d)))
+        return d
+
+    @property
+    def time(self) -> float:
+        return self.raw_diagnostics().time
+
+    @property
+    def steps(self) -> int:
+        return int(self.raw_diagnostics().steps)
+
+    def dischargedFractions(self):
+        out = (C.c_double * 3)()
+        has = C.c_int32()
+        check(load().pl3d_discharged_fractions(self._handle(), out, C.byref(has)))
+        return list(out) if has.value else None
+
+    def stats(self) -> dict:
+        """GpuStats (gpu3d/solver.ts:24-47) of the last substep."""
+        raw = (C.c_int32 * abi.STAT_SLOTS)()
+        check(load().pl3d_stats(self._handle(), raw))
+        r = np.frombuffer(bytes(raw), np.int32)
+        f = np.frombuffer(bytes(raw), np.float32)
+        ic = int(r[abi.STAT["interiorCount"]])
+        return {
+            "counts": [int(r[k]) for k in range(4)],
+            "dischargedByOrigin": [int(r[4]), int(r[5]), int(r[6])],
+            "pressureFailures": int(r[7]),
+            "maxSpeedSq": float(f[8]),
+            "maxDivergence": float(f[9]),
+            "interiorDensity": (r[11] / abi.SCALE_RATIO / ic) if ic > 0 else 1.0,
+            "pressureIterations": int(r[12]),
+            "correctionIterations": int(r[13]),
+            "liquidCells": int(r[14]),
+            "solverOverflow": int(r[15]),
+        }
+
+    # ------------------------------------------------------------------ parcel fields
+    def state(self) -> dict:
+        """x, y, z, vx, vy, vz (FP64 copies of the device FP32 state), status, origin, y0."""
+        n = self.n
+        f = {k: np.zeros(n, np.float64) for k in ("x", "y", "z", "vx", "vy", "vz")}
+        st = np.zeros(n, np.uint8)
+        org = np.zeros(n, np.uint8)
+        y0 = np.zeros(n, np.float32)
+        check(load().pl3d_read_state(self._handle(),
+                                     *[_ptr(f[k]) for k in ("x", "y", "z", "vx", "vy", "vz")],
+                                     _ptr(st), _ptr(org), _ptr(y0)))
+        f.update(status=st, origin=org, y0=y0)
+        return f
+
+    @property
+    def x(self):
+        return self.state()["x"]
+
+    @property
+    def y(self):
+        return self.state()["y"]
+
+    @property
+    def z(self):
+        return self.state()["z"]
+
+    @property
+    def status(self):
+        return self.state()["status"]
+
+    @property
+    def origin(self):
+        return self.state()["origin"]
+
+    @property
+    def dischargeOrder(self) -> np.ndarray:
+        cnt = C.c_int32()
+        check(load().pl3d_discharge_order(self._handle(), None, 0, C.byref(cnt)))
+        out = np.zeros(max(cnt.value, 1), np.int32)
+        check(load().pl3d_discharge_order(self._handle(),
+                                          out.ctypes.data_as(C.POINTER(C.c_int32)), cnt.value,
+                                          C.byref(cnt)))
+        return out[:cnt.value]
+
+    def readParticles(self) -> dict:
+        """readParticles (gpu3d/solver.ts:420-437): pos/vel float4, status uint32."""
+        n = self.n
+        pos = np.zeros(4 * n, np.float32)
+        vel = np.zeros(4 * n, np.float32)
+        st = np.zeros(n, np.uint32)
+        check(load().pl3d_read_particles(self._handle(), _ptr(pos), _ptr(vel), _ptr(st)))
+        return {"pos": pos, "vel": vel, "status": st}
+
+    def readRenderPositions(self) -> np.ndarray:
+        out = np.zeros(6 * self.n, np.float32)
+        check(load().pl3d_read_render_positions(self._handle(), _ptr(out)))
+        return out
+
+    def setTarget(self, angleDeg: float):
+        """Manual tilt target (PourController.setTarget, motion.ts:40-44)."""
+        check(load().pl3d_set_target(self._handle(), float(angleDeg)))
+
+    # ------------------------------------------------------------------ measurement
+    def set_timing(self, enabled: bool):
+        check(load().pl3d_set_timing(self._handle(), 1 if enabled else 0))
+
+    def kernel_times(self) -> dict:
+        cap = 32
+        names = (C.c_char_p * cap)()
+        ms = (C.c_double * cap)()
+        launches = (C.c_int64 * cap)()
+        cnt = C.c_int32()
+        check(load().pl3d_kernel_times(self._handle(), cap, names, ms, launches, C.byref(cnt)))
+        return {names[i].decode(): (ms[i], int(launches[i])) for i in range(cnt.value)}
+
+
+class Pcg3D:
+    """Pcg3D (solver3d.ts:827-974): matrix-free 7-point PCG on the FLUID cells, on the GPU."""
+
+    def __init__(self, nx: int, ny: int, nz: int, preconditioner: str = "mic0"):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.preconditioner = preconditioner
+
+    def solve(self, cell, b, q, tol: float, maxIter: int) -> dict:
+        """Fills q (FP64 array of nx*ny*nz) and returns {iterations, converged, residual,
+        trueResidual, liquidCells}."""
+        cell = np.ascontiguousarray(cell, np.uint8)
+        b = np.ascontiguousarray(b, np.float64)
+        n = self.nx * self.ny * self.nz
+        if cell.size != n or b.size != n or q.size != n:
+            raise ValueError("cell, b and q must hold nx*ny*nz entries")
+        qq = np.zeros(n, np.float64)
+        res = abi.SolveResult()
+        check(load().pl3d_pcg_solve(self.nx, self.ny, self.nz, _ptr(cell), _ptr(b), _ptr(qq),
+                                    float(tol), int(maxIter),
+                                    abi.PRECONDITIONERS[self.preconditioner], C.byref(res)))
+        q[...] = qq.reshape(q.shape)
+        return {
+            "iterations": res.iterations,
+            "converged": bool(res.converged),
+            "residual": res.residual,
+            "trueResidual": res.true_residual,
+            "liquidCells": res.liquid_cells,
+        }

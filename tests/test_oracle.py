@@ -1,0 +1,267 @@
+"""The FP64 CPU oracle, pinned against what the reference itself holds.
+
+Pins (reference files): grid sizes and parcel counts (benchmarks/run3d.json:7-12, :28-33); the
+CPU state after 200 substeps at 5 mm (benchmarks/gpu3d.json:14-33); the recorded 5 mm pour
+(public/data/pour3d.bin.gz: pivotWorld, domain, per-frame tilt angle and poured mL per layer,
+quantised positions of every 3rd parcel); the unit known answers of tests/sim3d.test.ts; closed
+forms (fill height, the eased tilt-in curve, A q = b against a dense solve).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle3d as O
+from tests.conftest import quantise
+
+
+def test_grid_and_parcels(pins):
+    for row in pins["run3d_rows"]:
+        s = O.OracleSim(O.gentle_pour(row["cellMm"], 20))
+        assert [s.nx, s.ny, s.nz] == row["grid"]
+        assert s.n == row["parcelsHalfPot"]
+        assert max(s.bandCounts) - min(s.bandCounts) <= 1
+
+
+def test_fill_height_closed_form():
+    s = O.OracleSim(O.gentle_pour(8, 1))
+    assert s.fillHeight == pytest.approx(1.5e-3 / (math.pi * 0.065 ** 2), rel=1e-15)
+    assert s.parcelVolume * 2 * s.n == pytest.approx(1.5e-3, rel=1e-12)
+
+
+def test_recording_header_and_frame0(recording):
+    header, rec = recording
+    s = O.OracleSim(O.gentle_pour(5, 14))
+    assert s.pivotWorld == header["pivotWorld"]  # exact doubles
+    assert [s.nx * s.h, s.ny * s.h, s.nz * s.h] == pytest.approx(header["domain"], rel=1e-15)
+    assert s.n == header["simulatedParcels"]
+    keep = np.arange(0, s.n, 3)
+    st = s.state()
+    # origin bands of the recorded parcels (Float32 y0 sort, ties by index)
+    assert np.array_equal(st["origin"][keep], rec["origin"])
+    # seeded positions (mulberry32 jitter, rejection by inCavity / teapotSdf)
+    qq = quantise(st["x"][keep], st["y"][keep], st["z"][keep], header["quantum"])
+    assert np.array_equal(qq, rec["positions"][0].astype(np.int64))
+
+
+def test_200_substeps_and_recording_frames(pins, recording):
+    """CPU block after 200 substeps (t = 0.907 s) bit-exact, then frames 1..10 of the recording."""
+    cpu = pins["gpu3d_step_comparison"]["cpu"]
+    s = O.OracleSim(O.gentle_pour(5, 20))
+    for _ in range(200):
+        s.step(min(s.stableDt(), s.cfg.duration - s.diagnostics()["time"]))
+    d = s.diagnostics()
+    for key in ("time", "angleDeg", "maxDivergence", "interiorDensity", "inPotMl"):
+        assert d[key] == cpu[key], key
+    for key in ("steps", "phase", "pressureIterations", "pressureFailures", "grid", "particles"):
+        assert d[key] == cpu[key], key
+
+    header, rec = recording
+    s = O.OracleSim(O.gentle_pour(5, 14))
+    keep = np.arange(0, s.n, 3)
+    frames = rec["frames"]
+    for f in range(1, 11):
+        s.advance(0.1)
+        assert s.diagnostics()["angleDeg"] == pytest.approx(header["angles"][f], abs=1e-12)
+        if f in frames:
+            st = s.state()
+            qq = quantise(st["x"][keep], st["y"][keep], st["z"][keep], header["quantum"])
+            ref = rec["positions"][frames.index(f)].astype(np.int64)
+            assert np.array_equal(qq, ref), f"frame {f}"
+
+
+@pytest.mark.slow
+def test_first_discharge_matches_recording(recording):
+    """Frames 11-22 (t = 1.1-2.24 s): the poured mL per layer of the first discharges (advect's
+    spout-plane test, solver3d.ts:769-781) match the recording exactly; the tilt angle to 1e-6 deg
+    (from ~frame 15 the substep lengths differ from V8's in the last bits, libm-level)."""
+    header, rec = recording
+    s = O.OracleSim(O.gentle_pour(5, 14))
+    for _ in range(10):  # the recording's chunks (scripts/record3d.ts:40-42)
+        s.advance(0.1)
+    ml = s.parcelVolume * 2e6
+    for f in range(11, 23):
+        s.advance(0.1)
+        assert s.diagnostics()["angleDeg"] == pytest.approx(header["angles"][f], abs=1e-6)
+        st = s.state()
+        order = s.dischargeOrder()
+        poured = [int((st["origin"][order] == k).sum()) * ml for k in range(3)]
+        assert poured == pytest.approx(header["poured"][f], abs=1e-9), f"frame {f}"
+    assert sum(header["poured"][22]) > 5.0  # the pour has started
+
+
+def test_geometry_known_answers():
+    """tests/sim3d.test.ts:28-58."""
+    t = O.default_teapot()
+    c = t
+    assert O.teapot_sdf(t, 0, 0.05, 0) > 0.03
+    assert O.teapot_sdf(t, -(c.radius + c.wall / 2), 0.08, 0) < 0
+    assert O.teapot_sdf(t, 0, -c.wall / 2, 0) < 0
+    assert O.teapot_sdf(t, 0, c.height + 0.02, 0) > 0
+    assert O.in_cavity(t, 0, 0.05, 0)
+    assert not O.in_cavity(t, 0, -0.01, 0)
+    p = O.build_teapot(t)
+    s = 0.0
+    while s < c.spout_length - 0.005:
+        x, y = p["ax"] + s * p["dx"], p["ay"] + s * p["dy"]
+        assert O.teapot_sdf(t, x, y, 0) > 0
+        assert O.in_cavity(t, x, y, 0)
+        s += 0.01
+    s = 0.08
+    off = c.bore_radius + c.wall / 2
+    assert O.teapot_sdf(t, p["ax"] + s * p["dx"] - p["dy"] * off,
+                        p["ay"] + s * p["dy"] + p["dx"] * off, 0) < 0
+    assert O.teapot_sdf(t, c.radius + c.wall / 2, 0.11, 0) < 0
+    assert p["tipY"] - c.bore_radius > 1.5e-3 / (math.pi * c.radius ** 2)
+    # closed form: spout axis
+    assert p["tipX"] == pytest.approx(0.057 + math.cos(math.pi / 4) * 0.14, abs=1e-15)
+
+
+def _column(n=8):
+    cell = np.zeros(n ** 3, np.uint8)
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                c = i + n * (j + n * k)
+                border = i in (0, n - 1) or j in (0, n - 1) or k in (0, n - 1)
+                cell[c] = 2 if border else (1 if j < 5 else 0)
+    b = np.where(cell == 1, np.sin(np.arange(n ** 3)), 0.0)
+    return cell, b
+
+
+def test_pcg_column_known_answers():
+    """tests/sim3d.test.ts:60-80, plus A q = b checked in FP64."""
+    n = 8
+    cell, b = _column(n)
+    q, r = O.pcg_solve(n, n, n, cell, b, 1e-8, 200)
+    assert r.converged and r.iterations < 40
+    assert r.true_residual <= 1e-8 * np.abs(b).max() * 1.0001
+    q0, r0 = O.pcg_solve(n, n, n, cell, np.zeros(n ** 3), 1e-8, 10)
+    assert r0.iterations == 0 and r0.converged
+    _, r1 = O.pcg_solve(n, n, n, cell, b, 1e-14, 1)
+    assert not r1.converged and r1.iterations == 1
+
+
+def _dense_laplacian(nx, ny, nz, cell):
+    """The 7-point operator as a dense matrix over FLUID cells (AIR neighbours: p = 0 ghost,
+    SOLID: no flux), built from its definition."""
+    liquid = np.flatnonzero(cell == 1)
+    index = {c: t for t, c in enumerate(liquid)}
+    A = np.zeros((len(liquid), len(liquid)))
+    for t, c in enumerate(liquid):
+        for o in (-1, 1, -nx, nx, -nx * ny, nx * ny):
+            if cell[c + o] != 2:
+                A[t, t] += 1
+            if cell[c + o] == 1:
+                A[t, index[c + o]] -= 1
+    return liquid, A
+
+
+def test_pcg_against_dense_solve():
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 9, 8, 7
+    cell = np.full(nx * ny * nz, 2, np.uint8)
+    inner = np.zeros((nz, ny, nx), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    flags = np.where(rng.random((nz, ny, nx)) < 0.75, 1, 0)
+    flags[:, 4:, :] = np.where(rng.random((nz, ny - 4, nx)) < 0.3, 1, 0)
+    flags[3, 3, 3] = 2
+    cell = np.where(inner, flags, 2).astype(np.uint8).ravel()
+    b = np.where(cell == 1, rng.standard_normal(cell.size), 0.0)
+    liquid, A = _dense_laplacian(nx, ny, nz, cell)
+    assert np.all(np.linalg.eigvalsh(A) > 0)  # SPD (every liquid component touches air)
+    exact = np.linalg.solve(A, b[liquid])
+    q, r = O.pcg_solve(nx, ny, nz, cell, b, 1e-12, 400)
+    assert r.converged
+    np.testing.assert_allclose(q[liquid], exact, rtol=0, atol=1e-9 * np.abs(exact).max())
+    assert np.all(q[cell != 1] == 0)
+    # the oracle's operator agrees with the dense one
+    np.testing.assert_allclose(O.apply_laplacian(nx, ny, nz, cell, q)[liquid], A @ q[liquid],
+                               atol=1e-12)
+
+
+def test_mic0_single_cell_by_hand():
+    """One FLUID cell with 5 non-solid neighbours: MIC(0) is exact (e = diag), PCG converges in
+    one iteration to q = b / 5."""
+    n = 5
+    cell = np.full(n ** 3, 2, np.uint8)
+    inner = [i + n * (j + n * k) for k in range(1, 4) for j in range(1, 4) for i in range(1, 4)]
+    cell[inner] = 0
+    c = 2 + n * (2 + n * 2)
+    cell[c] = 1
+    cell[c - n * n] = 2  # one solid neighbour below in z
+    b = np.zeros(n ** 3)
+    b[c] = 3.0
+    q, r = O.pcg_solve(n, n, n, cell, b, 1e-12, 10)
+    assert r.iterations == 1 and r.converged
+    assert q[c] == pytest.approx(3.0 / 5.0, rel=1e-15)
+
+
+def test_controller_tilt_in_closed_form():
+    """motion.ts:121-124 with no discharge: eased tilt to 22 deg over 2 s after 0.5 s."""
+    m = O.default_motion()
+    c = O.Controller(m)
+    t, dt = 0.0, 0.004
+    while t < 2.4:
+        t += dt
+        a = c.advance(t, dt, 0.0)
+        if t < 0.5:
+            assert a == 0.0 and c.phase == "wait"
+        else:
+            s = min(1.0, (t - 0.5) / 2.0)
+            expect = 22 * math.pi / 180 * 0.5 * (1 - math.cos(math.pi * s))
+            assert a == pytest.approx(expect, abs=1e-15)
+            assert c.phase == "tilt-in"
+    # hold ramp +3 deg/s, then the return triggers once the target volume is reached
+    for _ in range(50):
+        t += dt
+        c.advance(t, dt, 0.0)
+    assert c.phase == "hold"
+    assert c.angleDeg == pytest.approx(22 + 3 * (t - 2.5), abs=1e-9)
+    t += dt
+    c.advance(t, dt, 401e-6)
+    assert c.phase == "return"
+
+
+def _cfg8(**motion):
+    c = O.gentle_pour(8, 0.3)
+    c.pressure_max_iterations = 300
+    c.motion.kind = 0  # still
+    c.motion.target_volume = 0
+    for k, v in motion.items():
+        setattr(c.motion, k, v)
+    return c
+
+
+def test_rest_and_tilt_behaviour():
+    """tests/sim3d.test.ts:83-124 on the oracle: rest, steep-tilt pour, chunk invariance."""
+    s = O.OracleSim(_cfg8())
+    d = s.advance(0.3)
+    assert d["inPotMl"] == pytest.approx(1500, abs=1e-6)
+    assert d["dischargedMl"] + d["spilledMl"] + d["lostMl"] == 0
+    assert d["pressureFailures"] == 0
+    assert s.dischargedFractions() is None
+    assert np.all(s.state()["z"] >= s.h)
+
+    def tilt():
+        c = _cfg8(kind=1, start_delay=0, tilt_duration=0.4, hold_angle_deg=60, max_angle_deg=60,
+                  max_hold=5, target_volume=400e-6)
+        c.duration = 1.6
+        return O.OracleSim(c)
+
+    a, b = tilt(), tilt()
+    while not a.finished:
+        a.advance(0.4)
+    while not b.finished:
+        b.advance(0.8)
+    d = a.diagnostics()
+    assert d["dischargedMl"] > 20
+    assert d["lostMl"] == 0
+    assert d["inPotMl"] + d["dischargedMl"] + d["spilledMl"] + d["lostMl"] == pytest.approx(1500)
+    assert sum(a.dischargedFractions()) == pytest.approx(1)
+    sa = a.state()
+    order = a.dischargeOrder()
+    assert np.all(sa["status"][order] == 1)
+    assert sa["y"][order].min() < a.pivotWorld[1]
+    assert np.array_equal(sa["x"], b.state()["x"])
+    assert np.any(sa["status"] == 0)
