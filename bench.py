@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark: the 4 mm gentle pour (gentlePour3D(4, 20), 88,856 parcels, grid 81x81x22) on one
+MI355X, reported in simulated seconds per wall second.
+
+A "step" is one advance(0.5) chunk of the pour, the unit of the reference's driver loop
+(benchmarks/run3d.ts:15-16); the default --steps 24 covers the whole pour (it ends about 11.5 s
+simulated, 1.5 s after the controller is done).  Parcels, fields and the controller live in HBM
+from creation on; the timed region is the replay of the captured substeps plus one small
+control-block readback per batch.
+
+After the timed region an identical pour is replayed with per-kernel HIP events on the
+simulation's stream (timing mode, direct launches) to give each kernel's average duration; the
+roofline object is computed for the kernel with the largest total time.  The CPU baseline is the
+FP64 C++ oracle, single core, on a bounded sample of the same pour.
+
+Usage: python bench.py [--gpus 1] [--steps 24] [--warmup 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per liquid cell per PCG iteration (FP32 storage): MIC(0) = stencil 9 + vector
+# updates and max|r| 24 + forward sweep 13 + backward sweep and z.r 17 + s update 12; Jacobi drops
+# the sweeps and reads r, diag and writes z (9).
+BYTES_PER_CELL_ITER = {"mic0": 75.0, "jacobi": 54.0}
+# Published reference figure for this metric: the reference's WebGPU FP32 4 mm pour, 0.87x real
+# time on an RTX A2000 Laptop (benchmarks/gpu3d.json:97-137).
+REFERENCE_WEBGPU_4MM = 0.87
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(cell, sample_s):
+    """The FP64 oracle on one pinned core for about sample_s wall seconds of the same pour."""
+    cmd = [sys.executable, "-m", "oracle.run3d_oracle", str(cell), "20", "--max-wall",
+           str(sample_s)]
+    try:
+        cmd = ["taskset", "-c", "0"] + cmd
+        r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=sample_s * 4 + 120)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        return {"error": str(e)}
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    out = json.loads(r.stdout)
+    d = out["result"]
+    model, ncpu = cpu_info()
+    return {
+        "value": d["time"] / out["wallSeconds"],
+        "unit": "sim-s/wall-s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"first {d['steps']} substeps ({d['time']:.3f} simulated s) of the same "
+                  f"{cell} mm pour, FP64 C++ oracle pinned to one core, "
+                  f"{out['wallSeconds']:.1f} s wall; host {model}, {ncpu} CPUs",
+        "reference_published": "reference TypeScript FP64 CPU: 0.007055 sim-s/wall-s (4 mm, "
+                               "i9-12900H, 1 core, benchmarks/run3d.json:34,45)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cell", type=float, default=4.0)
+    ap.add_argument("--precond", default="mic0")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=float, default=20.0)
+    ap.add_argument("--no-kernel-replay", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs one pour on one GPU; --gpus {a.gpus} is not supported "
+              "(independent pours shard one per GPU with no exchange)", file=sys.stderr)
+        return 2
+
+    from teapotfluidsimulation_amd import Simulation3D, gentlePour3D
+
+    def make():
+        cfg = gentlePour3D(a.cell, 20)
+        cfg["preconditioner"] = a.precond
+        return Simulation3D(cfg)
+
+    # warmup: a throwaway pour (graph capture, first-touch, clocks)
+    warm = make()
+    for _ in range(a.warmup):
+        warm.advance(0.5)
+    warm.destroy()
+
+    sim = make()
+    sim.diagnostics()  # synchronised
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        d = sim.advance(0.5)
+    wall = time.perf_counter() - t0
+    sim_s = d["time"]
+    substeps = d["steps"]
+    cnt = sim.counters()
+    result = {
+        "dischargedMl": d["dischargedMl"],
+        "dischargedFractions": sim.dischargedFractions(),
+        "spilledMl": d["spilledMl"],
+        "lostMl": d["lostMl"],
+        "pressureFailures": d["pressureFailures"],
+        "phase": d["phase"],
+    }
+    sim.destroy()
+
+    roofline = None
+    kernels = {}
+    pressure_ms = None
+    if not a.no_kernel_replay:
+        rep = make()
+        rep.set_timing(True)
+        for _ in range(a.steps):
+            rep.advance(0.5)
+        kt = rep.kernel_times()
+        rc = rep.counters()
+        rep.destroy()
+        kernels = {k: {"total_ms": v[0], "launches": v[1],
+                       "avg_us": 1000 * v[0] / max(v[1], 1)} for k, v in kt.items()}
+        dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+        launches = kernels[dom]["launches"]
+        avg_s = kernels[dom]["total_ms"] / 1000 / max(launches, 1)
+        if "k_pcg" in kernels:
+            pressure_ms = kernels["k_pcg"]["total_ms"] / max(kernels["k_pcg"]["launches"], 1)
+        if dom == "k_pcg":
+            bytes_per_launch = BYTES_PER_CELL_ITER[a.precond] * rc["cellIterations"] / launches
+            unit_note = (f"{BYTES_PER_CELL_ITER[a.precond]:.0f} B per liquid cell per PCG "
+                         f"iteration x {rc['cellIterations'] / launches:.0f} cell-iterations per "
+                         "launch (pressure + correction solves)")
+        else:
+            bytes_per_launch = None
+            unit_note = "no byte model for this kernel"
+        if bytes_per_launch is not None:
+            achieved = bytes_per_launch / avg_s / 1e9
+            roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch":
+                        bytes_per_launch, "bytes_model": unit_note}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(int(a.cell) if a.cell == int(a.cell) else a.cell, a.cpu_sample)
+
+    value = sim_s / wall
+    line = {
+        "metric": "sim-s per wall-s (4 mm pour) + pressure ms/substep at 1e-6 resid, learned vs PCG",
+        "value": value,
+        "unit": "sim-s/wall-s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1000 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REFERENCE_WEBGPU_4MM,
+        "dtype": "f32",
+        "data": "synthetic: gentlePour3D seeded parcels (mulberry32 seed 1), no external data",
+        "config": {
+            "workload": f"gentle 400 mL pour, {a.cell:g} mm cells, 88,856 parcels (half pot), "
+                        "grid 81x81x22, PIC/FLIP 0.95, pressure 1e-6 / correction 1e-4",
+            "step": "advance(0.5 s)",
+            "preconditioner": a.precond,
+            "precision": "FP32 fields and parcels, FP64 dot products and reliable-update refinement",
+            "baseline": "reference WebGPU FP32 4 mm pour, 0.87 sim-s/wall-s (RTX A2000 Laptop)",
+        },
+        "simulated_s": sim_s,
+        "substeps": substeps,
+        "wall_s": wall,
+        "us_per_substep": 1e6 * wall / max(substeps, 1),
+        "solve_ms_per_substep": pressure_ms,
+        "mean_liquid_cells": cnt["liquidCellSum"] / max(cnt["substeps"], 1),
+        "result": result,
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

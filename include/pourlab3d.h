@@ -262,6 +262,14 @@ int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, cons
 int pl3d_set_timing(pl3d_sim *sim, int32_t enabled);
 int pl3d_kernel_times(pl3d_sim *sim, int32_t cap, const char **names, double *total_ms,
                       int64_t *launches, int32_t *count);
+/* Cumulative counters since creation: [0] substeps, [1] sum over solves of liquid cells x
+ * iterations, [2] solves that iterated, [3] sum over substeps of liquid cells, [4] liquid cells of
+ * the last substep, [5] solver capacity overflow flag, [6] cells, [7] faces.  Measurement only. */
+int pl3d_counters(pl3d_sim *sim, int64_t out[8]);
+/* Device-to-device streaming copy of `bytes` (multiple of 16), timed over reps launches: the
+ * measured HBM ceiling and the calibration kernel for per-dispatch byte counters.  No counterpart
+ * in the reference (measurement only). */
+int pl3d_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,8 @@ SIGNATURES = {
     "pl3d_set_timing": (C.c_int, [_vp, _i32]),
     "pl3d_kernel_times": (C.c_int, [_vp, _i32, _P(C.c_char_p), _P(_d), _P(C.c_int64),
                                     _P(_i32)]),
+    "pl3d_copy_probe": (C.c_int, [C.c_int64, _i32, _P(_d)]),
+    "pl3d_counters": (C.c_int, [_vp, _P(C.c_int64)]),
 }
 
 _lib = None

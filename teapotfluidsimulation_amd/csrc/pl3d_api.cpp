@@ -114,6 +114,11 @@ struct KernelTimer {
   int64_t launches = 0;
 };
 
+struct PendingTiming {
+  int idx;
+  hipEvent_t a, b;
+};
+
 }  // namespace
 
 int pl3d_set_error(int code, const std::string &msg) {
@@ -135,6 +140,8 @@ struct pl3d_sim {
   bool timing = false;
   std::vector<KernelTimer> timers;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> eventPool;   // timing mode: reused start/end events
+  std::vector<PendingTiming> pending;  // recorded, not yet harvested
   float *render = nullptr;    // 6 n floats, allocated on the first readRenderPositions
   std::vector<float> y0;      // Float32 local y at seeding (solver3d.ts:83, :218)
   bool destroyed = false;
@@ -144,9 +151,49 @@ struct pl3d_sim {
 
 namespace {
 
+// timing mode: harvest the event pairs recorded since the last synchronisation
+int harvest_timing(pl3d_sim *s) {
+  for (auto &p : s->pending) {
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, p.a, p.b));
+    s->timers[p.idx].ms += ms;
+    s->timers[p.idx].launches++;
+    s->eventPool.push_back(p.a);
+    s->eventPool.push_back(p.b);
+  }
+  s->pending.clear();
+  return PL3D_OK;
+}
+
 int sync_ctl(pl3d_sim *s) {
   HIP_OK(hipMemcpyAsync(s->hctl, s->B.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipStreamSynchronize(s->stream));
+  return harvest_timing(s);
+}
+
+int take_event(pl3d_sim *s, hipEvent_t *ev) {
+  if (s->eventPool.empty()) {
+    HIP_OK(hipEventCreate(ev));
+  } else {
+    *ev = s->eventPool.back();
+    s->eventPool.pop_back();
+  }
+  return PL3D_OK;
+}
+
+// timing mode: bracket one launch with a pair of events on the simulation's stream
+int timed_begin(pl3d_sim *s, hipEvent_t *a) {
+  if (!s->timing) return PL3D_OK;
+  if (int rc = take_event(s, a)) return rc;
+  HIP_OK(hipEventRecord(*a, s->stream));
+  return PL3D_OK;
+}
+int timed_end(pl3d_sim *s, int idx, hipEvent_t a) {
+  if (!s->timing) return PL3D_OK;
+  hipEvent_t b;
+  if (int rc = take_event(s, &b)) return rc;
+  HIP_OK(hipEventRecord(b, s->stream));
+  s->pending.push_back(PendingTiming{idx, a, b});
   return PL3D_OK;
 }
 
@@ -166,18 +213,11 @@ int push_command(pl3d_sim *s, int mode, int pending, double endTime, double dt) 
 
 template <class K>
 int launch(pl3d_sim *s, int idx, K kernel, int grid, int block) {
-  if (s->timing) HIP_OK(hipEventRecord(s->ev0, s->stream));
+  hipEvent_t a = nullptr;
+  if (int rc = timed_begin(s, &a)) return rc;
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, s->B, s->P);
   HIP_OK(hipGetLastError());
-  if (s->timing) {
-    HIP_OK(hipEventRecord(s->ev1, s->stream));
-    HIP_OK(hipEventSynchronize(s->ev1));
-    float ms = 0;
-    HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-    s->timers[idx].ms += ms;
-    s->timers[idx].launches++;
-  }
-  return PL3D_OK;
+  return timed_end(s, idx, a);
 }
 
 // One substep's launches, in order (Simulation3D.step, solver3d.ts:288-309).
@@ -200,18 +240,14 @@ int enqueue_substep(pl3d_sim *s) {
     A.planes = s->P.nx + s->P.ny + s->P.nz - 2;
     A.precond = s->cfg.preconditioner == PL3D_PRECOND_JACOBI ? 2 : 0;
     A.overflow = &s->B.ctl->overflow;
-    if (s->timing) HIP_OK(hipEventRecord(s->ev0, s->stream));
+    A.cellIterations = &s->B.ctl->cellIterations;
+    A.solveCount = &s->B.ctl->solveCount;
+    hipEvent_t a = nullptr;
+    if ((rc = timed_begin(s, &a))) return rc;
     hipLaunchKernelGGL(k_pcg, dim3(2), dim3(PL3D_SOLVE_THREADS), 0, s->stream, A, s->B.descs,
                        &s->B.ctl->active);
     HIP_OK(hipGetLastError());
-    if (s->timing) {
-      HIP_OK(hipEventRecord(s->ev1, s->stream));
-      HIP_OK(hipEventSynchronize(s->ev1));
-      float ms = 0;
-      HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-      s->timers[6].ms += ms;
-      s->timers[6].launches++;
-    }
+    if ((rc = timed_end(s, 6, a))) return rc;
   }
   if ((rc = launch(s, 7, k_faces_post, s->gridF, STEP_THREADS))) return rc;
   if ((rc = launch(s, 8, k_extrap, s->gridExtrap, STEP_THREADS))) return rc;
@@ -608,6 +644,12 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
 int pl3d_destroy(pl3d_sim *s) {
   if (!s) return fail(PL3D_EINVAL, "null simulation handle");
   if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto &p : s->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : s->eventPool) (void)hipEventDestroy(e);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -842,6 +884,7 @@ int pl3d_set_target(pl3d_sim *s, double angle_deg) {
 
 int pl3d_set_timing(pl3d_sim *s, int32_t enabled) {
   if (int rc = check_sim(s)) return rc;
+  if (int rc = sync_ctl(s)) return rc;
   s->timing = enabled != 0;
   for (auto &t : s->timers) {
     t.ms = 0;
@@ -861,6 +904,54 @@ int pl3d_kernel_times(pl3d_sim *s, int32_t cap, const char **names, double *tota
     if (launches) launches[i] = s->timers[i].launches;
   }
   return PL3D_OK;
+}
+
+int pl3d_counters(pl3d_sim *s, int64_t out[8]) {
+  if (int rc = check_sim(s)) return rc;
+  if (int rc = sync_ctl(s)) return rc;
+  const DevCtl &C = *s->hctl;
+  out[0] = C.steps;
+  out[1] = int64_t(C.cellIterations);
+  out[2] = int64_t(C.solveCount);
+  out[3] = int64_t(C.liquidCellSum);
+  out[4] = C.lastM;
+  out[5] = C.overflow;
+  out[6] = s->P.nc;
+  out[7] = s->P.nfaces;
+  return PL3D_OK;
+}
+
+int pl3d_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy) {
+  if (bytes <= 0 || bytes % 16 != 0 || reps <= 0 || !ms_per_copy)
+    return fail(PL3D_EINVAL, "copy probe needs bytes > 0, a multiple of 16, and reps > 0");
+  float4 *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto done = [&](int rc) {
+    if (st) (void)hipStreamSynchronize(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+  };
+  hipError_t e;
+  const size_t n4 = size_t(bytes) / 16;
+  if ((e = hipMalloc(&a, bytes)) || (e = hipMalloc(&b, bytes)) ||
+      (e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) || (e = hipEventCreate(&e0)) ||
+      (e = hipEventCreate(&e1)) || (e = hipMemsetAsync(a, 1, bytes, st)))
+    return done(fail(PL3D_EDEVICE, "copy probe setup: %s", hipGetErrorString(e)));
+  const int grid = int(std::min<size_t>((n4 + 255) / 256, 65536));
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n4);  // warm
+  if ((e = hipEventRecord(e0, st))) return done(fail(PL3D_EDEVICE, "event"));
+  for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n4);
+  if ((e = hipEventRecord(e1, st)) || (e = hipEventSynchronize(e1)))
+    return done(fail(PL3D_EDEVICE, "copy probe: %s", hipGetErrorString(e)));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  *ms_per_copy = double(ms) / reps;
+  return done(PL3D_OK);
 }
 
 }  // extern "C"

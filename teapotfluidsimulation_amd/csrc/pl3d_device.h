@@ -83,6 +83,10 @@ struct DevCtl {
   // --- scan bookkeeping ---
   unsigned int scanTicket;
   unsigned int doneTicket;
+  // --- measurement counters (cumulative since creation) ---
+  unsigned long long cellIterations;  // sum over solves of liquid cells * iterations
+  unsigned long long liquidCellSum;   // sum over substeps of liquid cells
+  unsigned long long solveCount;      // solves that iterated
 };
 
 // Fixed configuration, uploaded once.

@@ -135,6 +135,8 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   A.planes = nx + ny + nz - 2;
   A.precond = precond == PL3D_PRECOND_JACOBI ? 2 : 0;
   A.overflow = nullptr;
+  A.cellIterations = nullptr;
+  A.solveCount = nullptr;
   auto up = [&](size_t o, const void *src, size_t n) {
     return hipMemcpyAsync(mem + o, src, n, hipMemcpyHostToDevice, st);
   };

@@ -20,6 +20,8 @@ struct PcgShared {
   int planes;              // number of hyperplanes (nx + ny + nz - 2)
   int precond;             // 0 = MIC(0), 2 = Jacobi
   int *overflow;           // set when m exceeds PL3D_SOLVE_MMAX
+  unsigned long long *cellIterations;  // += m * iterations per solve (nullptr: not counted)
+  unsigned long long *solveCount;
 };
 
 // One solve.

@@ -444,6 +444,10 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
   }
 #undef FOR_OWNED
   if (tid == 0) {
+    if (A.cellIterations) {
+      atomicAdd(A.cellIterations, (unsigned long long)m * (unsigned long long)iters);
+      atomicAdd(A.solveCount, 1ull);
+    }
     *D.iterations = iters;
     *D.converged = converged ? 1 : 0;
     if (!converged && D.failures) atomicAdd(D.failures, 1);

@@ -49,3 +49,4 @@ __global__ void k_faces_post(DevBufs B, DevParams P);
 __global__ void k_extrap(DevBufs B, DevParams P);
 __global__ void k_advect(DevBufs B, DevParams P);
 __global__ void k_render_positions(DevBufs B, DevParams P, float *out);
+__global__ void k_copy_probe(const float4 *a, float4 *b, size_t n4);

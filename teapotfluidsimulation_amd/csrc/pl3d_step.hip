@@ -796,6 +796,7 @@ __global__ __launch_bounds__(STEP_THREADS) void k_advect(DevBufs B, DevParams P)
     C.steps += 1;
     C.phi = C.phi1;
     C.lastM = C.m;
+    C.liquidCellSum += (unsigned long long)C.m;
     __threadfence();
   }
 }
@@ -814,4 +815,13 @@ __global__ void k_render_positions(DevBufs B, DevParams P, float *out) {
   out[3 * q] = x;
   out[3 * q + 1] = y;
   out[3 * q + 2] = 2 * P.h - z;
+}
+
+// ============================================================================= copy probe
+// Plain streaming copy (16 B per lane per iteration), used to calibrate the per-dispatch HBM
+// byte counters and as the measured bandwidth ceiling.
+__global__ void k_copy_probe(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n4) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n4;
+       i += size_t(gridDim.x) * blockDim.x)
+    b[i] = a[i];
 }

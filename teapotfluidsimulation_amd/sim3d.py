@@ -197,6 +197,13 @@ class Simulation3D:
     def set_timing(self, enabled: bool):
         check(load().pl3d_set_timing(self._handle(), 1 if enabled else 0))
 
+    def counters(self) -> dict:
+        raw = (C.c_int64 * 8)()
+        check(load().pl3d_counters(self._handle(), raw))
+        keys = ["substeps", "cellIterations", "solves", "liquidCellSum", "liquidCells",
+                "solverOverflow", "cells", "faces"]
+        return dict(zip(keys, [int(v) for v in raw]))
+
     def kernel_times(self) -> dict:
         cap = 32
         names = (C.c_char_p * cap)()
