@@ -175,6 +175,7 @@ typedef struct pl3d_solve_result {
   double true_residual; /* max |b - A q| recomputed in FP64 after the solve */
   int32_t liquid_cells;
   int32_t pad;
+  double kernel_ms; /* device time of the solve kernel (HIP events around the launch) */
 } pl3d_solve_result;
 
 typedef struct pl3d_sim pl3d_sim;

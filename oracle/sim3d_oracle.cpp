@@ -1251,6 +1251,7 @@ void or3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, con
     out->true_residual = tr;
     out->liquid_cells = m;
     out->pad = 0;
+    out->kernel_ms = 0.0;
   }
 }
 

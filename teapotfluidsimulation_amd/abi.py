@@ -134,6 +134,7 @@ class SolveResult(C.Structure):
         ("true_residual", C.c_double),
         ("liquid_cells", C.c_int32),
         ("pad", C.c_int32),
+        ("kernel_ms", C.c_double),
     ]
 
 

@@ -244,7 +244,7 @@ int enqueue_substep(pl3d_sim *s) {
     A.solveCount = &s->B.ctl->solveCount;
     hipEvent_t a = nullptr;
     if ((rc = timed_begin(s, &a))) return rc;
-    hipLaunchKernelGGL(k_pcg, dim3(2), dim3(PL3D_SOLVE_THREADS), 0, s->stream, A, s->B.descs,
+    hipLaunchKernelGGL(A.precond == 2 ? k_pcg_jacobi : k_pcg_mic0, dim3(2), dim3(PL3D_SOLVE_THREADS), 0, s->stream, A, s->B.descs,
                        &s->B.ctl->active);
     HIP_OK(hipGetLastError());
     if ((rc = timed_end(s, 6, a))) return rc;

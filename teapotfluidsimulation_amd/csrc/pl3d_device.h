@@ -28,7 +28,9 @@
 #define P2G_SCALE 1099511627776.0  // 2^40
 #define RATIO_SCALE 65536.0        // STAT interiorSum unit (gpu3d/layout.ts:17)
 
-#define NO_NBR 0xFFFFu
+// neighbour slots of the liquid-cell structure (16 bits each): a liquid index, or one of
+#define NB_AIR 0xFFFEu    // AIR neighbour: p = 0 ghost, counts in the diagonal
+#define NB_SOLID 0xFFFFu  // SOLID neighbour: no coupling
 
 // Device controller and per-substep state.  FP64 scalars: time, controller, pose.
 struct DevCtl {

@@ -12,7 +12,8 @@
 #include "../../include/pourlab3d.h"
 #include "pl3d_solver.h"
 
-#define NO_NBR_H 0xFFFFu
+#define NB_AIR_H 0xFFFEu
+#define NB_SOLID_H 0xFFFFu
 
 extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell,
                               const double *b, double *q, double tol, int32_t max_iter,
@@ -44,7 +45,9 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
     }
   const int m = int(lcell.size());
   for (size_t c = 0; c < nc; c++) q[c] = 0.0;
-  if (m > PL3D_SOLVE_MMAX)
+  if (m >= int(NB_AIR_H))
+    return perr(PL3D_ECAPACITY, "liquid cells exceed the 16-bit neighbour index range");
+  if (m > PL3D_SOLVE_MMAX - 1)
     return perr(PL3D_ECAPACITY, "liquid cells " + std::to_string(m) + " exceed capacity " +
                                     std::to_string(PL3D_SOLVE_MMAX));
   if (nx + ny + nz - 2 > PL3D_MAX_PLANES) return perr(PL3D_ECAPACITY, "too many hyperplanes");
@@ -62,7 +65,8 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
     for (int d = 0; d < 6; d++) {
       const uint8_t f = cell[c + offs[d]];
       if (f != PL3D_SOLID) dg++;
-      nb[d] = f == PL3D_FLUID ? uint32_t(lidx[c + offs[d]]) : NO_NBR_H;
+      nb[d] = f == PL3D_FLUID ? uint32_t(lidx[c + offs[d]])
+                              : (f == PL3D_AIR ? NB_AIR_H : NB_SOLID_H);
     }
     nbr[t] = nb[0] | (nb[1] << 16);
     nbr[t + stride] = nb[2] | (nb[3] << 16);
@@ -145,8 +149,13 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
       (e = up(o_plane, plane.data(), 2 * S)) || (e = up(o_b, bl.data(), 8 * S)) ||
       (e = up(o_desc, &D, sizeof D)))
     return done(perr(PL3D_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
-  hipLaunchKernelGGL(k_pcg, dim3(1), dim3(PL3D_SOLVE_THREADS), 0, st, A,
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if ((e = hipEventCreate(&e0)) || (e = hipEventCreate(&e1)) || (e = hipEventRecord(e0, st)))
+    return done(perr(PL3D_EDEVICE, std::string("events: ") + hipGetErrorString(e)));
+  hipLaunchKernelGGL(A.precond == 2 ? k_pcg_jacobi : k_pcg_mic0, dim3(1),
+                     dim3(PL3D_SOLVE_THREADS), 0, st, A,
                      reinterpret_cast<const PcgDesc *>(mem + o_desc), nullptr);
+  (void)hipEventRecord(e1, st);
   if ((e = hipGetLastError()) != hipSuccess)
     return done(perr(PL3D_EDEVICE, std::string("launch: ") + hipGetErrorString(e)));
   std::vector<double> ql(S);
@@ -155,7 +164,12 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
       (e = hipStreamSynchronize(st)))
     return done(perr(PL3D_EDEVICE, std::string("solve: ") + hipGetErrorString(e)));
   for (int t = 0; t < m; t++) q[lcell[t]] = ql[t];
+  float kms = 0.f;
+  (void)hipEventElapsedTime(&kms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (out) {
+    out->kernel_ms = kms;
     out->iterations = m == 0 ? 0 : hres.iterations;
     out->converged = m == 0 ? 1 : hres.converged;
     out->residual = m == 0 ? 0.0 : hres.residual;

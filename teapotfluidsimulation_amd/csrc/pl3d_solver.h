@@ -5,7 +5,7 @@
 #include <stdint.h>
 
 #define PL3D_SOLVE_THREADS 1024
-#define PL3D_SOLVE_MMAX 16384  // liquid cells per solve (register/LDS capacity of one CU)
+#define PL3D_SOLVE_MMAX 16384  // LDS slots per solve; one is the zero slot, so m <= 16383
 #define PL3D_MAX_PLANES 1024   // hyperplanes i+j+k of the MIC(0) schedule
 
 // Liquid-cell structure shared by the solves of one launch, all in list order t = rank of the
@@ -42,4 +42,5 @@ struct PcgDesc {
   uint32_t *nbrPlane;
 };
 
-__global__ void k_pcg(PcgShared A, const PcgDesc *Ds, const int *activeFlag);
+__global__ void k_pcg_mic0(PcgShared A, const PcgDesc *Ds, const int *activeFlag);
+__global__ void k_pcg_jacobi(PcgShared A, const PcgDesc *Ds, const int *activeFlag);

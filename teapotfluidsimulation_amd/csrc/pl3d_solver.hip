@@ -2,16 +2,17 @@
 // solve (Pcg3D.solve, solver3d.ts:858-937).
 //
 // Why one workgroup: at 4-5 mm cells a pour has ~6-12 k liquid cells and every CG iteration is a
-// chain of dependent reductions; one CU with 1024 threads holds the whole problem in registers
-// plus LDS, so an iteration costs barriers instead of grid-wide synchronisation.  The pressure
-// and the density-correction solves of a substep are independent and run as two workgroups of
-// one launch.
+// chain of dependent reductions.  A workgroup barrier costs ~0.05 us on gfx950, a barrier between
+// workgroups through L2 ~0.7-2 us, so one CU with 1024 threads holding the whole problem in
+// registers plus LDS is the fastest place for the iteration.  The pressure and the density-
+// correction solves of a substep are independent and run as two workgroups of one launch.
 //
 // Storage and precision:
-//   owner mapping   cell t (list order) is owned by thread t % 1024, register slot t / 1024
-//   registers       r, x_lo (FP32 increments of the solution), s (own copy)
-//   LDS             s[MMAX] (stencil input), W[MMAX] (A s during the product, then z = M r;
-//                   the MIC(0) sweeps run in place in W)
+//   owner mapping   wave w owns the contiguous list chunk [w*ce*64, (w+1)*ce*64); slot k of lane l
+//                   is cell t = (w*ce + k)*64 + l, with ce = ceil(m / 1024) slots per thread
+//   registers       r, x_lo (FP32 increments of the solution), the cell's packed neighbour words
+//   LDS             s[MMAX] (stencil input), W[MMAX] (A s, then z = M r for MIC(0); the sweeps
+//                   run in place in W)
 //   global (FP64)   b, x_hi (solution)
 //   dot products    FP64 partial sums, fixed-order tree: deterministic
 //   refinement      x_hi += x_lo and r = b - A x_hi recomputed in FP64 ("reliable update",
@@ -25,33 +26,90 @@
 //           lexicographic order; each cell depends only on its -x/-y/-z neighbours, so processing
 //           whole hyperplanes i+j+k = l in increasing l (decreasing for the backward sweep) gives
 //           the same factor and the same triangular solves, with one barrier per hyperplane.
-//   Jacobi  z = r / diag.
+//   Jacobi  z = r / diag, recomputed from the register-resident neighbour words where needed.
 #include "pl3d_device.h"
 #include "pl3d_solver.h"
 
+// Optional phase timing (off unless built with -DPL3D_PHASE_TIMING): thread 0 of each block adds
+// the s_memrealtime ticks (100 MHz) spent in each phase of the iteration to pl3d_phase_ticks.
+#ifdef PL3D_PHASE_TIMING
+__device__ unsigned long long pl3d_phase_ticks[16];
+#define PHASE_T0() unsigned long long ph_t0_ = wall_clock64()
+#define PHASE_MARK(i)                                                \
+  do {                                                               \
+    if (threadIdx.x == 0) {                                          \
+      const unsigned long long ph_t1_ = wall_clock64();              \
+      atomicAdd(&pl3d_phase_ticks[i], ph_t1_ - ph_t0_);              \
+      ph_t0_ = ph_t1_;                                               \
+    }                                                                \
+  } while (0)
+#else
+#define PHASE_T0() (void)0
+#define PHASE_MARK(i) (void)0
+#endif
+
 #define NT PL3D_SOLVE_THREADS
 #define MMAX PL3D_SOLVE_MMAX
-#define CPT (MMAX / NT)
 #define NWAVES (NT / 64)
 #define RU_PERIOD 25
 
 namespace {
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// Wave reductions with DPP row shifts (inclusive scan inside each 16-lane row; lanes without a
+// source read 0) and the four row totals read from lanes 15/31/47/63, added in a fixed order: the
+// result is uniform across the wave and deterministic.
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr_f64(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(u & 0xFFFFFFFFull), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
 }
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double lane_f64(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane(int(u & 0xFFFFFFFFull), lane);
+  const unsigned hi = __builtin_amdgcn_readlane(int(u >> 32), lane);
+  return __longlong_as_double((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_shr_f64<0x111>(v);  // row_shr:1
+  v += dpp_shr_f64<0x112>(v);  // row_shr:2
+  v += dpp_shr_f64<0x114>(v);  // row_shr:4
+  v += dpp_shr_f64<0x118>(v);  // row_shr:8
+  return ((lane_f64(v, 15) + lane_f64(v, 31)) + lane_f64(v, 47)) + lane_f64(v, 63);
+}
+__device__ __forceinline__ float wave_max(float v) {  // v >= 0
+  v = fmaxf(v, dpp_shr_f32<0x111>(v));
+  v = fmaxf(v, dpp_shr_f32<0x112>(v));
+  v = fmaxf(v, dpp_shr_f32<0x114>(v));
+  v = fmaxf(v, dpp_shr_f32<0x118>(v));
+  const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
+  const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+  const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47));
+  const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  return fmaxf(fmaxf(a, b), fmaxf(c, d));
 }
 
 struct Red {
   double sum[2][NWAVES];
-  double sum2[2][NWAVES];
   float mx[2][NWAVES];
+};
+
+// LDS slot that always holds 0: solver-local neighbour words point missing (AIR or SOLID)
+// neighbours here, so the stencil product needs no branches.
+#define ZSLOT (MMAX - 1)
+
+struct SolveLds {
+  float s[MMAX];
+  float W[MMAX];
+  Red R;
+  int planeStart[PL3D_MAX_PLANES + 1];
+  int planeFill[PL3D_MAX_PLANES];
+  int planeRange[2];
 };
 
 // Block-wide reductions with alternating buffers: one barrier each; every thread adds the 16
@@ -90,7 +148,7 @@ __device__ __forceinline__ void block_sum_max(Red &R, int &buf, double v, float 
 }
 __device__ __forceinline__ double block_max_d(Red &R, int &buf, double v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));  // rare: setup, refinement
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) R.sum[buf][w] = v;
   __syncthreads();
@@ -103,64 +161,128 @@ __device__ __forceinline__ double block_max_d(Red &R, int &buf, double v) {
 
 __device__ __forceinline__ int nb_lo(uint32_t p) { return int(p & 0xFFFFu); }
 __device__ __forceinline__ int nb_hi(uint32_t p) { return int(p >> 16); }
+// neighbour slot: liquid index, NB_AIR (p = 0 ghost) or NB_SOLID (no coupling)
+__device__ __forceinline__ bool fl(int j) { return j < int(NB_AIR); }
+// non-solid neighbour count from the three packed words
+__device__ __forceinline__ float diag_of(uint32_t px, uint32_t py, uint32_t pz) {
+  int d = 0;
+  d += nb_lo(px) != int(NB_SOLID);
+  d += nb_hi(px) != int(NB_SOLID);
+  d += nb_lo(py) != int(NB_SOLID);
+  d += nb_hi(py) != int(NB_SOLID);
+  d += nb_lo(pz) != int(NB_SOLID);
+  d += nb_hi(pz) != int(NB_SOLID);
+  return float(d);
+}
+// Workgroup barrier that orders LDS only: global loads issued earlier (prefetches) stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
-}  // namespace
+// FP64 part of the refinement, kept out of line so its transient registers do not add to the
+// iteration loop's (the call is rare).  On entry W[t] holds this thread's x_lo increments; on exit
+// x_hi has absorbed them, W[t] holds the FP32-rounded true residual b - A x_hi of each owned cell,
+// and the return value is this thread's max |b - A x_hi|.
+__device__ __attribute__((noinline)) double fold_and_residual(
+    float *W, double *__restrict__ xhi, const double *__restrict__ b,
+    const uint32_t *__restrict__ nbx, const uint32_t *__restrict__ nby,
+    const uint32_t *__restrict__ nbz, int m, int ce, int tb) {
+  for (int k = 0; k < ce; k++) {
+    const int t = tb + k * 64;
+    if (t < m) xhi[t] += double(W[t]);
+  }
+  __syncthreads();
+  double mx = 0.0;
+  for (int k = 0; k < ce; k++) {
+    const int t = tb + k * 64;
+    if (t < m) {
+      const uint32_t px = nbx[t], py = nby[t], pz = nbz[t];
+      double acc = double(diag_of(px, py, pz)) * xhi[t];
+      int j;
+      if (fl(j = nb_hi(px))) acc -= xhi[j];
+      if (fl(j = nb_lo(px))) acc -= xhi[j];
+      if (fl(j = nb_hi(py))) acc -= xhi[j];
+      if (fl(j = nb_lo(py))) acc -= xhi[j];
+      if (fl(j = nb_hi(pz))) acc -= xhi[j];
+      if (fl(j = nb_lo(pz))) acc -= xhi[j];
+      const double rt = b[t] - acc;
+      W[t] = float(rt);
+      mx = fmax(mx, fabs(rt));
+    }
+  }
+  return mx;
+}
 
-// One workgroup per solve.  Arguments shared by the solves of one launch in `A`, per-solve
-// arguments in `D[blockIdx.x]`.
-__global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restrict__ Ds,
-                                            const int *__restrict__ activeFlag) {
-  if (activeFlag != nullptr && *activeFlag == 0) return;
-  const PcgDesc D = Ds[blockIdx.x];
-  __shared__ float s_lds[MMAX];
-  __shared__ float W[MMAX];
-  __shared__ Red R;
-  __shared__ int planeStart[PL3D_MAX_PLANES + 1];
-  __shared__ int planeFill[PL3D_MAX_PLANES];
+// FOR_SLOTS: every slot k < ce of this thread (a uniform condition).  Slots past the last cell
+// (t >= m, `own` false) run the same branch-free code on padding: their loads use the clamped
+// index tc = min(t, m - 1), their register values are zero and their LDS writes land in unused
+// capacity (t < ce * 1024 <= MMAX).  Only global writes test `own`.
+#define FOR_SLOTS(...)                                   \
+  _Pragma("unroll") for (int k = 0; k < CPT; k++) {      \
+    if (k < ce) {                                        \
+      const int t = tb + k * 64;                         \
+      const bool own = t < m;                            \
+      const int tc = own ? t : m - 1;                    \
+      (void)own;                                         \
+      (void)tc;                                          \
+      __VA_ARGS__                                        \
+    }                                                    \
+  }
+
+template <bool MIC, int CPT>
+__device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const PcgDesc &D,
+                                         const int m) {
+  float *__restrict__ s_lds = L.s;
+  float *__restrict__ W = L.W;
+  Red &R = L.R;
   int rb = 0;
   const int tid = threadIdx.x;
-  const int m = *A.m;
   double *__restrict__ xhi = D.q;
   const double *__restrict__ b = D.b;
   const uint32_t *__restrict__ nbx = A.nbr;
   const uint32_t *__restrict__ nby = A.nbr + A.stride;
   const uint32_t *__restrict__ nbz = A.nbr + 2 * A.stride;
-
-  if (m > MMAX) {  // beyond capacity: report, leave q = 0
-    for (int t = tid; t < m && t < A.stride; t += NT) xhi[t] = 0.0;
-    if (tid == 0) {
-      *D.iterations = 0;
-      *D.converged = 0;
-      if (D.failures) atomicAdd(D.failures, 1);
-      if (A.overflow) atomicOr(A.overflow, 1);
-      if (D.residual) *D.residual = -1.0;
-    }
-    return;
-  }
-
-  // ---- owned cells.  Wave w owns the contiguous chunk [w*ce*64, (w+1)*ce*64) of the list, slot
-  // k of lane l is t = (w*ce + k)*64 + l: one base register, slot offsets are immediates.
   const int ce = (m + NT - 1) / NT;
   const int tb = ((tid >> 6) * ce) * 64 + (tid & 63);
-#define FOR_OWNED(...)                         \
-  _Pragma("unroll") for (int k = 0; k < CPT; k++) { \
-    if (k < ce) {                              \
-      const int t = tb + k * 64;               \
-      if (t < m) { __VA_ARGS__ }               \
-    }                                          \
-  }
-  float r[CPT], xlo[CPT];
+
+  // registers per slot: r, x_lo, the diagonal, and the six neighbour slots as two 16-bit LDS
+  // indices per word (missing neighbours -> ZSLOT).  The words are made opaque at the top of each
+  // iteration so the compiler cannot hoist their decoding out of the loop (that would keep six
+  // addresses per slot live).
+  float r[CPT], xlo[CPT], dg[CPT];
+  uint32_t nbc[CPT][3];
   double bmax = 0.0;
+  auto local_word = [](uint32_t w) -> uint32_t {
+    const uint32_t lo = w & 0xFFFFu, hi = w >> 16;
+    return (lo < NB_AIR ? lo : uint32_t(ZSLOT)) | ((hi < NB_AIR ? hi : uint32_t(ZSLOT)) << 16);
+  };
+  constexpr uint32_t Z2 = uint32_t(ZSLOT) | (uint32_t(ZSLOT) << 16);
 #pragma unroll
-  for (int k = 0; k < CPT; k++) r[k] = xlo[k] = 0.f;
-  FOR_OWNED({
-    const double bt = b[t];
+  for (int k = 0; k < CPT; k++) {
+    r[k] = xlo[k] = 0.f;
+    dg[k] = 6.f;
+    nbc[k][0] = nbc[k][1] = nbc[k][2] = Z2;
+  }
+  FOR_SLOTS({
+    const uint32_t wx = nbx[tc], wy = nby[tc], wz = nbz[tc];
+    const double bl = b[tc];
+    dg[k] = own ? diag_of(wx, wy, wz) : 6.f;  // padding: 6, no coupling, r = 0
+    nbc[k][0] = own ? local_word(wx) : Z2;
+    nbc[k][1] = own ? local_word(wy) : Z2;
+    nbc[k][2] = own ? local_word(wz) : Z2;
+    const double bt = own ? bl : 0.0;
     r[k] = float(bt);
     bmax = fmax(bmax, fabs(bt));
-    xhi[t] = 0.0;
+    if (own) xhi[t] = 0.0;
   })
+  if (tid == 0) {
+    s_lds[ZSLOT] = 0.f;
+    W[ZSLOT] = 0.f;
+  }
   const double bnorm = block_max_d(R, rb, bmax);
-  if (m == 0 || bnorm < 1e-14) {  // solver3d.ts:882
+  if (bnorm < 1e-14) {  // solver3d.ts:882
     if (tid == 0) {
       *D.iterations = 0;
       *D.converged = 1;
@@ -171,15 +293,14 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
     return;
   }
   const double target = fmax(double(D.tol) * bnorm, 1e-14);
-  const bool mic = A.precond == 0;
 
   // ---- MIC(0): hyperplane ordering and factorisation
   int nplanes = 0;
-  __shared__ int planeRange[2];  // first and last non-empty hyperplane
   int *__restrict__ porder = D.porder;      // plane position p -> t
   float *__restrict__ pc2p = D.pc2;         // 1/e per plane position
   uint32_t *__restrict__ nbp = D.nbrPlane;  // 3 x m, plane order
-  if (mic) {
+  int *planeStart = L.planeStart, *planeFill = L.planeFill;
+  if constexpr (MIC) {
     nplanes = A.planes;
     for (int l = tid; l < nplanes; l += NT) planeFill[l] = 0;
     __syncthreads();
@@ -196,8 +317,8 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
         acc += planeFill[l];
       }
       planeStart[nplanes] = acc;
-      planeRange[0] = lo;
-      planeRange[1] = hi;
+      L.planeRange[0] = lo;
+      L.planeRange[1] = hi;
     }
     __syncthreads();
     for (int l = tid; l < nplanes; l += NT) planeFill[l] = 0;
@@ -213,17 +334,17 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
     __syncthreads();
     // factorisation sweep: W[t] holds 1/e_t (= pc_t^2) once plane(t) is done
     const float tau = 0.97f;
-    for (int l = planeRange[0]; l <= planeRange[1]; l++) {
+    for (int l = L.planeRange[0]; l <= L.planeRange[1]; l++) {
       const int p0 = planeStart[l], p1 = planeStart[l + 1];
       for (int p = p0 + tid; p < p1; p += NT) {
         const int t = porder[p];
         const uint32_t mc = A.mic[t];
-        const float d = float(A.diag[t]);
-        const int jx = nb_lo(nbp[p]), jy = nb_lo(nbp[p + A.stride]),
-                  jz = nb_lo(nbp[p + 2 * A.stride]);
-        const float pi2 = jx != int(NO_NBR) ? W[jx] : 0.f;
-        const float pj2 = jy != int(NO_NBR) ? W[jy] : 0.f;
-        const float pk2 = jz != int(NO_NBR) ? W[jz] : 0.f;
+        const uint32_t wx = nbp[p], wy = nbp[p + A.stride], wz = nbp[p + 2 * A.stride];
+        const float d = diag_of(wx, wy, wz);
+        const int jx = nb_lo(wx), jy = nb_lo(wy), jz = nb_lo(wz);
+        const float pi2 = fl(jx) ? W[jx] : 0.f;
+        const float pj2 = fl(jy) ? W[jy] : 0.f;
+        const float pk2 = fl(jz) ? W[jz] : 0.f;
         const float ci = float(mc & 3u), cj = float((mc >> 2) & 3u), ck = float((mc >> 4) & 3u);
         float e = d - pi2 - pj2 - pk2 - tau * (ci * pi2 + cj * pj2 + ck * pk2);
         if (e < 0.25f * d) e = d;
@@ -231,137 +352,77 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
         W[t] = inv;
         pc2p[p] = inv;
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
 
-  // W = z = M r.  Owners write W; with MIC(0) the sweeps then run in place.  Ends with a barrier
-  // so every owner can read its z_t = W[t].
+  // z = M r.  MIC(0): owners write W = r, the sweeps run in place, W = z afterwards (ends with a
+  // barrier).  Jacobi: nothing to do, z is recomputed as r / diag where it is used.
   auto precondition = [&]() {
-    if (!mic) {
-      FOR_OWNED({ W[t] = r[k] / float(A.diag[t]); })
+    if constexpr (MIC) {
+      FOR_SLOTS({ W[t] = r[k]; })
       __syncthreads();
-      return;
-    }
-    FOR_OWNED({ W[t] = r[k]; })
-    __syncthreads();
-    const int lo = planeRange[0], hi = planeRange[1];
-    // forward: W_t = (1/e_t) (r_t + sum_{-nb} W_nb).  The next plane's first item is loaded
-    // before the barrier so its global-memory latency overlaps the current plane.
-    {
-      int pt = -1, pj0 = 0, pj1 = 0, pj2 = 0;
-      float pinv = 0.f;
-      auto fetch = [&](int l) {
-        const int p = planeStart[l] + tid;
-        pt = -1;
-        if (p < planeStart[l + 1]) {
-          pt = porder[p];
-          pj0 = nb_lo(nbp[p]);
-          pj1 = nb_lo(nbp[p + A.stride]);
-          pj2 = nb_lo(nbp[p + 2 * A.stride]);
-          pinv = pc2p[p];
-        }
-      };
-      if (lo <= hi) fetch(lo);
+      const int lo = L.planeRange[0], hi = L.planeRange[1];
+      // forward: W_t = (1/e_t) (r_t + sum_{-nb} W_nb)
       for (int l = lo; l <= hi; l++) {
-        const int t = pt, j0 = pj0, j1 = pj1, j2 = pj2;
-        const float inv = pinv;
-        if (l < hi) fetch(l + 1);
-        if (t >= 0) {
+        for (int p = planeStart[l] + tid; p < planeStart[l + 1]; p += NT) {
+          const int t = porder[p];
+          const int j0 = nb_lo(nbp[p]), j1 = nb_lo(nbp[p + A.stride]),
+                    j2 = nb_lo(nbp[p + 2 * A.stride]);
           float v = W[t];
-          if (j0 != int(NO_NBR)) v += W[j0];
-          if (j1 != int(NO_NBR)) v += W[j1];
-          if (j2 != int(NO_NBR)) v += W[j2];
-          W[t] = inv * v;
+          if (fl(j0)) v += W[j0];
+          if (fl(j1)) v += W[j1];
+          if (fl(j2)) v += W[j2];
+          W[t] = pc2p[p] * v;
         }
-        for (int p = planeStart[l] + tid + NT; p < planeStart[l + 1]; p += NT) {  // planes > NT
-          const int tt = porder[p];
-          const int k0 = nb_lo(nbp[p]), k1 = nb_lo(nbp[p + A.stride]),
-                    k2 = nb_lo(nbp[p + 2 * A.stride]);
-          float v = W[tt];
-          if (k0 != int(NO_NBR)) v += W[k0];
-          if (k1 != int(NO_NBR)) v += W[k1];
-          if (k2 != int(NO_NBR)) v += W[k2];
-          W[tt] = pc2p[p] * v;
-        }
-        __syncthreads();
+        lds_barrier();
       }
-    }
-    // backward: z_t = W_t + (1/e_t) sum_{+nb} z_nb
-    {
-      int pt = -1, pj0 = 0, pj1 = 0, pj2 = 0;
-      float pinv = 0.f;
-      auto fetch = [&](int l) {
-        const int p = planeStart[l] + tid;
-        pt = -1;
-        if (p < planeStart[l + 1]) {
-          pt = porder[p];
-          pj0 = nb_hi(nbp[p]);
-          pj1 = nb_hi(nbp[p + A.stride]);
-          pj2 = nb_hi(nbp[p + 2 * A.stride]);
-          pinv = pc2p[p];
-        }
-      };
-      if (lo <= hi) fetch(hi);
+      // backward: z_t = W_t + (1/e_t) sum_{+nb} z_nb
       for (int l = hi; l >= lo; l--) {
-        const int t = pt, j0 = pj0, j1 = pj1, j2 = pj2;
-        const float inv = pinv;
-        if (l > lo) fetch(l - 1);
-        if (t >= 0) {
+        for (int p = planeStart[l] + tid; p < planeStart[l + 1]; p += NT) {
+          const int j0 = nb_hi(nbp[p]), j1 = nb_hi(nbp[p + A.stride]),
+                    j2 = nb_hi(nbp[p + 2 * A.stride]);
           float v = 0.f;
-          if (j0 != int(NO_NBR)) v += W[j0];
-          if (j1 != int(NO_NBR)) v += W[j1];
-          if (j2 != int(NO_NBR)) v += W[j2];
-          W[t] = W[t] + inv * v;
+          if (fl(j0)) v += W[j0];
+          if (fl(j1)) v += W[j1];
+          if (fl(j2)) v += W[j2];
+          const int t = porder[p];
+          W[t] = W[t] + pc2p[p] * v;
         }
-        for (int p = planeStart[l] + tid + NT; p < planeStart[l + 1]; p += NT) {
-          const int tt = porder[p];
-          const int k0 = nb_hi(nbp[p]), k1 = nb_hi(nbp[p + A.stride]),
-                    k2 = nb_hi(nbp[p + 2 * A.stride]);
-          float v = 0.f;
-          if (k0 != int(NO_NBR)) v += W[k0];
-          if (k1 != int(NO_NBR)) v += W[k1];
-          if (k2 != int(NO_NBR)) v += W[k2];
-          W[tt] = W[tt] + pc2p[p] * v;
-        }
-        __syncthreads();
+        lds_barrier();
       }
     }
   };
+  auto zval = [&](int k, int t) -> float {
+    if constexpr (MIC) {
+      return W[t];
+    } else {
+      return r[k] * __builtin_amdgcn_rcpf(dg[k]);
+    }
+  };
 
-  // FP64 true residual r = b - A x_hi (refinement).  Returns max|r_true| over the block and, in
-  // gap, max|r_true - r_recursive|.
+  // FP64 true residual r = b - A x_hi (refinement).  Uses W as staging (its contents must be
+  // consumed before the call).  Returns max|r_true| and, in gap, max|r_true - r_recursive|.
   auto true_residual = [&](double &gap) -> double {
-    double mx = 0.0, gp = 0.0;
-    FOR_OWNED({
-      xhi[t] += double(xlo[k]);
+    FOR_SLOTS({
+      W[t] = xlo[k];
       xlo[k] = 0.f;
     })
-    __syncthreads();
-    FOR_OWNED({
-      const uint32_t px = nbx[t], py = nby[t], pz = nbz[t];
-      double acc = double(A.diag[t]) * xhi[t];
-      int j;
-      if ((j = nb_hi(px)) != int(NO_NBR)) acc -= xhi[j];
-      if ((j = nb_lo(px)) != int(NO_NBR)) acc -= xhi[j];
-      if ((j = nb_hi(py)) != int(NO_NBR)) acc -= xhi[j];
-      if ((j = nb_lo(py)) != int(NO_NBR)) acc -= xhi[j];
-      if ((j = nb_hi(pz)) != int(NO_NBR)) acc -= xhi[j];
-      if ((j = nb_lo(pz)) != int(NO_NBR)) acc -= xhi[j];
-      const double rt = b[t] - acc;
-      gp = fmax(gp, fabs(rt - double(r[k])));
-      r[k] = float(rt);
-      mx = fmax(mx, fabs(rt));
+    const double mx = fold_and_residual(W, xhi, b, nbx, nby, nbz, m, ce, tb);
+    float gp = 0.f;
+    FOR_SLOTS({
+      gp = fmaxf(gp, fabsf(W[t] - r[k]));
+      r[k] = W[t];
     })
-    gap = block_max_d(R, rb, gp);
+    gap = block_max_d(R, rb, double(gp));
     return block_max_d(R, rb, mx);
   };
 
-  // ---- PCG (solver3d.ts:902-936); s lives in LDS only (s_lds), z in W after precondition()
+  // ---- PCG (solver3d.ts:902-936); s lives in LDS (s_lds)
   precondition();
   double sig_part = 0.0;
-  FOR_OWNED({
-    const float zt = W[t];
+  FOR_SLOTS({
+    const float zt = zval(k, t);
     s_lds[t] = zt;
     sig_part += double(zt) * double(r[k]);
   })
@@ -371,43 +432,50 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
   int iters = 0;
   bool converged = false;
   int lastUpdate = 0;
+  PHASE_T0();
   for (int it = 1; it <= D.maxIter; it++) {
+#pragma unroll
+    for (int k = 0; k < CPT; k++) asm volatile("" : "+v"(nbc[k][0]), "+v"(nbc[k][1]), "+v"(nbc[k][2]));
     __syncthreads();  // s complete
+    PHASE_MARK(0);
     double sas_part = 0.0;
-    FOR_OWNED({  // same neighbour order as applyA (solver3d.ts:944-951)
-      const uint32_t px = nbx[t], py = nby[t], pz = nbz[t];
+    FOR_SLOTS({  // A s = diag s_t - sum of the liquid neighbours (missing ones read the zero slot)
+      const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
       const float st = s_lds[t];
-      float acc = float(A.diag[t]) * st;
-      int j;
-      if ((j = nb_hi(px)) != int(NO_NBR)) acc -= s_lds[j];
-      if ((j = nb_lo(px)) != int(NO_NBR)) acc -= s_lds[j];
-      if ((j = nb_hi(py)) != int(NO_NBR)) acc -= s_lds[j];
-      if ((j = nb_lo(py)) != int(NO_NBR)) acc -= s_lds[j];
-      if ((j = nb_hi(pz)) != int(NO_NBR)) acc -= s_lds[j];
-      if ((j = nb_lo(pz)) != int(NO_NBR)) acc -= s_lds[j];
-      W[t] = acc;  // A s, owner-private until the next precondition
+      const float acc = dg[k] * st - (((s_lds[px >> 16] + s_lds[px & 0xFFFFu]) +
+                                       (s_lds[py >> 16] + s_lds[py & 0xFFFFu])) +
+                                      (s_lds[pz >> 16] + s_lds[pz & 0xFFFFu]));
+      W[t] = acc;  // A s, read back by the owner only
       sas_part += double(st) * double(acc);
     })
+    PHASE_MARK(1);
     const double sas = block_sum(R, rb, sas_part);
+    PHASE_MARK(2);
     if (!(sas > 0.0)) break;  // solver3d.ts:916
     const double alpha = sigma / sas;
     const float fa = float(alpha);
     float rmax = 0.f;
-    FOR_OWNED({
+    FOR_SLOTS({
       xlo[k] += fa * s_lds[t];
       r[k] -= fa * W[t];
       rmax = fmaxf(rmax, fabsf(r[k]));
     })
-    __syncthreads();  // every A s read before W is reused
+    PHASE_MARK(3);
     precondition();
+    PHASE_MARK(4);
     double sn_part = 0.0;
-    FOR_OWNED({ sn_part += double(W[t]) * double(r[k]); })
+    FOR_SLOTS({ sn_part += double(zval(k, t)) * double(r[k]); })
     double sn;
     float resf;
     block_sum_max(R, rb, sn_part, rmax, sn, resf);
+    PHASE_MARK(5);
     res = double(resf);
     iters = it;
-    bool restart = false;
+    // s = z + beta s now (z from the recursive residual); the refinement below may reuse W
+    const float fb = float(sn / sigma);
+    FOR_SLOTS({ s_lds[t] = zval(k, t) + fb * s_lds[t]; })
+    sigma = sn;
+    PHASE_MARK(6);
     // reliable update: at the stop test, after each decade of residual reduction, and at least
     // every RU_PERIOD iterations
     if (res <= target || res < 0.1 * resAtUpdate || it - lastUpdate >= RU_PERIOD) {
@@ -420,16 +488,18 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
         break;
       }
       if (gap > 0.01 * double(resf)) {  // FP32 drift visible: restart from the true residual
-        restart = true;
+        __syncthreads();                // W reads of the refinement done
         precondition();
         double p2 = 0.0;
-        FOR_OWNED({ p2 += double(W[t]) * double(r[k]); })
-        sn = block_sum(R, rb, p2);
+        FOR_SLOTS({
+          const float zt = zval(k, t);
+          s_lds[t] = zt;
+          p2 += double(zt) * double(r[k]);
+        })
+        sigma = block_sum(R, rb, p2);
       }
+      PHASE_MARK(7);
     }
-    const float fb = restart ? 0.f : float(sn / sigma);
-    FOR_OWNED({ s_lds[t] = W[t] + fb * s_lds[t]; })
-    sigma = sn;
   }
   double tres;
   if (!converged) {
@@ -442,7 +512,6 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
   } else {
     tres = res;
   }
-#undef FOR_OWNED
   if (tid == 0) {
     if (A.cellIterations) {
       atomicAdd(A.cellIterations, (unsigned long long)m * (unsigned long long)iters);
@@ -455,4 +524,60 @@ __global__ __launch_bounds__(NT) void k_pcg(PcgShared A, const PcgDesc *__restri
     if (D.trueResidual) *D.trueResidual = tres;
     if (D.bnorm) *D.bnorm = bnorm;
   }
+}
+
+#undef FOR_SLOTS
+
+// Common entry: capacity check, empty problem, then the body compiled for the smallest number of
+// register slots per thread that holds the liquid cells (fewer slots, fewer live registers).
+template <bool MIC>
+__device__ __forceinline__ void pcg_entry(SolveLds &L, const PcgShared &A,
+                                          const PcgDesc *__restrict__ Ds,
+                                          const int *__restrict__ activeFlag) {
+  if (activeFlag != nullptr && *activeFlag == 0) return;
+  const PcgDesc D = Ds[blockIdx.x];
+  const int m = *A.m;
+  if (m > MMAX - 1) {  // beyond capacity (one LDS slot is the zero slot): report, leave q = 0
+    for (int t = threadIdx.x; t < m && t < A.stride; t += NT) D.q[t] = 0.0;
+    if (threadIdx.x == 0) {
+      *D.iterations = 0;
+      *D.converged = 0;
+      if (D.failures) atomicAdd(D.failures, 1);
+      if (A.overflow) atomicOr(A.overflow, 1);
+      if (D.residual) *D.residual = -1.0;
+    }
+    return;
+  }
+  if (m == 0) {
+    if (threadIdx.x == 0) {
+      *D.iterations = 0;
+      *D.converged = 1;
+      if (D.residual) *D.residual = 0.0;
+      if (D.trueResidual) *D.trueResidual = 0.0;
+      if (D.bnorm) *D.bnorm = 0.0;
+    }
+    return;
+  }
+  const int ce = (m + NT - 1) / NT;
+  if (ce <= 8)
+    pcg_body<MIC, 8>(L, A, D, m);
+  else if (ce <= 12)
+    pcg_body<MIC, 12>(L, A, D, m);
+  else
+    pcg_body<MIC, 16>(L, A, D, m);
+}
+
+}  // namespace
+
+// One workgroup per solve.  Arguments shared by the solves of one launch in `A`, per-solve
+// arguments in `Ds[blockIdx.x]`; the preconditioner is a compile-time choice.
+__global__ __launch_bounds__(NT) void k_pcg_mic0(PcgShared A, const PcgDesc *__restrict__ Ds,
+                                                 const int *__restrict__ activeFlag) {
+  __shared__ SolveLds L;
+  pcg_entry<true>(L, A, Ds, activeFlag);
+}
+__global__ __launch_bounds__(NT) void k_pcg_jacobi(PcgShared A, const PcgDesc *__restrict__ Ds,
+                                                   const int *__restrict__ activeFlag) {
+  __shared__ SolveLds L;
+  pcg_entry<false>(L, A, Ds, activeFlag);
 }

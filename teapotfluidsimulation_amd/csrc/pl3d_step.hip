@@ -447,7 +447,8 @@ __global__ __launch_bounds__(STEP_THREADS) void k_liquid(DevBufs B, DevParams P)
       f6[q] = fq;
       if (fq != PL3D_SOLID_) dg++;
       if (fq == PL3D_AIR_) interior = false;
-      nb[q] = fq == PL3D_FLUID_ ? uint32_t(B.lidx[c + offs[q]]) : NO_NBR;
+      nb[q] = fq == PL3D_FLUID_ ? uint32_t(B.lidx[c + offs[q]])
+                                : (fq == PL3D_AIR_ ? NB_AIR : NB_SOLID);
     }
     B.nbr[t] = nb[0] | (nb[1] << 16);
     B.nbr[t + P.nc] = nb[2] | (nb[3] << 16);

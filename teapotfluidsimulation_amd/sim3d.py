@@ -241,4 +241,5 @@ class Pcg3D:
             "residual": res.residual,
             "trueResidual": res.true_residual,
             "liquidCells": res.liquid_cells,
+            "kernelMs": res.kernel_ms,
         }

@@ -15,12 +15,13 @@ def _row(pins, cell):
     return next(r for r in pins["run3d_rows"] if r["cellMm"] == cell)
 
 
-@pytest.mark.parametrize("cell,vol_tol,share_tol", [(5, 0.01, 0.02), (4, 0.003, 0.013)])
-def test_full_pour_matches_cpu_reference(pins, cell, vol_tol, share_tol):
+@pytest.mark.parametrize("cell,vol_tol,share_tol,precond", [
+    (5, 0.01, 0.02, "mic0"), (4, 0.003, 0.013, "mic0"), (4, 0.003, 0.013, "jacobi")])
+def test_full_pour_matches_cpu_reference(pins, cell, vol_tol, share_tol, precond):
     ref = _row(pins, cell)
-    out = run_pour(cell, 20)
+    out = run_pour(cell, 20, precond)
     r = out["result"]
-    print(f"{cell} mm: {r['dischargedMl']:.2f} mL vs {ref['dischargedMl']:.2f}; "
+    print(f"{cell} mm {precond}: {r['dischargedMl']:.2f} mL vs {ref['dischargedMl']:.2f}; "
           f"shares {out['dischargedFractions']} vs {ref['dischargedFractions']}; "
           f"wall {out['wallSeconds']:.2f}s ({out['realTimeFactor']:.2f}x real time)")
     assert out["grid"] == ref["grid"] and out["parcelsHalfPot"] == ref["parcelsHalfPot"]
