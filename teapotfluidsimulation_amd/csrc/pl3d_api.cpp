@@ -227,7 +227,7 @@ int enqueue_substep(pl3d_sim *s) {
   if ((rc = launch(s, 1, k_count, s->gridN, STEP_THREADS))) return rc;
   if ((rc = launch(s, 2, k_classify_scan, s->gridScan, STEP_THREADS))) return rc;
   if ((rc = launch(s, 3, k_scatter, s->gridN, STEP_THREADS))) return rc;
-  if ((rc = launch(s, 4, k_p2g, s->gridFC, STEP_THREADS))) return rc;
+  if ((rc = launch(s, 4, k_p2g, s->gridFC, P2G_THREADS))) return rc;
   if ((rc = launch(s, 5, k_liquid, s->gridC, STEP_THREADS))) return rc;
   {
     PcgShared A;
@@ -627,7 +627,8 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   s->gridN = (n + STEP_THREADS - 1) / STEP_THREADS;
   s->gridC = (P.nc + STEP_THREADS - 1) / STEP_THREADS;
   s->gridF = (P.nfaces + STEP_THREADS - 1) / STEP_THREADS;
-  s->gridFC = (P.nfaces + P.nc + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridFC = ((nx + P2G_TX) / P2G_TX) * ((ny + P2G_TY) / P2G_TY) *
+              ((nz + P2G_TZ) / P2G_TZ);  // bricks covering the (n+1)-sized face index space
   s->gridExtrap = (((P.nfaces + 63) & ~63) + P.nc + STEP_THREADS - 1) / STEP_THREADS;
   s->gridScan = P.scanBlocks;
   static const char *names[] = {"k_begin",   "k_count",  "k_classify_scan", "k_scatter",

@@ -5,6 +5,11 @@
 #include "pl3d_solver.h"
 
 #define STEP_THREADS 256
+// k_p2g: one block of P2G_THREADS per P2G_TX x P2G_TY x P2G_TZ brick of faces and cells
+#define P2G_TX 8
+#define P2G_TY 8
+#define P2G_TZ 4
+#define P2G_THREADS 1024
 #define SCAN_ITEMS 4
 #define SCAN_TILE (STEP_THREADS * SCAN_ITEMS)
 

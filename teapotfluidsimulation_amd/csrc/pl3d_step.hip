@@ -246,22 +246,24 @@ __global__ __launch_bounds__(STEP_THREADS) void k_classify_scan(DevBufs B, DevPa
     if (w < wave) waveOff += wsum[w];
     blockTotal += wsum[w];
   }
-  // look-back: status word = flag(2 bits) << 62 | fluid(31) << 31 | count(31)
+  // look-back: status word = flag(2 bits) << 62 | fluid(31) << 31 | count(31).  The word carries
+  // its own payload, so relaxed device-scope atomics suffice: an acquire load would invalidate
+  // this XCD's caches on every spin.
   if (threadIdx.x == 0) {
     const unsigned long long aggC = blockTotal & 0xFFFFFFFFull, aggF = blockTotal >> 32;
     const unsigned long long A_FLAG = 1ull << 62, P_FLAG = 2ull << 62;
     unsigned long long exC = 0ull, exF = 0ull;
     if (tile == 0) {
-      __hip_atomic_store(&B.scanStatus[0], P_FLAG | (aggF << 31) | aggC, __ATOMIC_RELEASE,
+      __hip_atomic_store(&B.scanStatus[0], P_FLAG | (aggF << 31) | aggC, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     } else {
-      __hip_atomic_store(&B.scanStatus[tile], A_FLAG | (aggF << 31) | aggC, __ATOMIC_RELEASE,
+      __hip_atomic_store(&B.scanStatus[tile], A_FLAG | (aggF << 31) | aggC, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       int j = tile - 1;
       while (true) {
         unsigned long long s;
         do {
-          s = __hip_atomic_load(&B.scanStatus[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+          s = __hip_atomic_load(&B.scanStatus[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } while ((s >> 62) == 0ull);
         exC += s & 0x7FFFFFFFull;
         exF += (s >> 31) & 0x7FFFFFFFull;
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(STEP_THREADS) void k_classify_scan(DevBufs B, DevPa
         j--;
       }
       __hip_atomic_store(&B.scanStatus[tile],
-                         P_FLAG | ((exF + aggF) << 31) | (exC + aggC), __ATOMIC_RELEASE,
+                         P_FLAG | ((exF + aggF) << 31) | (exC + aggC), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
     }
     prefixS = exC | (exF << 32);
@@ -311,108 +313,121 @@ __global__ void k_scatter(DevBufs B, DevParams P) {
 }
 
 // ============================================================================= k_p2g
-// particlesToGrid (solver3d.ts:410-431) as a gather: each face sums the parcels of the cells whose
-// clamped stencils can reach it, in 64-bit fixed point; then uPrev = u (:295), gravity on v faces
-// next to FLUID (:433-443), wall velocity on faces next to SOLID (:446-471) and the validity
-// mask (:545-562).  Extra threads after the faces compute the cell-centre parcel density of FLUID
-// cells (densityCorrection, :635-642).
-__device__ __forceinline__ bool corner_w(float g, int s, int f, float &w) {
-  const Stencil1 st = stencil1(g, s);
-  if (st.i == f) {
-    w = 1.f - st.f;
-    return true;
+// particlesToGrid (solver3d.ts:410-431) per tile: a block owns the faces and cells of one
+// P2G_TX x P2G_TY x P2G_TZ brick of the index space, streams the parcels of the cells that can
+// reach them (the brick plus a one-cell halo, from the cell bins) and adds each owned corner's
+// w*v and w into LDS as 64-bit fixed point (integer sums are associative: the result does not
+// depend on the order of parcels or threads).  The epilogue normalises the owned faces, stores
+// uPrev = u (:295), applies gravity on v faces next to FLUID (:433-443), the wall velocity on faces
+// next to SOLID (:446-471) and the validity mask (:545-562), and writes the cell-centre parcel
+// density of FLUID cells (densityCorrection, :635-642).
+#define P2G_TILE (P2G_TX * P2G_TY * P2G_TZ)
+#define P2G_MAXROWS ((P2G_TY + 2) * (P2G_TZ + 2))
+
+__device__ __forceinline__ void p2g_corners(long long (*acc)[2], const int lo[3], int sx, int sy,
+                                            int sz, float gx, float gy, float gz, float vel,
+                                            bool withVel) {
+  const Stencil1 a = stencil1(gx, sx), b = stencil1(gy, sy), c = stencil1(gz, sz);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
+    const int li = a.i + dx - lo[0], lj = b.i + dy - lo[1], lk = c.i + dz - lo[2];
+    if (li < 0 || li >= P2G_TX || lj < 0 || lj >= P2G_TY || lk < 0 || lk >= P2G_TZ) continue;
+    // corner weight in the reference's order: x factor * y factor * z factor (:369-376)
+    const float w = (dx ? a.f : 1.f - a.f) * (dy ? b.f : 1.f - b.f) * (dz ? c.f : 1.f - c.f);
+    const int e = li + P2G_TX * (lj + P2G_TY * lk);
+    if (withVel) atomicAdd(reinterpret_cast<unsigned long long *>(&acc[e][0]),
+                           static_cast<unsigned long long>(fx40(w * vel)));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&acc[e][1]),
+              static_cast<unsigned long long>(fx40(w)));
   }
-  if (st.i + 1 == f) {
-    w = st.f;
-    return true;
-  }
-  return false;
 }
 
-__global__ __launch_bounds__(STEP_THREADS) void k_p2g(DevBufs B, DevParams P) {
+__global__ __launch_bounds__(P2G_THREADS) void k_p2g(DevBufs B, DevParams P) {
   if (!is_active(B)) return;
   const DevCtl &C = *B.ctl;
-  int id = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ long long acc[4][P2G_TILE][2];  // u, v, w faces (w*v, w); cells (-, w)
+  __shared__ int rowOff[P2G_MAXROWS + 1];
+  __shared__ int rowStart[P2G_MAXROWS];
   const int nx = P.nx, ny = P.ny, nz = P.nz;
-  if (id < P.nfaces) {
-    int comp, i, j, k, sx, sy, sz;
-    float ox, oy, oz;
-    int f = id;
-    if (f < P.nu) {
-      comp = 0; sx = nx + 1; sy = ny; sz = nz; ox = 0.f; oy = 0.5f; oz = 0.5f;
-    } else if (f < P.nu + P.nv) {
-      f -= P.nu;
-      comp = 1; sx = nx; sy = ny + 1; sz = nz; ox = 0.5f; oy = 0.f; oz = 0.5f;
-    } else {
-      f -= P.nu + P.nv;
-      comp = 2; sx = nx; sy = ny; sz = nz + 1; ox = 0.5f; oy = 0.5f; oz = 0.f;
+  const int tx = (nx + P2G_TX) / P2G_TX, ty = (ny + P2G_TY) / P2G_TY;  // ceil((n+1)/T)
+  const int bt = blockIdx.x;
+  const int lo[3] = {(bt % tx) * P2G_TX, ((bt / tx) % ty) * P2G_TY, (bt / (tx * ty)) * P2G_TZ};
+  for (int e = threadIdx.x; e < 4 * P2G_TILE * 2; e += blockDim.x) (&acc[0][0][0])[e] = 0;
+  // candidate rows (cy, cz) and the x-run of cells of each row
+  const int xa = max(lo[0] - 1, 0), xb = min(lo[0] + P2G_TX, nx - 1);
+  const int ya = max(lo[1] - 1, 0), yb = min(lo[1] + P2G_TY, ny - 1);
+  const int za = max(lo[2] - 1, 0), zb = min(lo[2] + P2G_TZ, nz - 1);
+  const int nyr = max(yb - ya + 1, 0), nrows = nyr * max(zb - za + 1, 0);
+  if (threadIdx.x < nrows) {
+    const int cy = ya + threadIdx.x % nyr, cz = za + threadIdx.x / nyr;
+    const int r0 = B.binStart[cidx(P, xa, cy, cz)];
+    rowStart[threadIdx.x] = r0;
+    rowOff[threadIdx.x + 1] = B.binStart[cidx(P, xb, cy, cz) + 1] - r0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    rowOff[0] = 0;
+    for (int r = 1; r <= nrows; r++) rowOff[r] += rowOff[r - 1];
+  }
+  __syncthreads();
+  const int total = rowOff[nrows];
+  const float invh = P.invh;
+  // Parcels of one cell are adjacent in the bins and hit the same corners; within each full run
+  // of 512 slots, slot g takes parcel (g % 64) * 8 + (g / 64) % 8, so the 64 lanes of a wave are
+  // 8 parcels apart and their LDS atomics rarely meet on one address.
+  const int full = total & ~511;
+  for (int g = threadIdx.x; g < total; g += blockDim.x) {
+    const int p = g < full ? (g & ~511) | ((g & 63) << 3) | ((g >> 6) & 7) : g;
+    int lo_r = 0, hi_r = nrows - 1;  // the row holding parcel p: rowOff[r] <= p < rowOff[r + 1]
+    while (lo_r < hi_r) {
+      const int mid = (lo_r + hi_r + 1) >> 1;
+      if (rowOff[mid] <= p) lo_r = mid; else hi_r = mid - 1;
     }
-    i = f % sx;
-    j = (f / sx) % sy;
-    k = f / (sx * sy);
-    // candidate cell ranges per dimension: offset 0 -> {f-1, f}, offset 1/2 -> {f-1, f, f+1}
-    const int xa = max(i - 1, 0), xb = min(comp == 0 ? i : i + 1, nx - 1);
-    const int ya = max(j - 1, 0), yb = min(comp == 1 ? j : j + 1, ny - 1);
-    const int za = max(k - 1, 0), zb = min(comp == 2 ? k : k + 1, nz - 1);
-    long long sw = 0, swv = 0;
-    for (int cz = za; cz <= zb; cz++)
-      for (int cy = ya; cy <= yb; cy++) {
-        const int r0 = B.binStart[cidx(P, xa, cy, cz)];
-        const int r1 = B.binStart[cidx(P, xb, cy, cz) + 1];
-        for (int q = r0; q < r1; q++) {
-          const float4 pp = B.binPos[q];
-          float wx, wy, wz;
-          if (!corner_w(pp.x * P.invh - ox, sx, i, wx)) continue;
-          if (!corner_w(pp.y * P.invh - oy, sy, j, wy)) continue;
-          if (!corner_w(pp.z * P.invh - oz, sz, k, wz)) continue;
-          const float4 vv = B.binVel[q];
-          const float vel = comp == 0 ? vv.x : (comp == 1 ? vv.y : vv.z);
-          const float w = wx * wy * wz;
-          sw += fx40(w);
-          swv += fx40(w * vel);
-        }
+    const int q = rowStart[lo_r] + (p - rowOff[lo_r]);
+    const float4 pp = B.binPos[q];
+    const float4 vv = B.binVel[q];
+    const float gx = pp.x * invh, gy = pp.y * invh, gz = pp.z * invh;
+    p2g_corners(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true);
+    p2g_corners(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true);
+    p2g_corners(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true);
+    p2g_corners(acc[3], lo, nx, ny, nz, gx - 0.5f, gy - 0.5f, gz - 0.5f, 0.f, false);
+  }
+  __syncthreads();
+  // epilogue over the owned faces of the three components and the owned cells
+  for (int e = threadIdx.x; e < 4 * P2G_TILE; e += blockDim.x) {
+    const int comp = e / P2G_TILE, le = e % P2G_TILE;
+    const int i = lo[0] + le % P2G_TX, j = lo[1] + (le / P2G_TX) % P2G_TY,
+              k = lo[2] + le / (P2G_TX * P2G_TY);
+    if (comp == 3) {
+      if (i < nx && j < ny && k < nz) {
+        const int c = cidx(P, i, j, k);
+        if (B.flag[c] == PL3D_FLUID_) B.density[c] = float(double(acc[3][le][1]) / P2G_SCALE);
       }
+      continue;
+    }
+    const int sx = nx + (comp == 0), sy = ny + (comp == 1), sz = nz + (comp == 2);
+    if (i >= sx || j >= sy || k >= sz) continue;
+    const int id = (comp == 0 ? 0 : (comp == 1 ? P.nu : P.nu + P.nv)) + i + sx * (j + sy * k);
+    const long long sw = acc[comp][le][1], swv = acc[comp][le][0];
     float val = sw > (long long)(1e-9 * P2G_SCALE) ? float(double(swv) / double(sw)) : 0.f;
     B.uPrev[id] = val;
     // neighbouring cells across the face (outside the grid counts as SOLID)
     int ia = i, ja = j, ka = k;
     if (comp == 0) ia--; else if (comp == 1) ja--; else ka--;
-    const uint8_t a = (ia >= 0 && ja >= 0 && ka >= 0) ? B.flag[cidx(P, ia, ja, ka)] : PL3D_SOLID_;
-    const uint8_t b = (i < nx && j < ny && k < nz) ? B.flag[cidx(P, i, j, k)] : PL3D_SOLID_;
-    if (comp == 1 && j >= 1 && j <= ny - 1 && (a == PL3D_FLUID_ || b == PL3D_FLUID_))
+    const uint8_t fa = (ia >= 0 && ja >= 0 && ka >= 0) ? B.flag[cidx(P, ia, ja, ka)] : PL3D_SOLID_;
+    const uint8_t fb = (i < nx && j < ny && k < nz) ? B.flag[cidx(P, i, j, k)] : PL3D_SOLID_;
+    if (comp == 1 && j >= 1 && j <= ny - 1 && (fa == PL3D_FLUID_ || fb == PL3D_FLUID_))
       val -= float(P.gravity * C.dt);
-    if (a == PL3D_SOLID_ || b == PL3D_SOLID_) {
+    if (fa == PL3D_SOLID_ || fb == PL3D_SOLID_) {
       if (comp == 0) val = float(-C.omega * ((j + 0.5) * P.hd - P.pivotWY));
       else if (comp == 1) val = float(C.omega * ((i + 0.5) * P.hd - P.pivotWX));
       else val = 0.f;
     }
     B.u[id] = val;
-    B.valid[id] = (a == PL3D_SOLID_ || b == PL3D_SOLID_ || a == PL3D_FLUID_ || b == PL3D_FLUID_)
+    B.valid[id] = (fa == PL3D_SOLID_ || fb == PL3D_SOLID_ || fa == PL3D_FLUID_ || fb == PL3D_FLUID_)
                       ? 1 : 0;
-    return;
   }
-  id -= P.nfaces;
-  if (id >= P.nc) return;
-  if (B.flag[id] != PL3D_FLUID_) return;
-  const int i = id % nx, j = (id / nx) % ny, k = id / (nx * ny);
-  const int xa = max(i - 1, 0), xb = min(i + 1, nx - 1);
-  const int ya = max(j - 1, 0), yb = min(j + 1, ny - 1);
-  const int za = max(k - 1, 0), zb = min(k + 1, nz - 1);
-  long long sw = 0;
-  for (int cz = za; cz <= zb; cz++)
-    for (int cy = ya; cy <= yb; cy++) {
-      const int r0 = B.binStart[cidx(P, xa, cy, cz)];
-      const int r1 = B.binStart[cidx(P, xb, cy, cz) + 1];
-      for (int q = r0; q < r1; q++) {
-        const float4 pp = B.binPos[q];
-        float wx, wy, wz;
-        if (!corner_w(pp.x * P.invh - 0.5f, nx, i, wx)) continue;
-        if (!corner_w(pp.y * P.invh - 0.5f, ny, j, wy)) continue;
-        if (!corner_w(pp.z * P.invh - 0.5f, nz, k, wz)) continue;
-        sw += fx40(wx * wy * wz);
-      }
-    }
-  B.density[id] = float(double(sw) / P2G_SCALE);
 }
 
 // ============================================================================= k_liquid
