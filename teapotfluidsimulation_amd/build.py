@@ -1,4 +1,6 @@
-"""Build libpourlab3d.so in-tree for gfx950 with hipcc (no JIT cache, no torch extension).
+"""Build libpourlab3d.so in-tree for gfx950 with hipcc (no JIT cache, no torch extension), then
+the Node binding node/pourlab3d.node with gcc against the system's node_api.h (skipped with a
+message when that header is absent).
 
 Usage: python -m teapotfluidsimulation_amd.build [--jobs N]
 """
@@ -14,6 +16,11 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpourlab3d.so")
 ARCH = os.environ.get("PL3D_ARCH", "gfx950")
+
+NODE_DIR = os.path.join(HERE, "node")
+NODE_ADDON = os.path.join(NODE_DIR, "pourlab3d.node")
+NODE_INCLUDES = ["/usr/include/node", "/usr/local/include/node"]
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["pl3d_solver.hip", "pl3d_step.hip", "pl3d_api.cpp", "pl3d_pcg_api.cpp"]
 HIPCC_FLAGS = [
@@ -54,7 +61,26 @@ def build(jobs: int = 4) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_node_addon()
     return LIB
+
+
+def build_node_addon() -> str:
+    """pourlab3d.node: the N-API layer over libpourlab3d.so (rpath $ORIGIN/..)."""
+    inc = next((d for d in NODE_INCLUDES if os.path.exists(os.path.join(d, "node_api.h"))), None)
+    if inc is None:
+        print("build: node_api.h not found; Node binding not built", file=sys.stderr)
+        return ""
+    tmp = NODE_ADDON + ".tmp"
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter",
+           "-DNODE_GYP_MODULE_NAME=pourlab3d", f"-I{inc}", f"-I{INCLUDE}",
+           os.path.join(NODE_DIR, "pourlab3d_napi.c"), f"-L{HERE}", "-lpourlab3d",
+           "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Node binding build failed:\n{r.stderr}")
+    os.replace(tmp, NODE_ADDON)
+    return NODE_ADDON
 
 
 def main():

@@ -1,0 +1,699 @@
+/* pourlab3d_napi.c — N-API binding of libpourlab3d.so for Node.
+ *
+ * The reference's host is TypeScript on Node (package.json:9-11, benchmarks/run3d.ts).  This addon
+ * is the thin layer between that host and the C ABI of include/pourlab3d.h: every function takes
+ * and returns plain JS values (numbers, objects in the reference's camelCase shapes, typed arrays)
+ * and calls exactly one pl3d_* entry point.  Errors surface as thrown JS Errors carrying
+ * pl3d_last_error().  The JS façade (src/sim3d/solver3d.js, src/sim3d/presets3d.js) wraps these
+ * functions in the reference's classes.
+ *
+ * Built by teapotfluidsimulation_amd/build.py with gcc against the system's node_api.h; the
+ * resulting pourlab3d.node finds libpourlab3d.so through an $ORIGIN rpath.
+ */
+#define NAPI_VERSION 4
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pourlab3d.h"
+
+#define CHECK_NAPI(call)                                                                      \
+  do {                                                                                        \
+    if ((call) != napi_ok) {                                                                  \
+      napi_throw_error(env, NULL, "pourlab3d: N-API call failed: " #call);                    \
+      return NULL;                                                                            \
+    }                                                                                         \
+  } while (0)
+
+/* Throws the library's message for a non-OK return code. */
+static napi_value throw_pl3d(napi_env env, int rc) {
+  char code[16];
+  snprintf(code, sizeof code, "%d", rc);
+  napi_throw_error(env, code, pl3d_last_error());
+  return NULL;
+}
+
+#define CHECK_PL3D(call)                    \
+  do {                                      \
+    int rc_ = (call);                       \
+    if (rc_ != PL3D_OK) return throw_pl3d(env, rc_); \
+  } while (0)
+
+/* ------------------------------------------------------------------ small value helpers */
+
+static int set_num(napi_env env, napi_value obj, const char *key, double v) {
+  napi_value x;
+  if (napi_create_double(env, v, &x) != napi_ok) return 0;
+  return napi_set_named_property(env, obj, key, x) == napi_ok;
+}
+
+static int set_str(napi_env env, napi_value obj, const char *key, const char *s) {
+  napi_value x;
+  if (napi_create_string_utf8(env, s, NAPI_AUTO_LENGTH, &x) != napi_ok) return 0;
+  return napi_set_named_property(env, obj, key, x) == napi_ok;
+}
+
+static int set_val(napi_env env, napi_value obj, const char *key, napi_value x) {
+  return napi_set_named_property(env, obj, key, x) == napi_ok;
+}
+
+static napi_value num_array(napi_env env, const double *v, int n) {
+  napi_value a, x;
+  if (napi_create_array_with_length(env, (size_t)n, &a) != napi_ok) return NULL;
+  for (int i = 0; i < n; i++) {
+    if (napi_create_double(env, v[i], &x) != napi_ok) return NULL;
+    if (napi_set_element(env, a, (uint32_t)i, x) != napi_ok) return NULL;
+  }
+  return a;
+}
+
+/* Reads obj[key] as a number; throws TypeError naming the key when it is missing or not a
+ * number (the reference's Config3D has no optional numeric fields). */
+static int get_num(napi_env env, napi_value obj, const char *key, double *out) {
+  napi_value v;
+  napi_valuetype t;
+  bool has = false;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) goto bad;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) goto bad;
+  if (napi_typeof(env, v, &t) != napi_ok || t != napi_number) goto bad;
+  return napi_get_value_double(env, v, out) == napi_ok;
+bad: {
+  char msg[160];
+  snprintf(msg, sizeof msg, "Config3D: '%s' must be a number", key);
+  napi_throw_type_error(env, NULL, msg);
+  return 0;
+}
+}
+
+static int get_obj(napi_env env, napi_value obj, const char *key, napi_value *out) {
+  napi_valuetype t;
+  bool has = false;
+  if (napi_has_named_property(env, obj, key, &has) == napi_ok && has &&
+      napi_get_named_property(env, obj, key, out) == napi_ok && napi_typeof(env, *out, &t) == napi_ok &&
+      t == napi_object)
+    return 1;
+  char msg[160];
+  snprintf(msg, sizeof msg, "Config3D: '%s' must be an object", key);
+  napi_throw_type_error(env, NULL, msg);
+  return 0;
+}
+
+/* Optional string property: copies into buf, returns 1 if present, 0 if absent, -1 on error. */
+static int get_opt_str(napi_env env, napi_value obj, const char *key, char *buf, size_t cap) {
+  bool has = false;
+  napi_value v;
+  napi_valuetype t;
+  size_t len;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok) return -1;
+  if (!has) return 0;
+  if (napi_get_named_property(env, obj, key, &v) != napi_ok) return -1;
+  if (napi_typeof(env, v, &t) != napi_ok) return -1;
+  if (t == napi_undefined) return 0;
+  if (t != napi_string || napi_get_value_string_utf8(env, v, buf, cap, &len) != napi_ok) {
+    char msg[160];
+    snprintf(msg, sizeof msg, "Config3D: '%s' must be a string", key);
+    napi_throw_type_error(env, NULL, msg);
+    return -1;
+  }
+  return 1;
+}
+
+static const char *MOTION_KINDS[] = {"still", "pour", "tilt", "manual"};
+static const char *PRECONDS[] = {"mic0", "mg", "jacobi"};
+static const char *TRANSFERS[] = {"flip", "apic"};
+
+static int name_index(const char *const *names, int n, const char *s) {
+  for (int i = 0; i < n; i++)
+    if (strcmp(names[i], s) == 0) return i;
+  return -1;
+}
+
+/* ------------------------------------------------------------------ Config3D <-> pl3d_config */
+
+/* Config3D (sim3d/solver3d.ts:25-43) as a JS object -> pl3d_config.  Device-only keys
+ * 'preconditioner' and 'transfer' are optional strings. */
+static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
+  napi_value t, m, pv, x;
+  double d;
+  char buf[32];
+  memset(c, 0, sizeof *c);
+  if (!get_obj(env, o, "teapot", &t) || !get_obj(env, o, "motion", &m)) return 0;
+#define NUM(obj, key, dst) \
+  if (!get_num(env, obj, key, &d)) return 0; \
+  dst = d
+  NUM(t, "radius", c->teapot.radius);
+  NUM(t, "height", c->teapot.height);
+  NUM(t, "wall", c->teapot.wall);
+  NUM(t, "inletHeight", c->teapot.inlet_height);
+  NUM(t, "boreRadius", c->teapot.bore_radius);
+  NUM(t, "spoutAngleDeg", c->teapot.spout_angle_deg);
+  NUM(t, "spoutLength", c->teapot.spout_length);
+  NUM(o, "h", c->h);
+  NUM(o, "initialVolume", c->initial_volume);
+  NUM(m, "startDelay", c->motion.start_delay);
+  NUM(m, "tiltDuration", c->motion.tilt_duration);
+  NUM(m, "holdAngleDeg", c->motion.hold_angle_deg);
+  NUM(m, "holdTiltRate", c->motion.hold_tilt_rate);
+  NUM(m, "maxAngleDeg", c->motion.max_angle_deg);
+  NUM(m, "returnDuration", c->motion.return_duration);
+  NUM(m, "maxHold", c->motion.max_hold);
+  NUM(m, "holdTime", c->motion.hold_time);
+  NUM(m, "targetVolume", c->motion.target_volume);
+  NUM(m, "anticipation", c->motion.anticipation);
+  NUM(m, "repeats", c->motion.repeats);
+  NUM(m, "repeatGap", c->motion.repeat_gap);
+  NUM(m, "settleTime", c->motion.settle_time);
+  NUM(o, "gravity", c->gravity);
+  NUM(o, "flipRatio", c->flip_ratio);
+  NUM(o, "densityCorrection", c->density_correction);
+  NUM(o, "cfl", c->cfl);
+  NUM(o, "maxSubstep", c->max_substep);
+  NUM(o, "pressureTolerance", c->pressure_tolerance);
+  NUM(o, "pressureMaxIterations", c->pressure_max_iterations);
+  NUM(o, "seed", c->seed);
+  NUM(o, "duration", c->duration);
+#undef NUM
+  {
+    int r = get_opt_str(env, m, "kind", buf, sizeof buf);
+    if (r < 0) return 0;
+    int k = r == 0 ? -1 : name_index(MOTION_KINDS, 4, buf);
+    if (k < 0) {
+      napi_throw_type_error(env, NULL, "Config3D: motion.kind must be 'still', 'pour', 'tilt' or 'manual'");
+      return 0;
+    }
+    c->motion.kind = k;
+  }
+  {
+    bool isArr = false;
+    uint32_t len = 0;
+    if (napi_get_named_property(env, o, "pivot", &pv) != napi_ok ||
+        napi_is_array(env, pv, &isArr) != napi_ok || !isArr ||
+        napi_get_array_length(env, pv, &len) != napi_ok || len != 2) {
+      napi_throw_type_error(env, NULL, "Config3D: 'pivot' must be [x, y]");
+      return 0;
+    }
+    for (uint32_t i = 0; i < 2; i++) {
+      if (napi_get_element(env, pv, i, &x) != napi_ok || napi_get_value_double(env, x, &d) != napi_ok) {
+        napi_throw_type_error(env, NULL, "Config3D: 'pivot' must hold numbers");
+        return 0;
+      }
+      c->pivot[i] = d;
+    }
+  }
+  {
+    int r = get_opt_str(env, o, "preconditioner", buf, sizeof buf);
+    if (r < 0) return 0;
+    c->preconditioner = r == 0 ? PL3D_PRECOND_MIC0 : name_index(PRECONDS, 3, buf);
+    if (c->preconditioner < 0) {
+      napi_throw_type_error(env, NULL, "Config3D: preconditioner must be 'mic0', 'mg' or 'jacobi'");
+      return 0;
+    }
+    r = get_opt_str(env, o, "transfer", buf, sizeof buf);
+    if (r < 0) return 0;
+    c->transfer = r == 0 ? PL3D_TRANSFER_FLIP : name_index(TRANSFERS, 2, buf);
+    if (c->transfer < 0) {
+      napi_throw_type_error(env, NULL, "Config3D: transfer must be 'flip' or 'apic'");
+      return 0;
+    }
+  }
+  return 1;
+}
+
+static napi_value config_to_js(napi_env env, const pl3d_config *c) {
+  napi_value o, t, m, pv;
+  CHECK_NAPI(napi_create_object(env, &o));
+  CHECK_NAPI(napi_create_object(env, &t));
+  CHECK_NAPI(napi_create_object(env, &m));
+  int ok = set_num(env, t, "radius", c->teapot.radius) && set_num(env, t, "height", c->teapot.height) &&
+           set_num(env, t, "wall", c->teapot.wall) &&
+           set_num(env, t, "inletHeight", c->teapot.inlet_height) &&
+           set_num(env, t, "boreRadius", c->teapot.bore_radius) &&
+           set_num(env, t, "spoutAngleDeg", c->teapot.spout_angle_deg) &&
+           set_num(env, t, "spoutLength", c->teapot.spout_length) && set_val(env, o, "teapot", t) &&
+           set_num(env, o, "h", c->h) && set_num(env, o, "initialVolume", c->initial_volume) &&
+           set_str(env, m, "kind",
+                   c->motion.kind >= 0 && c->motion.kind < 4 ? MOTION_KINDS[c->motion.kind] : "pour") &&
+           set_num(env, m, "startDelay", c->motion.start_delay) &&
+           set_num(env, m, "tiltDuration", c->motion.tilt_duration) &&
+           set_num(env, m, "holdAngleDeg", c->motion.hold_angle_deg) &&
+           set_num(env, m, "holdTiltRate", c->motion.hold_tilt_rate) &&
+           set_num(env, m, "maxAngleDeg", c->motion.max_angle_deg) &&
+           set_num(env, m, "returnDuration", c->motion.return_duration) &&
+           set_num(env, m, "maxHold", c->motion.max_hold) &&
+           set_num(env, m, "holdTime", c->motion.hold_time) &&
+           set_num(env, m, "targetVolume", c->motion.target_volume) &&
+           set_num(env, m, "anticipation", c->motion.anticipation) &&
+           set_num(env, m, "repeats", c->motion.repeats) &&
+           set_num(env, m, "repeatGap", c->motion.repeat_gap) &&
+           set_num(env, m, "settleTime", c->motion.settle_time) && set_val(env, o, "motion", m);
+  if (!ok) {
+    napi_throw_error(env, NULL, "pourlab3d: building Config3D failed");
+    return NULL;
+  }
+  pv = num_array(env, c->pivot, 2);
+  if (!pv) return NULL;
+  ok = set_val(env, o, "pivot", pv) && set_num(env, o, "gravity", c->gravity) &&
+       set_num(env, o, "flipRatio", c->flip_ratio) &&
+       set_num(env, o, "densityCorrection", c->density_correction) && set_num(env, o, "cfl", c->cfl) &&
+       set_num(env, o, "maxSubstep", c->max_substep) &&
+       set_num(env, o, "pressureTolerance", c->pressure_tolerance) &&
+       set_num(env, o, "pressureMaxIterations", c->pressure_max_iterations) &&
+       set_num(env, o, "seed", c->seed) && set_num(env, o, "duration", c->duration);
+  if (!ok) {
+    napi_throw_error(env, NULL, "pourlab3d: building Config3D failed");
+    return NULL;
+  }
+  return o;
+}
+
+/* Diagnostics3D (sim3d/solver3d.ts:45-60, filled as :796-814) */
+static napi_value diagnostics_to_js(napi_env env, const pl3d_diagnostics *d) {
+  napi_value o;
+  const double grid[3] = {d->grid[0], d->grid[1], d->grid[2]};
+  CHECK_NAPI(napi_create_object(env, &o));
+  napi_value g = num_array(env, grid, 3);
+  if (!g) return NULL;
+  int ok = set_num(env, o, "time", d->time) && set_num(env, o, "steps", (double)d->steps) &&
+           set_num(env, o, "angleDeg", d->angle_deg) &&
+           set_str(env, o, "phase", pl3d_phase_name(d->phase)) &&
+           set_num(env, o, "particles", d->particles) && set_val(env, o, "grid", g) &&
+           set_num(env, o, "dischargedMl", d->discharged_ml) &&
+           set_num(env, o, "spilledMl", d->spilled_ml) && set_num(env, o, "lostMl", d->lost_ml) &&
+           set_num(env, o, "inPotMl", d->in_pot_ml) &&
+           set_num(env, o, "pressureIterations", d->pressure_iterations) &&
+           set_num(env, o, "pressureFailures", d->pressure_failures) &&
+           set_num(env, o, "maxDivergence", d->max_divergence) &&
+           set_num(env, o, "interiorDensity", d->interior_density);
+  if (!ok) {
+    napi_throw_error(env, NULL, "pourlab3d: building Diagnostics3D failed");
+    return NULL;
+  }
+  return o;
+}
+
+/* ------------------------------------------------------------------ argument helpers */
+
+static int args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok || argc < want) {
+    napi_throw_type_error(env, NULL, "pourlab3d: too few arguments");
+    return 0;
+  }
+  return 1;
+}
+
+static void finalize_sim(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  pl3d_destroy((pl3d_sim *)data);
+}
+
+/* Handles are JS objects wrapping an external; destroy() clears the pointer so a later call
+ * raises instead of touching freed memory. */
+typedef struct {
+  pl3d_sim *sim;
+} handle_box;
+
+static void finalize_box(napi_env env, void *data, void *hint) {
+  handle_box *b = (handle_box *)data;
+  if (b->sim) finalize_sim(env, b->sim, hint);
+  free(b);
+}
+
+static int get_sim(napi_env env, napi_value v, pl3d_sim **out) {
+  handle_box *b = NULL;
+  napi_valuetype t;
+  if (napi_typeof(env, v, &t) != napi_ok || t != napi_external ||
+      napi_get_value_external(env, v, (void **)&b) != napi_ok || !b) {
+    napi_throw_type_error(env, NULL, "pourlab3d: expected a simulation handle");
+    return 0;
+  }
+  if (!b->sim) {
+    napi_throw_error(env, NULL, "pourlab3d: simulation was destroyed");
+    return 0;
+  }
+  *out = b->sim;
+  return 1;
+}
+
+static int get_double(napi_env env, napi_value v, double *out, const char *what) {
+  if (napi_get_value_double(env, v, out) != napi_ok) {
+    char msg[128];
+    snprintf(msg, sizeof msg, "pourlab3d: %s must be a number", what);
+    napi_throw_type_error(env, NULL, msg);
+    return 0;
+  }
+  return 1;
+}
+
+/* A typed array argument of the given element type; returns its data pointer and length. */
+static int get_typed(napi_env env, napi_value v, napi_typedarray_type want, void **data, size_t *len,
+                     const char *what) {
+  bool is = false;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is ||
+      napi_get_typedarray_info(env, v, &t, len, data, &ab, &off) != napi_ok || t != want) {
+    char msg[128];
+    snprintf(msg, sizeof msg, "pourlab3d: %s has the wrong typed-array type", what);
+    napi_throw_type_error(env, NULL, msg);
+    return 0;
+  }
+  return 1;
+}
+
+/* A new typed array of n elements; *data receives the backing store. */
+static napi_value new_typed(napi_env env, napi_typedarray_type t, size_t elem, size_t n, void **data) {
+  napi_value ab, ta;
+  CHECK_NAPI(napi_create_arraybuffer(env, elem * (n > 0 ? n : 1), data, &ab));
+  CHECK_NAPI(napi_create_typedarray(env, t, n, ab, 0, &ta));
+  return ta;
+}
+
+/* ------------------------------------------------------------------ exported functions */
+
+static napi_value js_version(napi_env env, napi_callback_info info) {
+  (void)info;
+  napi_value s;
+  CHECK_NAPI(napi_create_string_utf8(env, pl3d_version(), NAPI_AUTO_LENGTH, &s));
+  return s;
+}
+
+/* gentlePour(cellMm, seconds) -> Config3D (presets3d.ts:9-26) */
+static napi_value js_gentle_pour(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  double cell, seconds;
+  pl3d_config c;
+  if (!args(env, info, 2, argv) || !get_double(env, argv[0], &cell, "cellMm") ||
+      !get_double(env, argv[1], &seconds, "seconds"))
+    return NULL;
+  CHECK_PL3D(pl3d_gentle_pour(cell, seconds, &c));
+  return config_to_js(env, &c);
+}
+
+/* create(cfg) -> handle */
+static napi_value js_create(napi_env env, napi_callback_info info) {
+  napi_value argv[1], ext;
+  pl3d_config c;
+  pl3d_sim *sim = NULL;
+  if (!args(env, info, 1, argv) || !config_from_js(env, argv[0], &c)) return NULL;
+  CHECK_PL3D(pl3d_create(&c, &sim));
+  handle_box *b = (handle_box *)malloc(sizeof *b);
+  if (!b) {
+    pl3d_destroy(sim);
+    napi_throw_error(env, NULL, "pourlab3d: out of host memory");
+    return NULL;
+  }
+  b->sim = sim;
+  if (napi_create_external(env, b, finalize_box, NULL, &ext) != napi_ok) {
+    pl3d_destroy(sim);
+    free(b);
+    napi_throw_error(env, NULL, "pourlab3d: napi_create_external failed");
+    return NULL;
+  }
+  return ext;
+}
+
+static napi_value js_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  handle_box *b = NULL;
+  if (!args(env, info, 1, argv)) return NULL;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], (void **)&b));
+  if (b && b->sim) {
+    pl3d_sim *s = b->sim;
+    b->sim = NULL;
+    CHECK_PL3D(pl3d_destroy(s));
+  }
+  return NULL;
+}
+
+/* info(h) -> the readonly facts of Simulation3D (solver3d.ts:63-85) */
+static napi_value js_info(napi_env env, napi_callback_info info) {
+  napi_value argv[1], o;
+  pl3d_sim *sim;
+  pl3d_info in;
+  if (!args(env, info, 1, argv) || !get_sim(env, argv[0], &sim)) return NULL;
+  CHECK_PL3D(pl3d_get_info(sim, &in));
+  CHECK_NAPI(napi_create_object(env, &o));
+  const double bands[3] = {in.band_counts[0], in.band_counts[1], in.band_counts[2]};
+  napi_value pw = num_array(env, in.pivot_world, 2), bc = num_array(env, bands, 3);
+  if (!pw || !bc) return NULL;
+  int ok = set_num(env, o, "n", in.n) && set_num(env, o, "nx", in.nx) && set_num(env, o, "ny", in.ny) &&
+           set_num(env, o, "nz", in.nz) && set_num(env, o, "h", in.h) &&
+           set_num(env, o, "fillHeight", in.fill_height) && set_val(env, o, "pivotWorld", pw) &&
+           set_num(env, o, "parcelVolume", in.parcel_volume) && set_val(env, o, "bandCounts", bc) &&
+           set_num(env, o, "maxLiquidCells", in.max_liquid_cells);
+  if (!ok) {
+    napi_throw_error(env, NULL, "pourlab3d: building info failed");
+    return NULL;
+  }
+  return o;
+}
+
+static napi_value js_step(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  pl3d_sim *sim;
+  double dt;
+  if (!args(env, info, 2, argv) || !get_sim(env, argv[0], &sim) || !get_double(env, argv[1], &dt, "dt"))
+    return NULL;
+  CHECK_PL3D(pl3d_step(sim, dt));
+  return NULL;
+}
+
+static napi_value js_advance(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  pl3d_sim *sim;
+  double s;
+  pl3d_diagnostics d;
+  if (!args(env, info, 2, argv) || !get_sim(env, argv[0], &sim) ||
+      !get_double(env, argv[1], &s, "seconds"))
+    return NULL;
+  CHECK_PL3D(pl3d_advance(sim, s, &d));
+  return diagnostics_to_js(env, &d);
+}
+
+static napi_value js_diagnostics(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  pl3d_sim *sim;
+  pl3d_diagnostics d;
+  if (!args(env, info, 1, argv) || !get_sim(env, argv[0], &sim)) return NULL;
+  CHECK_PL3D(pl3d_get_diagnostics(sim, &d));
+  return diagnostics_to_js(env, &d);
+}
+
+static napi_value js_stable_dt(napi_env env, napi_callback_info info) {
+  napi_value argv[1], r;
+  pl3d_sim *sim;
+  double dt;
+  if (!args(env, info, 1, argv) || !get_sim(env, argv[0], &sim)) return NULL;
+  CHECK_PL3D(pl3d_stable_dt(sim, &dt));
+  CHECK_NAPI(napi_create_double(env, dt, &r));
+  return r;
+}
+
+static napi_value js_finished(napi_env env, napi_callback_info info) {
+  napi_value argv[1], r;
+  pl3d_sim *sim;
+  int32_t f;
+  if (!args(env, info, 1, argv) || !get_sim(env, argv[0], &sim)) return NULL;
+  CHECK_PL3D(pl3d_finished(sim, &f));
+  CHECK_NAPI(napi_get_boolean(env, f != 0, &r));
+  return r;
+}
+
+/* dischargedFractions() -> [b, m, t] or null (solver3d.ts:816-823) */
+static napi_value js_discharged_fractions(napi_env env, napi_callback_info info) {
+  napi_value argv[1], r;
+  pl3d_sim *sim;
+  double f[3];
+  int32_t has;
+  if (!args(env, info, 1, argv) || !get_sim(env, argv[0], &sim)) return NULL;
+  CHECK_PL3D(pl3d_discharged_fractions(sim, f, &has));
+  if (!has) {
+    CHECK_NAPI(napi_get_null(env, &r));
+    return r;
+  }
+  return num_array(env, f, 3);
+}
+
+/* stats(h) -> Int32Array(16), the STAT block (gpu3d/layout.ts:20-37) */
+static napi_value js_stats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  pl3d_sim *sim;
+  void *data;
+  if (!args(env, info, 1, argv) || !get_sim(env, argv[0], &sim)) return NULL;
+  napi_value ta = new_typed(env, napi_int32_array, 4, PL3D_STAT_SLOTS, &data);
+  if (!ta) return NULL;
+  CHECK_PL3D(pl3d_stats(sim, (int32_t *)data));
+  return ta;
+}
+
+/* readState(h, n) -> {x, y, z, vx, vy, vz: Float64Array, status, origin: Uint8Array,
+ * y0: Float32Array} */
+static napi_value js_read_state(napi_env env, napi_callback_info info) {
+  napi_value argv[2], o;
+  pl3d_sim *sim;
+  double dn;
+  if (!args(env, info, 2, argv) || !get_sim(env, argv[0], &sim) || !get_double(env, argv[1], &dn, "n"))
+    return NULL;
+  const size_t n = (size_t)dn;
+  static const char *keys[6] = {"x", "y", "z", "vx", "vy", "vz"};
+  void *p[9];
+  napi_value a[9];
+  for (int i = 0; i < 6; i++)
+    if (!(a[i] = new_typed(env, napi_float64_array, 8, n, &p[i]))) return NULL;
+  if (!(a[6] = new_typed(env, napi_uint8_array, 1, n, &p[6])) ||
+      !(a[7] = new_typed(env, napi_uint8_array, 1, n, &p[7])) ||
+      !(a[8] = new_typed(env, napi_float32_array, 4, n, &p[8])))
+    return NULL;
+  CHECK_PL3D(pl3d_read_state(sim, (double *)p[0], (double *)p[1], (double *)p[2], (double *)p[3],
+                             (double *)p[4], (double *)p[5], (uint8_t *)p[6], (uint8_t *)p[7],
+                             (float *)p[8]));
+  CHECK_NAPI(napi_create_object(env, &o));
+  for (int i = 0; i < 6; i++)
+    if (!set_val(env, o, keys[i], a[i])) return NULL;
+  if (!set_val(env, o, "status", a[6]) || !set_val(env, o, "origin", a[7]) || !set_val(env, o, "y0", a[8]))
+    return NULL;
+  return o;
+}
+
+/* dischargeOrder(h) -> Int32Array */
+static napi_value js_discharge_order(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  pl3d_sim *sim;
+  int32_t count = 0;
+  void *data;
+  if (!args(env, info, 1, argv) || !get_sim(env, argv[0], &sim)) return NULL;
+  CHECK_PL3D(pl3d_discharge_order(sim, NULL, 0, &count));
+  napi_value ta = new_typed(env, napi_int32_array, 4, (size_t)count, &data);
+  if (!ta) return NULL;
+  if (count > 0) CHECK_PL3D(pl3d_discharge_order(sim, (int32_t *)data, count, &count));
+  return ta;
+}
+
+/* readParticles(h, n) -> {pos, vel: Float32Array(4n), status: Uint32Array(n)} (gpu3d/solver.ts:420-437) */
+static napi_value js_read_particles(napi_env env, napi_callback_info info) {
+  napi_value argv[2], o;
+  pl3d_sim *sim;
+  double dn;
+  void *pp, *pv, *ps;
+  if (!args(env, info, 2, argv) || !get_sim(env, argv[0], &sim) || !get_double(env, argv[1], &dn, "n"))
+    return NULL;
+  const size_t n = (size_t)dn;
+  napi_value a = new_typed(env, napi_float32_array, 4, 4 * n, &pp);
+  napi_value b = a ? new_typed(env, napi_float32_array, 4, 4 * n, &pv) : NULL;
+  napi_value c = b ? new_typed(env, napi_uint32_array, 4, n, &ps) : NULL;
+  if (!c) return NULL;
+  CHECK_PL3D(pl3d_read_particles(sim, (float *)pp, (float *)pv, (uint32_t *)ps));
+  CHECK_NAPI(napi_create_object(env, &o));
+  if (!set_val(env, o, "pos", a) || !set_val(env, o, "vel", b) || !set_val(env, o, "status", c)) return NULL;
+  return o;
+}
+
+/* readRenderPositions(h, n) -> Float32Array(6n) (gpu3d/shaders.ts:779-801) */
+static napi_value js_read_render_positions(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  pl3d_sim *sim;
+  double dn;
+  void *p;
+  if (!args(env, info, 2, argv) || !get_sim(env, argv[0], &sim) || !get_double(env, argv[1], &dn, "n"))
+    return NULL;
+  napi_value a = new_typed(env, napi_float32_array, 4, 6 * (size_t)dn, &p);
+  if (!a) return NULL;
+  CHECK_PL3D(pl3d_read_render_positions(sim, (float *)p));
+  return a;
+}
+
+static napi_value js_set_target(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  pl3d_sim *sim;
+  double deg;
+  if (!args(env, info, 2, argv) || !get_sim(env, argv[0], &sim) ||
+      !get_double(env, argv[1], &deg, "angleDeg"))
+    return NULL;
+  CHECK_PL3D(pl3d_set_target(sim, deg));
+  return NULL;
+}
+
+/* pcgSolve(nx, ny, nz, cell: Uint8Array, b: Float64Array, q: Float64Array, tol, maxIter,
+ * preconditioner) -> {iterations, converged, residual, trueResidual, liquidCells, kernelMs}
+ * (Pcg3D.solve, solver3d.ts:858-937) */
+static napi_value js_pcg_solve(napi_env env, napi_callback_info info) {
+  napi_value argv[9], o, cv;
+  double dims[3], tol, maxIter;
+  void *cell, *b, *q;
+  size_t lc, lb, lq;
+  char buf[16];
+  size_t len;
+  if (!args(env, info, 9, argv)) return NULL;
+  for (int i = 0; i < 3; i++)
+    if (!get_double(env, argv[i], &dims[i], "grid size")) return NULL;
+  if (!get_typed(env, argv[3], napi_uint8_array, &cell, &lc, "cell") ||
+      !get_typed(env, argv[4], napi_float64_array, &b, &lb, "b") ||
+      !get_typed(env, argv[5], napi_float64_array, &q, &lq, "q") ||
+      !get_double(env, argv[6], &tol, "tol") || !get_double(env, argv[7], &maxIter, "maxIter"))
+    return NULL;
+  if (napi_get_value_string_utf8(env, argv[8], buf, sizeof buf, &len) != napi_ok) {
+    napi_throw_type_error(env, NULL, "pourlab3d: preconditioner must be a string");
+    return NULL;
+  }
+  const int pc = name_index(PRECONDS, 3, buf);
+  if (pc < 0) {
+    napi_throw_type_error(env, NULL, "pourlab3d: preconditioner must be 'mic0', 'mg' or 'jacobi'");
+    return NULL;
+  }
+  const size_t cells = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
+  if (lc != cells || lb != cells || lq != cells) {
+    napi_throw_range_error(env, NULL, "pourlab3d: cell, b and q must hold nx*ny*nz entries");
+    return NULL;
+  }
+  pl3d_solve_result r;
+  CHECK_PL3D(pl3d_pcg_solve((int32_t)dims[0], (int32_t)dims[1], (int32_t)dims[2], (const uint8_t *)cell,
+                            (const double *)b, (double *)q, tol, (int32_t)maxIter, pc, &r));
+  CHECK_NAPI(napi_create_object(env, &o));
+  CHECK_NAPI(napi_get_boolean(env, r.converged != 0, &cv));
+  if (!set_num(env, o, "iterations", r.iterations) || !set_val(env, o, "converged", cv) ||
+      !set_num(env, o, "residual", r.residual) || !set_num(env, o, "trueResidual", r.true_residual) ||
+      !set_num(env, o, "liquidCells", r.liquid_cells) || !set_num(env, o, "kernelMs", r.kernel_ms))
+    return NULL;
+  return o;
+}
+
+/* ------------------------------------------------------------------ module */
+
+static napi_value init(napi_env env, napi_value exports) {
+  static const struct {
+    const char *name;
+    napi_callback fn;
+  } fns[] = {
+      {"version", js_version},
+      {"gentlePour", js_gentle_pour},
+      {"create", js_create},
+      {"destroy", js_destroy},
+      {"info", js_info},
+      {"step", js_step},
+      {"advance", js_advance},
+      {"diagnostics", js_diagnostics},
+      {"stableDt", js_stable_dt},
+      {"finished", js_finished},
+      {"dischargedFractions", js_discharged_fractions},
+      {"stats", js_stats},
+      {"readState", js_read_state},
+      {"dischargeOrder", js_discharge_order},
+      {"readParticles", js_read_particles},
+      {"readRenderPositions", js_read_render_positions},
+      {"setTarget", js_set_target},
+      {"pcgSolve", js_pcg_solve},
+  };
+  for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
+    napi_value f;
+    CHECK_NAPI(napi_create_function(env, fns[i].name, NAPI_AUTO_LENGTH, fns[i].fn, NULL, &f));
+    CHECK_NAPI(napi_set_named_property(env, exports, fns[i].name, f));
+  }
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init)

@@ -1,0 +1,11 @@
+'use strict';
+// gentlePour3D (src/sim3d/presets3d.ts:9-26): the gentle 400 mL pour preset as a Config3D object.
+// The values come from the library (pl3d_gentle_pour), so the JS, Python and C hosts share one
+// definition.
+const { load } = require('../addon');
+
+function gentlePour3D(cellMm, seconds) {
+  return load().gentlePour(Number(cellMm), Number(seconds));
+}
+
+module.exports = { gentlePour3D };

@@ -1,0 +1,70 @@
+"""The Node host (N-API binding + JS façade under teapotfluidsimulation_amd/node): the binding
+loads under node, exports the façade's functions, builds the same gentlePour3D preset as the
+Python host and reports bad arguments the way the reference's TypeScript would (TypeError /
+RangeError) before any device work.  No GPU compute is called here."""
+import json
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from teapotfluidsimulation_amd import gentlePour3D
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NODE_DIR = os.path.join(ROOT, "teapotfluidsimulation_amd", "node")
+ADDON = os.path.join(NODE_DIR, "pourlab3d.node")
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
+                                reason="node or the built binding is absent")
+
+EXPORTS = ["version", "gentlePour", "create", "destroy", "info", "step", "advance", "diagnostics",
+           "stableDt", "finished", "dischargedFractions", "stats", "readState", "dischargeOrder",
+           "readParticles", "readRenderPositions", "setTarget", "pcgSolve"]
+
+
+def node_json(script: str):
+    r = subprocess.run([NODE, "-e", script], cwd=NODE_DIR, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def test_binding_exports():
+    keys = node_json("console.log(JSON.stringify(Object.keys(require('./src/addon').load())))")
+    assert sorted(keys) == sorted(EXPORTS)
+
+
+def test_preset_matches_python_host():
+    for cell in (4, 5, 10):
+        js = node_json(f"console.log(JSON.stringify(require('./src/sim3d/presets3d')"
+                       f".gentlePour3D({cell}, 20)))")
+        py = gentlePour3D(cell, 20)
+        py.pop("preconditioner")
+        py.pop("transfer")
+        assert js == py
+
+
+def test_argument_errors_are_thrown_before_device_work():
+    out = node_json(r"""
+const a = require('./src/addon').load();
+const cfg = require('./src/sim3d/presets3d').gentlePour3D(5, 20);
+const res = {};
+function t(name, f) { try { f(); res[name] = 'no error'; } catch (e) { res[name] = e.constructor.name + ': ' + e.message; } }
+t('empty', () => a.create({}));
+t('kind', () => a.create(Object.assign({}, cfg, { motion: Object.assign({}, cfg.motion, { kind: 'spin' }) })));
+t('pivot', () => a.create(Object.assign({}, cfg, { pivot: [1] })));
+t('precond', () => a.create(Object.assign({}, cfg, { preconditioner: 'ilu' })));
+t('handle', () => a.advance({}, 0.5));
+t('sizes', () => a.pcgSolve(4, 4, 4, new Uint8Array(64), new Float64Array(63), new Float64Array(64), 1e-6, 10, 'mic0'));
+t('types', () => a.pcgSolve(4, 4, 4, new Float64Array(64), new Float64Array(64), new Float64Array(64), 1e-6, 10, 'mic0'));
+console.log(JSON.stringify(res));
+""")
+    assert out["empty"].startswith("TypeError") and "teapot" in out["empty"]
+    assert out["kind"].startswith("TypeError") and "motion.kind" in out["kind"]
+    assert out["pivot"].startswith("TypeError") and "pivot" in out["pivot"]
+    assert out["precond"].startswith("TypeError") and "preconditioner" in out["precond"]
+    assert out["handle"].startswith("TypeError") and "handle" in out["handle"]
+    assert out["sizes"].startswith("RangeError")
+    assert out["types"].startswith("TypeError") and "cell" in out["types"]
