@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
-"""Benchmark: the 4 mm gentle pour (gentlePour3D(4, 20), 88,856 parcels, grid 81x81x22) on one
-MI355X, reported in simulated seconds per wall second.
+"""Benchmark: the 4 mm gentle pour (gentlePour3D(4, 20), 88,856 parcels, grid 81x81x22) with the
+APIC transfer on one MI355X, reported in simulated seconds per wall second.
 
 A "step" is one advance(0.5) chunk of the pour, the unit of the reference's driver loop
 (benchmarks/run3d.ts:15-16); the default --steps 24 covers the whole pour (it ends about 11.5 s
@@ -11,9 +11,10 @@ control-block readback per batch.
 After the timed region an identical pour is replayed with per-kernel HIP events on the
 simulation's stream (timing mode, direct launches) to give each kernel's average duration; the
 roofline object is computed for the kernel with the largest total time.  The CPU baseline is the
-FP64 C++ oracle, single core, on a bounded sample of the same pour.
+FP64 C++ oracle, single core, on a bounded sample of the same pour (same transfer).
 
-Usage: python bench.py [--gpus 1] [--steps 24] [--warmup 1] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps 24] [--warmup 1] [--transfer apic|flip]
+                       [--precond jacobi|mic0] [--no-cpu-baseline]
 """
 import argparse
 import json
@@ -47,10 +48,10 @@ def cpu_info():
     return model, os.cpu_count()
 
 
-def cpu_baseline(cell, sample_s):
+def cpu_baseline(cell, sample_s, transfer):
     """The FP64 oracle on one pinned core for about sample_s wall seconds of the same pour."""
     cmd = [sys.executable, "-m", "oracle.run3d_oracle", str(cell), "20", "--max-wall",
-           str(sample_s)]
+           str(sample_s), "--transfer", transfer]
     try:
         cmd = ["taskset", "-c", "0"] + cmd
         r = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True, timeout=sample_s * 4 + 120)
@@ -67,7 +68,7 @@ def cpu_baseline(cell, sample_s):
         "cores": 1,
         "kind": "port",
         "sample": f"first {d['steps']} substeps ({d['time']:.3f} simulated s) of the same "
-                  f"{cell} mm pour, FP64 C++ oracle pinned to one core, "
+                  f"{cell} mm {transfer.upper()} pour, FP64 C++ oracle pinned to one core, "
                   f"{out['wallSeconds']:.1f} s wall; host {model}, {ncpu} CPUs",
         "reference_published": "reference TypeScript FP64 CPU: 0.007055 sim-s/wall-s (4 mm, "
                                "i9-12900H, 1 core, benchmarks/run3d.json:34,45)",
@@ -80,7 +81,8 @@ def main():
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cell", type=float, default=4.0)
-    ap.add_argument("--precond", default="mic0")
+    ap.add_argument("--precond", default="jacobi", choices=["jacobi", "mic0"])
+    ap.add_argument("--transfer", default="apic", choices=["apic", "flip"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=float, default=20.0)
     ap.add_argument("--no-kernel-replay", action="store_true")
@@ -95,6 +97,7 @@ def main():
     def make():
         cfg = gentlePour3D(a.cell, 20)
         cfg["preconditioner"] = a.precond
+        cfg["transfer"] = a.transfer
         return Simulation3D(cfg)
 
     # warmup: a throwaway pour (graph capture, first-touch, clocks)
@@ -104,6 +107,7 @@ def main():
     warm.destroy()
 
     sim = make()
+    n_parcels, grid = sim.n, (sim.nx, sim.ny, sim.nz)
     sim.diagnostics()  # synchronised
     t0 = time.perf_counter()
     for _ in range(a.steps):
@@ -157,7 +161,8 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(int(a.cell) if a.cell == int(a.cell) else a.cell, a.cpu_sample)
+        cpu = cpu_baseline(int(a.cell) if a.cell == int(a.cell) else a.cell, a.cpu_sample,
+                           a.transfer)
 
     value = sim_s / wall
     line = {
@@ -174,12 +179,16 @@ def main():
         "dtype": "f32",
         "data": "synthetic: gentlePour3D seeded parcels (mulberry32 seed 1), no external data",
         "config": {
-            "workload": f"gentle 400 mL pour, {a.cell:g} mm cells, 88,856 parcels (half pot), "
-                        "grid 81x81x22, PIC/FLIP 0.95, pressure 1e-6 / correction 1e-4",
+            "workload": f"gentle 400 mL pour, {a.cell:g} mm cells, {n_parcels:,} parcels (half "
+                        f"pot), grid {grid[0]}x{grid[1]}x{grid[2]}, "
+                        f"{'APIC' if a.transfer == 'apic' else 'PIC/FLIP 0.95'}, pressure 1e-6 / "
+                        "correction 1e-4",
+            "transfer": a.transfer,
             "step": "advance(0.5 s)",
             "preconditioner": a.precond,
             "precision": "FP32 fields and parcels, FP64 dot products and reliable-update refinement",
-            "baseline": "reference WebGPU FP32 4 mm pour, 0.87 sim-s/wall-s (RTX A2000 Laptop)",
+            "baseline": "reference WebGPU FP32 4 mm pour, 0.87 sim-s/wall-s (RTX A2000 Laptop; "
+                        "the reference's 3D pour is FLIP only, it has no APIC figure)",
         },
         "simulated_s": sim_s,
         "substeps": substeps,

@@ -242,6 +242,10 @@ int pl3d_read_particles(pl3d_sim *sim, float *pos4, float *vel4, uint32_t *statu
 /* readRenderPositions(): xyz of the simulated half, then the mirror image z -> 2h - z
  * (gpu3d/shaders.ts:779-801); out holds 6n floats. */
 int pl3d_read_render_positions(pl3d_sim *sim, float *out);
+/* APIC affine rows of every parcel, 9n doubles: out[9p + 3c + d] = d(velocity c)/d(axis d).  The
+ * 3D reference is PIC/FLIP only; this is the 3D form of the 2D solver's per-parcel affine vectors
+ * cux/cuy/cvx/cvy (src/sim/solver.ts:537-609, :944-989).  Zero under PL3D_TRANSFER_FLIP. */
+int pl3d_read_affine(pl3d_sim *sim, double *out);
 /* Manual tilt target, degrees (PourController.setTarget, sim/motion.ts:40-44). */
 int pl3d_set_target(pl3d_sim *sim, double angle_deg);
 

@@ -64,6 +64,12 @@ void or3d_read_state(or3d_sim *s, double *x, double *y, double *z, double *vx, d
 void or3d_write_state(or3d_sim *s, const double *x, const double *y, const double *z,
                       const double *vx, const double *vy, const double *vz,
                       const uint8_t *status);
+/* APIC affine rows, 9 per parcel: out[9p + 3c + d] = d(velocity component c)/d(axis d) */
+void or3d_read_affine(or3d_sim *s, double *out);
+void or3d_write_affine(or3d_sim *s, const double *in);
+/* particlesToGrid, validity = transfer weight > 1e-9, gridToParticles: the transfer round trip
+ * of the APIC verification test (tests/solver.test.ts:323-381) */
+void or3d_transfer_roundtrip(or3d_sim *s);
 int32_t or3d_discharge_order(or3d_sim *s, int32_t *out, int32_t cap);
 double or3d_phi(or3d_sim *s);
 /* Grid fields of the last step, by name: "cell" (flags as double), "u","v","w" (final, after

@@ -57,6 +57,9 @@ def lib():
             "or3d_discharged_fractions": (i32, [vp, P(d)]),
             "or3d_read_state": (None, [vp] + [vp] * 9),
             "or3d_write_state": (None, [vp] + [vp] * 7),
+            "or3d_read_affine": (None, [vp, vp]),
+            "or3d_write_affine": (None, [vp, vp]),
+            "or3d_transfer_roundtrip": (None, [vp]),
             "or3d_discharge_order": (i32, [vp, P(i32), i32]),
             "or3d_phi": (d, [vp]),
             "or3d_grid_field": (C.c_int64, [vp, C.c_char_p, vp]),
@@ -210,6 +213,20 @@ class OracleSim:
                 for a in (x, y, z, vx, vy, vz)]
         st = None if status is None else np.ascontiguousarray(status, np.uint8)
         lib().or3d_write_state(self._h, *[_ptr(a) for a in arrs], _ptr(st))
+
+    def affine(self) -> np.ndarray:
+        """APIC affine rows, shape (n, 3, 3): [p, c, d] = d(velocity c)/d(axis d)."""
+        out = np.zeros((self.n, 3, 3), np.float64)
+        lib().or3d_read_affine(self._h, _ptr(out))
+        return out
+
+    def write_affine(self, c):
+        c = np.ascontiguousarray(np.broadcast_to(c, (self.n, 3, 3)), np.float64)
+        lib().or3d_write_affine(self._h, _ptr(c))
+
+    def transfer_roundtrip(self):
+        """particlesToGrid, validity from the transfer weights, gridToParticles."""
+        lib().or3d_transfer_roundtrip(self._h)
 
     def dischargeOrder(self) -> np.ndarray:
         cnt = lib().or3d_discharge_order(self._h, None, 0)

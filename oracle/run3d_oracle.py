@@ -1,7 +1,7 @@
 """run3d-style driver for the FP64 CPU oracle (mirrors benchmarks/run3d.ts:9-38).
 
 TEST INFRASTRUCTURE / CPU BASELINE ONLY.
-Usage: python -m oracle.run3d_oracle [cellMm] [seconds] [--max-wall S] > out.json
+Usage: python -m oracle.run3d_oracle [cellMm] [seconds] [--max-wall S] [--transfer apic] > out.json
 With --max-wall the run stops after S wall seconds (a bounded CPU-baseline sample).
 """
 import argparse
@@ -13,8 +13,10 @@ from oracle import oracle3d as O
 
 
 def run(cell_mm: float, seconds: float, max_wall: float = 0.0, chunk: float = 0.5,
-        progress=True) -> dict:
-    sim = O.OracleSim(O.gentle_pour(cell_mm, seconds))
+        progress=True, transfer: str = "flip") -> dict:
+    cfg = O.gentle_pour(cell_mm, seconds)
+    cfg.transfer = {"flip": 0, "apic": 1}[transfer]
+    sim = O.OracleSim(cfg)
     t0 = time.perf_counter()
     rows = []
     if progress:
@@ -43,6 +45,7 @@ def run(cell_mm: float, seconds: float, max_wall: float = 0.0, chunk: float = 0.
     counts = [int((st["origin"][order] == k).sum()) for k in range(3)]
     return {
         "cellMm": cell_mm,
+        "transfer": transfer,
         "result": d,
         "dischargedFractions": sim.dischargedFractions(),
         "dischargedByOrigin": counts,
@@ -57,8 +60,9 @@ def main():
     ap.add_argument("cell", nargs="?", type=float, default=4.0)
     ap.add_argument("seconds", nargs="?", type=float, default=20.0)
     ap.add_argument("--max-wall", type=float, default=0.0)
+    ap.add_argument("--transfer", choices=["flip", "apic"], default="flip")
     a = ap.parse_args()
-    print(json.dumps(run(a.cell, a.seconds, a.max_wall), indent=1))
+    print(json.dumps(run(a.cell, a.seconds, a.max_wall, transfer=a.transfer), indent=1))
 
 
 if __name__ == "__main__":

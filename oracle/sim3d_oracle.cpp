@@ -364,6 +364,9 @@ struct Sim {
   int n;
   double parcelVolume;
   std::vector<double> x, y, z, vx, vy, vz;
+  // APIC affine velocity rows per parcel, aff[9p + 3c + d] = d(vel_c)/d(x_d): 3D MAC-APIC after
+  // src/sim/solver.ts:537-609 and :944-989 (the 3D reference is PIC/FLIP only)
+  std::vector<double> aff;
   std::vector<uint8_t> origin, status;
   std::vector<float> y0;
   int bandCounts[3] = {0, 0, 0};
@@ -452,6 +455,7 @@ struct Sim {
     parcelVolume = cfg.initial_volume / (2 * n);
     x.resize(n); y.resize(n); z.resize(n);
     vx.assign(n, 0); vy.assign(n, 0); vz.assign(n, 0);
+    aff.assign(size_t(9) * n, 0);
     origin.assign(n, 0);
     status.assign(n, IN_POT);
     y0.resize(n);
@@ -629,6 +633,31 @@ struct Sim {
         if (status[p] != IN_POT) continue;
         size_t base = stencil(x[p], y[p], z[p], ox, oy, oz, sx, sy, sz, st);
         double vp = vv[p];
+        if (cfg.transfer == PL3D_TRANSFER_APIC) {
+          // corner value vel + C . (x_face - x_p): base corner first, then one h step per axis
+          // (solver.ts:553-568 in 3D)
+          const size_t sxy = size_t(sx) * sy;
+          const size_t i = base % sx, j = (base / sx) % sy, k = base / sxy;
+          const double *c = &aff[size_t(9) * p + 3 * comp];
+          const double fx0 = (double(i) + ox) * h - x[p];
+          const double fy0 = (double(j) + oy) * h - y[p];
+          const double fz0 = (double(k) + oz) * h - z[p];
+          const double a0 = vp + c[0] * fx0 + c[1] * fy0 + c[2] * fz0;
+          for_each_corner(st, base, sx, sy, [&](size_t idx, double wgt) {
+            const size_t r = idx - base;
+            const int dz = r >= sxy ? 1 : 0;
+            const size_t ry = r - dz * sxy;
+            const int dy = ry >= size_t(sx) ? 1 : 0;
+            const int dx = int(ry - dy * size_t(sx));
+            double a = a0;
+            if (dx) a += c[0] * h;
+            if (dy) a += c[1] * h;
+            if (dz) a += c[2] * h;
+            f[idx] += wgt * a;
+            wt[idx] += wgt;
+          });
+          continue;
+        }
         for_each_corner(st, base, sx, sy, [&](size_t idx, double wgt) {
           f[idx] += wgt * vp;
           wt[idx] += wgt;
@@ -807,8 +836,42 @@ struct Sim {
     ex(w, wValid, nx, ny, nz + 1);
   }
 
+  // APIC gradient of the trilinear interpolant of `field` at the parcel from the unmasked
+  // corners (solver.ts:958-967 in 3D): out[d] = d/dx_d of the sample, / h.
+  void affine_row(const std::vector<double> &field, int comp, double px, double py, double pz,
+                  double *out) {
+    double ox, oy, oz;
+    int sx, sy, sz;
+    dims(comp, ox, oy, oz, sx, sy, sz);
+    const size_t b = stencil(px, py, pz, ox, oy, oz, sx, sy, sz, st);
+    const size_t sxy = size_t(sx) * sy;
+    const double fx = st[0], fy = st[1], fz = st[2];
+    const double f000 = field[b], f100 = field[b + 1], f010 = field[b + sx],
+                 f110 = field[b + sx + 1], f001 = field[b + sxy], f101 = field[b + sxy + 1],
+                 f011 = field[b + sxy + sx], f111 = field[b + sxy + sx + 1];
+    out[0] = ((1 - fy) * (1 - fz) * (f100 - f000) + fy * (1 - fz) * (f110 - f010) +
+              (1 - fy) * fz * (f101 - f001) + fy * fz * (f111 - f011)) / h;
+    out[1] = ((1 - fx) * (1 - fz) * (f010 - f000) + fx * (1 - fz) * (f110 - f100) +
+              (1 - fx) * fz * (f011 - f001) + fx * fz * (f111 - f101)) / h;
+    out[2] = ((1 - fx) * (1 - fy) * (f001 - f000) + fx * (1 - fy) * (f101 - f100) +
+              (1 - fx) * fy * (f011 - f010) + fx * fy * (f111 - f110)) / h;
+  }
+
   void grid_to_particles() {  // :607-624
     double a = cfg.flip_ratio;
+    if (cfg.transfer == PL3D_TRANSFER_APIC) {  // solver.ts:944-989 in 3D: no FLIP blend
+      for (int p = 0; p < n; p++) {
+        if (status[p] != IN_POT) continue;
+        double px = x[p], py = y[p], pz = z[p];
+        vx[p] = sample(u, 0, px, py, pz, &uValid);
+        vy[p] = sample(v, 1, px, py, pz, &vValid);
+        vz[p] = sample(w, 2, px, py, pz, &wValid);
+        affine_row(u, 0, px, py, pz, &aff[size_t(9) * p]);
+        affine_row(v, 1, px, py, pz, &aff[size_t(9) * p + 3]);
+        affine_row(w, 2, px, py, pz, &aff[size_t(9) * p + 6]);
+      }
+      return;
+    }
     for (int p = 0; p < n; p++) {
       if (status[p] != IN_POT) continue;
       double px = x[p], py = y[p], pz = z[p];
@@ -1161,6 +1224,23 @@ void or3d_write_state(or3d_sim *s, const double *x, const double *y, const doubl
     for (int k = 0; k < 4; k++) m.counts[k] = 0;
     for (size_t p = 0; p < n; p++) m.counts[m.status[p]]++;
   }
+}
+
+void or3d_read_affine(or3d_sim *s, double *out) {
+  std::copy(s->sim.aff.begin(), s->sim.aff.end(), out);
+}
+
+void or3d_write_affine(or3d_sim *s, const double *in) {
+  std::copy(in, in + s->sim.aff.size(), s->sim.aff.begin());
+}
+
+void or3d_transfer_roundtrip(or3d_sim *s) {
+  Sim &m = s->sim;
+  m.particles_to_grid();
+  for (size_t f = 0; f < m.uw.size(); f++) m.uValid[f] = m.uw[f] > 1e-9;
+  for (size_t f = 0; f < m.vw.size(); f++) m.vValid[f] = m.vw[f] > 1e-9;
+  for (size_t f = 0; f < m.ww.size(); f++) m.wValid[f] = m.ww[f] > 1e-9;
+  m.grid_to_particles();
 }
 
 int32_t or3d_discharge_order(or3d_sim *s, int32_t *out, int32_t cap) {

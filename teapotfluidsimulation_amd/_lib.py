@@ -33,6 +33,7 @@ SIGNATURES = {
     "pl3d_read_particles": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pl3d_read_render_positions": (C.c_int, [_vp, _vp]),
     "pl3d_set_target": (C.c_int, [_vp, _d]),
+    "pl3d_read_affine": (C.c_int, [_vp, _vp]),
     "pl3d_pcg_solve": (C.c_int, [_i32, _i32, _i32, _vp, _vp, _vp, _d, _i32, _i32,
                                  _P(abi.SolveResult)]),
     "pl3d_set_timing": (C.c_int, [_vp, _i32]),

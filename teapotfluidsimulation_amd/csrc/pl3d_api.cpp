@@ -363,7 +363,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     return fail(PL3D_EINVAL, "invalid Config3D");
   if (c.preconditioner != PL3D_PRECOND_MIC0 && c.preconditioner != PL3D_PRECOND_JACOBI)
     return fail(PL3D_EINVAL, "preconditioner %d is not available", c.preconditioner);
-  if (c.transfer != PL3D_TRANSFER_FLIP)
+  if (c.transfer != PL3D_TRANSFER_FLIP && c.transfer != PL3D_TRANSFER_APIC)
     return fail(PL3D_EINVAL, "transfer %d is not available", c.transfer);
   auto *s = new pl3d_sim();
   s->cfg = c;
@@ -487,6 +487,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   P.pressureTolerance = c.pressure_tolerance;
   P.pressureMaxIterations = c.pressure_max_iterations;
   P.preconditioner = c.preconditioner;
+  P.transfer = c.transfer;
   {  // Math.hypot(nx, ny) * h
     const double a = nx, b = ny, mx = std::max(a, b);
     const double na = a / mx, nb = b / mx;
@@ -531,6 +532,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       {(void **)&B.ctl, sizeof(DevCtl)},
       {(void **)&B.x, NP * 4}, {(void **)&B.y, NP * 4}, {(void **)&B.z, NP * 4},
       {(void **)&B.vx, NP * 4}, {(void **)&B.vy, NP * 4}, {(void **)&B.vz, NP * 4},
+      {(void **)&B.aff, 9 * NP * 4}, {(void **)&B.binAff, 3 * NP * 16},
       {(void **)&B.status, NP}, {(void **)&B.origin, NP},
       {(void **)&B.cellOf, NP * 4}, {(void **)&B.slot, NP * 4}, {(void **)&B.dstep, NP * 4},
       {(void **)&B.cellCount, NC * 4}, {(void **)&B.binStart, (NC + 1) * 4},
@@ -616,6 +618,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       (e = up(B.status, st0.data(), NP)) || (e = up(B.origin, origin.data(), NP)) ||
       (e = up(B.dstep, minus1.data(), NP * 4)) || (e = up(B.cellOf, minus1.data(), NP * 4)) ||
       (e = hipMemsetAsync(B.cellCount, 0, NC * 4, s->stream)) ||
+      (e = hipMemsetAsync(B.aff, 0, 9 * NP * 4, s->stream)) ||
       (e = hipMemsetAsync(B.u, 0, NF * 4, s->stream)) ||
       (e = hipMemsetAsync(B.uE, 0, NF * 4, s->stream)) ||
       (e = hipMemsetAsync(B.uc, 0, NF * 4, s->stream)) ||
@@ -867,6 +870,18 @@ int pl3d_read_render_positions(pl3d_sim *s, float *out) {
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(out, s->render, n * 24, hipMemcpyDeviceToHost, s->stream));
   HIP_OK(hipStreamSynchronize(s->stream));
+  return PL3D_OK;
+}
+
+int pl3d_read_affine(pl3d_sim *s, double *out) {
+  if (int rc = check_sim(s)) return rc;
+  if (!out) return fail(PL3D_EINVAL, "out is NULL");
+  const size_t n = size_t(s->P.n);
+  std::vector<float> tmp(9 * n);
+  HIP_OK(hipMemcpyAsync(tmp.data(), s->B.aff, 9 * n * 4, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  for (size_t p = 0; p < n; p++)
+    for (int r = 0; r < 9; r++) out[9 * p + r] = tmp[size_t(r) * n + p];
   return PL3D_OK;
 }
 

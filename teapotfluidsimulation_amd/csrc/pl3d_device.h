@@ -17,6 +17,7 @@
 #define PL3D_AIR_ 0
 #define PL3D_FLUID_ 1
 #define PL3D_SOLID_ 2
+#define PL3D_TRANSFER_APIC_ 1  // PL3D_TRANSFER_APIC of the C ABI
 #define ST_IN_POT 0
 #define ST_DISCHARGED 1
 #define ST_SPILLED 2
@@ -111,6 +112,7 @@ struct DevParams {
   double pressureTolerance;
   int pressureMaxIterations;
   int preconditioner;
+  int transfer;  // PL3D_TRANSFER_FLIP / PL3D_TRANSFER_APIC
   double Rdiag;  // hypot(nx, ny) * h (solver3d.ts:275)
   // motion config (types.ts:55-84)
   int motionKind, repeats;
@@ -186,6 +188,25 @@ __device__ __forceinline__ float sample_field(const float *__restrict__ f,
   }
   if (valid == nullptr) return all;
   return ws > 1e-9f ? acc / ws : all;
+}
+
+// APIC affine row of one staggered component (3D MAC-APIC after src/sim/solver.ts:958-967): the
+// gradient of the trilinear interpolant at the parcel from the unmasked corners, divided by h.
+__device__ __forceinline__ void affine_row(const float *__restrict__ f, float gx, float gy, float gz,
+                                           int sx, int sy, int sz, float invh, float out[3]) {
+  const Stencil1 a = stencil1(gx, sx), b = stencil1(gy, sy), c = stencil1(gz, sz);
+  const int sxy = sx * sy;
+  const int base = a.i + sx * (b.i + sy * c.i);
+  const float fx = a.f, fy = b.f, fz = c.f;
+  const float f000 = f[base], f100 = f[base + 1], f010 = f[base + sx], f110 = f[base + sx + 1],
+              f001 = f[base + sxy], f101 = f[base + sxy + 1], f011 = f[base + sxy + sx],
+              f111 = f[base + sxy + sx + 1];
+  out[0] = ((1 - fy) * (1 - fz) * (f100 - f000) + fy * (1 - fz) * (f110 - f010) +
+            (1 - fy) * fz * (f101 - f001) + fy * fz * (f111 - f011)) * invh;
+  out[1] = ((1 - fx) * (1 - fz) * (f010 - f000) + fx * (1 - fz) * (f110 - f100) +
+            (1 - fx) * fz * (f011 - f001) + fx * fz * (f111 - f101)) * invh;
+  out[2] = ((1 - fx) * (1 - fy) * (f001 - f000) + fx * (1 - fy) * (f101 - f100) +
+            (1 - fx) * fy * (f011 - f010) + fx * fy * (f111 - f110)) * invh;
 }
 
 __device__ __forceinline__ unsigned int float_as_ordered(float v) {  // v >= 0

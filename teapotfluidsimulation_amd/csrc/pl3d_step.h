@@ -17,12 +17,14 @@ struct DevBufs {
   DevCtl *ctl;
   // parcels
   float *x, *y, *z, *vx, *vy, *vz;
+  float *aff;  // APIC affine rows, 9 arrays of n: aff[(3c + d) n + p] = d(vel_c)/d(x_d)
   uint8_t *status, *origin;
   int *cellOf, *slot, *dstep;
   // bins
   int *cellCount;  // zero between substeps
   int *binStart;   // C + 1
   float4 *binPos, *binVel;
+  float4 *binAff;  // APIC: 3 per bin slot, row c of the parcel's affine matrix
   unsigned long long *scanStatus;  // one word per scan tile
   // cells
   uint8_t *flag;

@@ -310,6 +310,13 @@ __global__ void k_scatter(DevBufs B, DevParams P) {
   const int dst = B.binStart[c] + B.slot[p];
   B.binPos[dst] = make_float4(B.x[p], B.y[p], B.z[p], 0.f);
   B.binVel[dst] = make_float4(B.vx[p], B.vy[p], B.vz[p], 0.f);
+  if (P.transfer == PL3D_TRANSFER_APIC_) {
+    const size_t n = size_t(P.n);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      B.binAff[3 * size_t(dst) + c] = make_float4(B.aff[(3 * c) * n + p], B.aff[(3 * c + 1) * n + p],
+                                                  B.aff[(3 * c + 2) * n + p], 0.f);
+  }
 }
 
 // ============================================================================= k_p2g
@@ -324,10 +331,20 @@ __global__ void k_scatter(DevBufs B, DevParams P) {
 #define P2G_TILE (P2G_TX * P2G_TY * P2G_TZ)
 #define P2G_MAXROWS ((P2G_TY + 2) * (P2G_TZ + 2))
 
+// One parcel's eight corner contributions to one component's owned faces.  APIC: the corner
+// value is vel + C . (x_face - x_p), formed as the base corner's value plus one C h step per
+// axis (src/sim/solver.ts:553-568 in 3D); (ox, oy, oz) is the component's grid offset.
+template <bool APIC>
 __device__ __forceinline__ void p2g_corners(long long (*acc)[2], const int lo[3], int sx, int sy,
                                             int sz, float gx, float gy, float gz, float vel,
-                                            bool withVel) {
+                                            bool withVel, float4 cr = float4{}, float px = 0.f,
+                                            float py = 0.f, float pz = 0.f, float ox = 0.f,
+                                            float oy = 0.f, float oz = 0.f, float h = 0.f) {
   const Stencil1 a = stencil1(gx, sx), b = stencil1(gy, sy), c = stencil1(gz, sz);
+  float a0 = vel;
+  if (APIC)
+    a0 = vel + cr.x * ((float(a.i) + ox) * h - px) + cr.y * ((float(b.i) + oy) * h - py) +
+         cr.z * ((float(c.i) + oz) * h - pz);
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
@@ -336,10 +353,58 @@ __device__ __forceinline__ void p2g_corners(long long (*acc)[2], const int lo[3]
     // corner weight in the reference's order: x factor * y factor * z factor (:369-376)
     const float w = (dx ? a.f : 1.f - a.f) * (dy ? b.f : 1.f - b.f) * (dz ? c.f : 1.f - c.f);
     const int e = li + P2G_TX * (lj + P2G_TY * lk);
+    float v = vel;
+    if (APIC) {
+      v = a0;
+      if (dx) v += cr.x * h;
+      if (dy) v += cr.y * h;
+      if (dz) v += cr.z * h;
+    }
     if (withVel) atomicAdd(reinterpret_cast<unsigned long long *>(&acc[e][0]),
-                           static_cast<unsigned long long>(fx40(w * vel)));
+                           static_cast<unsigned long long>(fx40(w * v)));
     atomicAdd(reinterpret_cast<unsigned long long *>(&acc[e][1]),
               static_cast<unsigned long long>(fx40(w)));
+  }
+}
+
+// The parcel loop of k_p2g for one transfer mode.
+template <bool APIC>
+__device__ __forceinline__ void p2g_parcels(const DevBufs &B, const DevParams &P,
+                                            long long (*acc)[P2G_TILE][2], const int lo[3],
+                                            const int *rowOff, const int *rowStart, int nrows,
+                                            int total) {
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const float invh = P.invh, h = P.h;
+  // Parcels of one cell are adjacent in the bins and hit the same corners; within each full run
+  // of 512 slots, slot g takes parcel (g % 64) * 8 + (g / 64) % 8, so the 64 lanes of a wave are
+  // 8 parcels apart and their LDS atomics rarely meet on one address.
+  const int full = total & ~511;
+  for (int g = threadIdx.x; g < total; g += blockDim.x) {
+    const int p = g < full ? (g & ~511) | ((g & 63) << 3) | ((g >> 6) & 7) : g;
+    int lo_r = 0, hi_r = nrows - 1;  // the row holding parcel p: rowOff[r] <= p < rowOff[r + 1]
+    while (lo_r < hi_r) {
+      const int mid = (lo_r + hi_r + 1) >> 1;
+      if (rowOff[mid] <= p) lo_r = mid; else hi_r = mid - 1;
+    }
+    const int q = rowStart[lo_r] + (p - rowOff[lo_r]);
+    const float4 pp = B.binPos[q];
+    const float4 vv = B.binVel[q];
+    const float gx = pp.x * invh, gy = pp.y * invh, gz = pp.z * invh;
+    if (APIC) {
+      const float4 cu = B.binAff[3 * size_t(q)], cv = B.binAff[3 * size_t(q) + 1],
+                   cw = B.binAff[3 * size_t(q) + 2];
+      p2g_corners<true>(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true, cu,
+                        pp.x, pp.y, pp.z, 0.f, 0.5f, 0.5f, h);
+      p2g_corners<true>(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true, cv,
+                        pp.x, pp.y, pp.z, 0.5f, 0.f, 0.5f, h);
+      p2g_corners<true>(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true, cw,
+                        pp.x, pp.y, pp.z, 0.5f, 0.5f, 0.f, h);
+    } else {
+      p2g_corners<false>(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true);
+      p2g_corners<false>(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true);
+      p2g_corners<false>(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true);
+    }
+    p2g_corners<false>(acc[3], lo, nx, ny, nz, gx - 0.5f, gy - 0.5f, gz - 0.5f, 0.f, false);
   }
 }
 
@@ -372,27 +437,10 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(DevBufs B, DevParams P) {
   }
   __syncthreads();
   const int total = rowOff[nrows];
-  const float invh = P.invh;
-  // Parcels of one cell are adjacent in the bins and hit the same corners; within each full run
-  // of 512 slots, slot g takes parcel (g % 64) * 8 + (g / 64) % 8, so the 64 lanes of a wave are
-  // 8 parcels apart and their LDS atomics rarely meet on one address.
-  const int full = total & ~511;
-  for (int g = threadIdx.x; g < total; g += blockDim.x) {
-    const int p = g < full ? (g & ~511) | ((g & 63) << 3) | ((g >> 6) & 7) : g;
-    int lo_r = 0, hi_r = nrows - 1;  // the row holding parcel p: rowOff[r] <= p < rowOff[r + 1]
-    while (lo_r < hi_r) {
-      const int mid = (lo_r + hi_r + 1) >> 1;
-      if (rowOff[mid] <= p) lo_r = mid; else hi_r = mid - 1;
-    }
-    const int q = rowStart[lo_r] + (p - rowOff[lo_r]);
-    const float4 pp = B.binPos[q];
-    const float4 vv = B.binVel[q];
-    const float gx = pp.x * invh, gy = pp.y * invh, gz = pp.z * invh;
-    p2g_corners(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true);
-    p2g_corners(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true);
-    p2g_corners(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true);
-    p2g_corners(acc[3], lo, nx, ny, nz, gx - 0.5f, gy - 0.5f, gz - 0.5f, 0.f, false);
-  }
+  if (P.transfer == PL3D_TRANSFER_APIC_)
+    p2g_parcels<true>(B, P, acc, lo, rowOff, rowStart, nrows, total);
+  else
+    p2g_parcels<false>(B, P, acc, lo, rowOff, rowStart, nrows, total);
   __syncthreads();
   // epilogue over the owned faces of the three components and the owned cells
   for (int e = threadIdx.x; e < 4 * P2G_TILE; e += blockDim.x) {
@@ -662,17 +710,39 @@ __global__ __launch_bounds__(STEP_THREADS) void k_advect(DevBufs B, DevParams P)
       const uint8_t *uV = B.valid, *vV = B.valid + P.nu, *wV = B.valid + P.nu + P.nv;
       const float x0 = B.x[p], y0 = B.y[p], z0 = B.z[p];
       const float gx = x0 * invh, gy = y0 * invh, gz = z0 * invh;
-      // G2P, FLIP blend with the pre-extrapolation mask (:614-622)
-      const float a = float(P.flipRatio);
+      // G2P with the pre-extrapolation mask (:614-622)
       const float un = sample_field(uE, uV, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz);
       const float vn = sample_field(vE, vV, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz);
       const float wn = sample_field(wE, wV, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1);
-      const float uo = sample_field(uP, uV, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz);
-      const float vo = sample_field(vP, vV, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz);
-      const float wo = sample_field(wP, wV, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1);
-      float pvx = (1 - a) * un + a * (B.vx[p] + un - uo);
-      float pvy = (1 - a) * vn + a * (B.vy[p] + vn - vo);
-      float pvz = (1 - a) * wn + a * (B.vz[p] + wn - wo);
+      float pvx, pvy, pvz;
+      if (P.transfer == PL3D_TRANSFER_APIC_) {
+        // APIC (src/sim/solver.ts:944-989 in 3D): the new grid velocity and its affine rows
+        pvx = un;
+        pvy = vn;
+        pvz = wn;
+        const size_t n = size_t(P.n);
+        float r[3];
+        affine_row(uE, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz, invh, r);
+        B.aff[0 * n + p] = r[0];
+        B.aff[1 * n + p] = r[1];
+        B.aff[2 * n + p] = r[2];
+        affine_row(vE, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz, invh, r);
+        B.aff[3 * n + p] = r[0];
+        B.aff[4 * n + p] = r[1];
+        B.aff[5 * n + p] = r[2];
+        affine_row(wE, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1, invh, r);
+        B.aff[6 * n + p] = r[0];
+        B.aff[7 * n + p] = r[1];
+        B.aff[8 * n + p] = r[2];
+      } else {  // FLIP/PIC blend
+        const float a = float(P.flipRatio);
+        const float uo = sample_field(uP, uV, gx, gy - 0.5f, gz - 0.5f, nx + 1, ny, nz);
+        const float vo = sample_field(vP, vV, gx - 0.5f, gy, gz - 0.5f, nx, ny + 1, nz);
+        const float wo = sample_field(wP, wV, gx - 0.5f, gy - 0.5f, gz, nx, ny, nz + 1);
+        pvx = (1 - a) * un + a * (B.vx[p] + un - uo);
+        pvy = (1 - a) * vn + a * (B.vy[p] + vn - vo);
+        pvz = (1 - a) * wn + a * (B.vz[p] + wn - wo);
+      }
       // advect
       const float c0 = C.c0, s0 = C.s0, c1 = C.c1, s1 = C.s1;
       const float pwx = float(P.pivotWX), pwy = float(P.pivotWY);

@@ -607,6 +607,20 @@ static napi_value js_read_render_positions(napi_env env, napi_callback_info info
   return a;
 }
 
+/* readAffine(h, n) -> Float64Array(9n), APIC affine rows [9p + 3c + d] */
+static napi_value js_read_affine(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  pl3d_sim *sim;
+  double dn;
+  void *p;
+  if (!args(env, info, 2, argv) || !get_sim(env, argv[0], &sim) || !get_double(env, argv[1], &dn, "n"))
+    return NULL;
+  napi_value a = new_typed(env, napi_float64_array, 8, 9 * (size_t)dn, &p);
+  if (!a) return NULL;
+  CHECK_PL3D(pl3d_read_affine(sim, (double *)p));
+  return a;
+}
+
 static napi_value js_set_target(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   pl3d_sim *sim;
@@ -685,6 +699,7 @@ static napi_value init(napi_env env, napi_value exports) {
       {"dischargeOrder", js_discharge_order},
       {"readParticles", js_read_particles},
       {"readRenderPositions", js_read_render_positions},
+      {"readAffine", js_read_affine},
       {"setTarget", js_set_target},
       {"pcgSolve", js_pcg_solve},
   };

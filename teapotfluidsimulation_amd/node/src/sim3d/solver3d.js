@@ -156,6 +156,11 @@ class Simulation3D {
     };
   }
 
+  // APIC affine rows, Float64Array(9n): [9p + 3c + d] = d(velocity c)/d(axis d)
+  affine() {
+    return this._lib.readAffine(this._h, this.n);
+  }
+
   readParticles() {
     return this._lib.readParticles(this._h, this.n);
   }

@@ -189,6 +189,12 @@ class Simulation3D:
         check(load().pl3d_read_render_positions(self._handle(), _ptr(out)))
         return out
 
+    def affine(self) -> np.ndarray:
+        """APIC affine rows, shape (n, 3, 3): [p, c, d] = d(velocity c)/d(axis d)."""
+        out = np.zeros((self.n, 3, 3), np.float64)
+        check(load().pl3d_read_affine(self._handle(), _ptr(out)))
+        return out
+
     def setTarget(self, angleDeg: float):
         """Manual tilt target (PourController.setTarget, motion.ts:40-44)."""
         check(load().pl3d_set_target(self._handle(), float(angleDeg)))

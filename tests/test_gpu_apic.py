@@ -1,0 +1,108 @@
+"""APIC transfer on the device against the FP64 oracle's APIC (3D MAC-APIC after
+src/sim/solver.ts:537-609 and :944-989; the 3D reference is PIC/FLIP only, so the oracle's own
+APIC, pinned by test_oracle.py::test_apic_affine_preservation, is the judge).
+
+Bars, each stated at its assertion:
+- lockstep 20 substeps at 10 mm: the FLIP lockstep thresholds (e2e/gpuSolver.spec.ts:35-41), plus
+  affine rows within 1e-3 of the oracle's, relative to their RMS;
+- lockstep at 5 mm: FP32-rounding agreement through 60 substeps, bounded chaotic growth to 200;
+- the full 5 mm and 4 mm APIC pours: poured volume within 0.3 % and layer shares within 1.3
+  points of the oracle's APIC pours (tests/golden/apic_pours.json, written by
+  tests/golden/make_apic_golden.py).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle3d as O
+from teapotfluidsimulation_amd import Simulation3D
+from teapotfluidsimulation_amd.abi import config_to_dict
+
+from test_gpu_sim import compare
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "apic_pours.json")
+
+
+def apic(cell):
+    cfg = O.gentle_pour(cell, 20)
+    cfg.transfer = 1
+    return cfg
+
+
+def lockstep(cfg, steps):
+    cpu, gpu = O.OracleSim(cfg), Simulation3D(config_to_dict(cfg))
+    for _ in range(steps):
+        dt = min(cpu.stableDt(), cfg.duration - cpu.diagnostics()["time"])
+        cpu.step(dt)
+        gpu.step(dt)
+    return cpu, gpu
+
+
+def test_apic_lockstep_20_substeps_10mm():
+    cfg = apic(10)
+    cpu, gpu = lockstep(cfg, 20)
+    r = compare(cpu, gpu, cfg.h)
+    assert r["rms"] < 1e-3 and r["max"] < 1e-2 and r["relv"] < 1e-3, r
+    assert r["mism"] == 0
+    inpot = cpu.state()["status"] == 0
+    ca, cb = cpu.affine()[inpot], gpu.affine()[inpot]
+    scale = np.sqrt(np.mean(ca ** 2))
+    assert scale > 0
+    assert np.sqrt(np.mean((ca - cb) ** 2)) / scale < 1e-3
+    d = gpu.diagnostics()
+    assert d["pressureFailures"] == 0 and d["lostMl"] == 0
+    gpu.destroy()
+
+
+def test_apic_lockstep_200_substeps_5mm():
+    """Two bars.  Through 60 substeps the device and the oracle differ only by FP32 rounding:
+    RMS < 1e-4 cells and relative velocity RMS < 1e-3 (measured 3.0e-5 and 1.1e-4; a transfer
+    mistake shows at 1e-2 or more within a few substeps).  Near substep 90 one discrete event
+    (a parcel or cell flag resolving differently) makes both transfers jump; APIC, with no PIC
+    damping, amplifies it about ten times more than FLIP 0.95 does (FLIP: RMS 8e-4 at 200
+    substeps; APIC: 1.3e-2).  At 200 substeps: RMS < 0.05 cells, no status mismatch; the full-pour
+    test below bounds the outcome."""
+    cfg = apic(5)
+    cpu, gpu = lockstep(cfg, 60)
+    r = compare(cpu, gpu, cfg.h)
+    assert r["rms"] < 1e-4 and r["relv"] < 1e-3 and r["mism"] == 0, r
+    for _ in range(140):
+        dt = min(cpu.stableDt(), cfg.duration - cpu.diagnostics()["time"])
+        cpu.step(dt)
+        gpu.step(dt)
+    r = compare(cpu, gpu, cfg.h)
+    assert r["rms"] < 0.05 and r["mism"] == 0, r
+    gpu.destroy()
+
+
+def pour(cfg):
+    sim = Simulation3D(config_to_dict(cfg))
+    done_at = -1.0
+    while not sim.finished:
+        d = sim.advance(0.5)
+        if d["phase"] == "done" and done_at < 0:
+            done_at = d["time"]
+        if done_at >= 0 and d["time"] > done_at + 1.5:
+            break
+    out = sim.diagnostics(), sim.dischargedFractions()
+    sim.destroy()
+    return out
+
+
+@pytest.mark.parametrize("cell", [5, 4])
+def test_apic_full_pour_matches_oracle(cell):
+    rows = {r["cellMm"]: r for r in json.load(open(GOLDEN))["rows"]}
+    if float(cell) not in rows:
+        pytest.skip(f"no oracle APIC pour at {cell} mm in the fixture")
+    ref = rows[float(cell)]
+    print(f"APIC pour {cell} mm", flush=True)
+    d, frac = pour(apic(cell))
+    vol = (d["dischargedMl"] - ref["dischargedMl"]) / ref["dischargedMl"]
+    pts = [100 * (a - b) for a, b in zip(frac, ref["dischargedFractions"])]
+    print(f"APIC {cell} mm: {d['dischargedMl']:.2f} mL vs {ref['dischargedMl']:.2f} "
+          f"({100 * vol:+.3f} %), shares {pts}", flush=True)
+    assert abs(vol) <= 0.003 and max(abs(p) for p in pts) <= 1.3
+    assert d["lostMl"] == 0 and d["pressureFailures"] == 0

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
 
 EXPORTS = ["version", "gentlePour", "create", "destroy", "info", "step", "advance", "diagnostics",
            "stableDt", "finished", "dischargedFractions", "stats", "readState", "dischargeOrder",
-           "readParticles", "readRenderPositions", "setTarget", "pcgSolve"]
+           "readParticles", "readRenderPositions", "readAffine", "setTarget", "pcgSolve"]
 
 
 def node_json(script: str):

@@ -265,3 +265,50 @@ def test_rest_and_tilt_behaviour():
     assert sa["y"][order].min() < a.pivotWorld[1]
     assert np.array_equal(sa["x"], b.state()["x"])
     assert np.any(sa["status"] == 0)
+
+
+@pytest.mark.parametrize("fields", [
+    # (velocity gradient rows, constant part): constant, general affine, rigid rotation about z
+    ([[0, 0, 0], [0, 0, 0], [0, 0, 0]], [0.2, -0.1, 0.05]),
+    ([[0.3, -0.4, 0.2], [0.2, 0.5, -0.1], [0.1, 0.3, -0.6]], [0.1, -0.2, 0.05]),
+    ([[0, -2, 0], [2, 0, 0], [0, 0, 0]], [0, 0, 0]),
+])
+def test_apic_affine_preservation(fields):
+    """The 3D form of the reference's APIC verification (tests/solver.test.ts:323-381): with
+    v = G x + c and C = G on every parcel, particlesToGrid, validity from the transfer weights and
+    gridToParticles return v and C unchanged (reference tolerances 1e-12 and 1e-11).  The
+    reference checks interior parcels only; every corner of every parcel's stencil receives that
+    parcel's own weight, so the check holds for all parcels here.  A wrong face offset, sign or
+    corner step in either transfer breaks it for the non-constant fields."""
+    G = np.array(fields[0], float)
+    c0 = np.array(fields[1], float)
+    cfg = O.gentle_pour(10, 20)
+    cfg.transfer = 1
+    sim = O.OracleSim(cfg)
+    st = sim.state()
+    pos = np.stack([st["x"], st["y"], st["z"]], 1)
+    vel = pos @ G.T + c0
+    sim.write_state(vx=vel[:, 0], vy=vel[:, 1], vz=vel[:, 2])
+    sim.write_affine(G)
+    sim.transfer_roundtrip()
+    out = sim.state()
+    got = np.stack([out["vx"], out["vy"], out["vz"]], 1)
+    assert sim.n > 1000
+    assert np.abs(got - vel).max() < 1e-12
+    assert np.abs(sim.affine() - G).max() < 1e-11
+
+
+def test_apic_is_not_flip():
+    """The transfer switch changes the step: after 30 substeps the APIC and FLIP oracles differ,
+    and APIC carries non-zero affine rows while FLIP keeps them zero."""
+    runs = {}
+    for t in (0, 1):
+        cfg = O.gentle_pour(10, 20)
+        cfg.transfer = t
+        sim = O.OracleSim(cfg)
+        for _ in range(30):
+            sim.step(sim.stableDt())
+        runs[t] = (sim.state(), sim.affine())
+    assert np.abs(runs[0][1]).max() == 0
+    assert np.abs(runs[1][1]).max() > 0
+    assert np.abs(runs[0][0]["vy"] - runs[1][0]["vy"]).max() > 1e-6
