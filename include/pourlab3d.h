@@ -249,6 +249,33 @@ int pl3d_read_affine(pl3d_sim *sim, double *out);
 /* Manual tilt target, degrees (PourController.setTarget, sim/motion.ts:40-44). */
 int pl3d_set_target(pl3d_sim *sim, double angle_deg);
 
+/* ---- ensembles ------------------------------------------------------------------------------ */
+
+/* An ensemble of independent pours, possibly on different grids, stepped together: every kernel of
+ * a substep is one launch for all members (one pressure and one correction workgroup per member in
+ * the solve launch), captured once as a HIP graph.  No counterpart in the reference, whose driver
+ * runs one Simulation3D at a time (benchmarks/run3d.ts); each member behaves exactly like a
+ * pl3d_sim created from the same config and advanced by the same durations.  All members must use
+ * the same preconditioner. */
+typedef struct pl3d_ensemble pl3d_ensemble;
+/* Member `index` of a family of varied pours drawn from one mulberry32 stream (seed): pot radius
+ * 5-8 cm, fill 0.8-1.5 L in three equal-volume layers, spout bore diameter 1.5-2.5 cm with its
+ * inlet 1-10 cm above the floor, hold tilt rate 5-20 deg/s, cells 3-6 mm (coarsened where the half
+ * pot would hold more than ~11,000 liquid cells), the rest as gentlePour3D(., 20).  Build-only:
+ * the reference has one preset (presets3d.ts:9-26). */
+int pl3d_varied_pour(uint32_t seed, int32_t index, pl3d_config *out);
+int pl3d_ensemble_create(const pl3d_config *cfgs, int32_t count, pl3d_ensemble **out);
+int pl3d_ensemble_destroy(pl3d_ensemble *ens);
+int pl3d_ensemble_size(pl3d_ensemble *ens, int32_t *count);
+/* Borrowed handle of member `index` for the query entry points (diagnostics, fractions, state,
+ * stats, info).  Stepping, timing, setTarget and destroy on a member return PL3D_ESTATE. */
+int pl3d_ensemble_member(pl3d_ensemble *ens, int32_t index, pl3d_sim **out);
+/* advance(seconds) for every member (Simulation3D.advance, solver3d.ts:281-286); out, if not
+ * NULL, receives `count` Diagnostics3D records. */
+int pl3d_ensemble_advance(pl3d_ensemble *ens, double seconds, pl3d_diagnostics *out);
+/* finished per member: out receives `count` flags. */
+int pl3d_ensemble_finished(pl3d_ensemble *ens, int32_t *out);
+
 /* ---- standalone pressure solve (Pcg3D) ------------------------------------------------------ */
 
 /* Pcg3D(nx, ny, nz).solve(cell, b, q, tol, maxIter) (solver3d.ts:858-937): solves the 7-point

@@ -6,6 +6,7 @@ include/pourlab3d.h); this package is the host-side mirror of the reference's so
 from .sim3d import (  # noqa: F401
     AIR,
     DISCHARGED,
+    Ensemble3D,
     FLUID,
     IN_POT,
     LOST,
@@ -14,4 +15,5 @@ from .sim3d import (  # noqa: F401
     Pcg3D,
     Simulation3D,
     gentlePour3D,
+    variedPour3D,
 )

@@ -41,6 +41,13 @@ SIGNATURES = {
                                     _P(_i32)]),
     "pl3d_copy_probe": (C.c_int, [C.c_int64, _i32, _P(_d)]),
     "pl3d_counters": (C.c_int, [_vp, _P(C.c_int64)]),
+    "pl3d_varied_pour": (C.c_int, [C.c_uint32, _i32, _P(abi.Config)]),
+    "pl3d_ensemble_create": (C.c_int, [_P(abi.Config), _i32, _P(_vp)]),
+    "pl3d_ensemble_destroy": (C.c_int, [_vp]),
+    "pl3d_ensemble_size": (C.c_int, [_vp, _P(_i32)]),
+    "pl3d_ensemble_member": (C.c_int, [_vp, _i32, _P(_vp)]),
+    "pl3d_ensemble_advance": (C.c_int, [_vp, _d, _P(abi.Diagnostics)]),
+    "pl3d_ensemble_finished": (C.c_int, [_vp, _P(_i32)]),
 }
 
 _lib = None

@@ -147,6 +147,29 @@ struct pl3d_sim {
   bool destroyed = false;
   double lastDt = 0.005;
   int gridN = 1, gridC = 1, gridF = 1, gridFC = 1, gridExtrap = 1, gridScan = 1;
+  // device copies of B, P and the solver's shared arguments: the one-entry tables the kernels
+  // index by blockIdx.y
+  DevBufs *dB = nullptr;
+  DevParams *dP = nullptr;
+  PcgShared *dA = nullptr;
+  PcgShared A{};
+  pl3d_ensemble *ensemble = nullptr;  // set while a member of an ensemble (stepped by it)
+};
+
+// An ensemble: independent pours whose substeps run as one launch per kernel (blockIdx.y selects
+// the member).  The members are ordinary simulations; the ensemble holds the launch tables.
+struct pl3d_ensemble {
+  std::vector<pl3d_sim *> members;
+  hipStream_t stream = nullptr;
+  void *mem = nullptr;  // tables: B[K], P[K], A[K], descs[2K]
+  DevBufs *tB = nullptr;
+  DevParams *tP = nullptr;
+  PcgShared *tA = nullptr;
+  PcgDesc *tD = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int precond = 0;
+  int gridN = 1, gridC = 1, gridF = 1, gridFC = 1, gridExtrap = 1, gridScan = 1;
+  bool destroyed = false;
 };
 
 namespace {
@@ -198,7 +221,8 @@ int timed_end(pl3d_sim *s, int idx, hipEvent_t a) {
 }
 
 // the command fields (mode, pending, endTime, dtExplicit) are contiguous in DevCtl
-int push_command(pl3d_sim *s, int mode, int pending, double endTime, double dt) {
+int push_command_on(pl3d_sim *s, hipStream_t stream, int mode, int pending, double endTime,
+                    double dt) {
   s->hctl->mode = mode;
   s->hctl->pending = pending;
   s->hctl->endTime = endTime;
@@ -207,79 +231,96 @@ int push_command(pl3d_sim *s, int mode, int pending, double endTime, double dt) 
   const size_t len = offsetof(DevCtl, active) - off;
   HIP_OK(hipMemcpyAsync(reinterpret_cast<char *>(s->B.ctl) + off,
                         reinterpret_cast<char *>(s->hctl) + off, len, hipMemcpyHostToDevice,
-                        s->stream));
+                        stream));
   return PL3D_OK;
+}
+
+int push_command(pl3d_sim *s, int mode, int pending, double endTime, double dt) {
+  return push_command_on(s, s->stream, mode, pending, endTime, dt);
+}
+
+// One set of launches: a single simulation (tables of one, optional per-kernel timing) or an
+// ensemble (tables of K members, grids sized for the largest member).
+struct LaunchSet {
+  hipStream_t stream;
+  const DevBufs *tB;
+  const DevParams *tP;
+  const PcgShared *tA;
+  const PcgDesc *tD;
+  int members;
+  int gridN, gridC, gridF, gridFC, gridExtrap, gridScan;
+  int precond;
+  pl3d_sim *timed;  // timing mode of a single simulation, else nullptr
+};
+
+LaunchSet single_set(pl3d_sim *s) {
+  return LaunchSet{s->stream, s->dB, s->dP, s->dA, s->B.descs, 1, s->gridN, s->gridC, s->gridF,
+                   s->gridFC, s->gridExtrap, s->gridScan, s->A.precond,
+                   s->timing ? s : nullptr};
 }
 
 template <class K>
-int launch(pl3d_sim *s, int idx, K kernel, int grid, int block) {
+int launch(const LaunchSet &L, int idx, K kernel, int grid, int block) {
   hipEvent_t a = nullptr;
-  if (int rc = timed_begin(s, &a)) return rc;
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s->stream, s->B, s->P);
+  if (L.timed)
+    if (int rc = timed_begin(L.timed, &a)) return rc;
+  hipLaunchKernelGGL(kernel, dim3(grid, L.members), dim3(block), 0, L.stream, L.tB, L.tP);
   HIP_OK(hipGetLastError());
-  return timed_end(s, idx, a);
+  return L.timed ? timed_end(L.timed, idx, a) : PL3D_OK;
 }
 
 // One substep's launches, in order (Simulation3D.step, solver3d.ts:288-309).
-int enqueue_substep(pl3d_sim *s) {
+int enqueue_substep(const LaunchSet &L) {
   int rc;
-  if ((rc = launch(s, 0, k_begin, 1, 256))) return rc;
-  if ((rc = launch(s, 1, k_count, s->gridN, STEP_THREADS))) return rc;
-  if ((rc = launch(s, 2, k_classify_scan, s->gridScan, STEP_THREADS))) return rc;
-  if ((rc = launch(s, 3, k_scatter, s->gridN, STEP_THREADS))) return rc;
-  if ((rc = launch(s, 4, k_p2g, s->gridFC, P2G_THREADS))) return rc;
-  if ((rc = launch(s, 5, k_liquid, s->gridC, STEP_THREADS))) return rc;
+  if ((rc = launch(L, 0, k_begin, 1, 256))) return rc;
+  if ((rc = launch(L, 1, k_count, L.gridN, STEP_THREADS))) return rc;
+  if ((rc = launch(L, 2, k_classify_scan, L.gridScan, STEP_THREADS))) return rc;
+  if ((rc = launch(L, 3, k_scatter, L.gridN, STEP_THREADS))) return rc;
+  if ((rc = launch(L, 4, k_p2g, L.gridFC, P2G_THREADS))) return rc;
+  if ((rc = launch(L, 5, k_liquid, L.gridC, STEP_THREADS))) return rc;
   {
-    PcgShared A;
-    A.m = &s->B.ctl->m;
-    A.nbr = s->B.nbr;
-    A.diag = s->B.diag;
-    A.mic = s->B.mic;
-    A.plane = s->B.plane;
-    A.stride = s->P.nc;
-    A.planes = s->P.nx + s->P.ny + s->P.nz - 2;
-    A.precond = s->cfg.preconditioner == PL3D_PRECOND_JACOBI ? 2 : 0;
-    A.overflow = &s->B.ctl->overflow;
-    A.cellIterations = &s->B.ctl->cellIterations;
-    A.solveCount = &s->B.ctl->solveCount;
     hipEvent_t a = nullptr;
-    if ((rc = timed_begin(s, &a))) return rc;
-    hipLaunchKernelGGL(A.precond == 2 ? k_pcg_jacobi : k_pcg_mic0, dim3(2), dim3(PL3D_SOLVE_THREADS), 0, s->stream, A, s->B.descs,
-                       &s->B.ctl->active);
+    if (L.timed)
+      if ((rc = timed_begin(L.timed, &a))) return rc;
+    hipLaunchKernelGGL(L.precond == 2 ? k_pcg_jacobi : k_pcg_mic0, dim3(2, L.members),
+                       dim3(PL3D_SOLVE_THREADS), 0, L.stream, L.tA, L.tD);
     HIP_OK(hipGetLastError());
-    if ((rc = timed_end(s, 6, a))) return rc;
+    if (L.timed)
+      if ((rc = timed_end(L.timed, 6, a))) return rc;
   }
-  if ((rc = launch(s, 7, k_faces_post, s->gridF, STEP_THREADS))) return rc;
-  if ((rc = launch(s, 8, k_extrap, s->gridExtrap, STEP_THREADS))) return rc;
-  if ((rc = launch(s, 9, k_advect, s->gridN, STEP_THREADS))) return rc;
+  if ((rc = launch(L, 7, k_faces_post, L.gridF, STEP_THREADS))) return rc;
+  if ((rc = launch(L, 8, k_extrap, L.gridExtrap, STEP_THREADS))) return rc;
+  if ((rc = launch(L, 9, k_advect, L.gridN, STEP_THREADS))) return rc;
   return PL3D_OK;
 }
 
-int build_graph(pl3d_sim *s) {
-  if (s->graph) return PL3D_OK;
-  hipGraph_t g;
-  HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_substep(s);
-  hipGraph_t gg = nullptr;
-  hipError_t e = hipStreamEndCapture(s->stream, &gg);
-  if (rc) return rc;
+int capture(const LaunchSet &L, hipGraphExec_t *out) {
+  HIP_OK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_substep(L);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(L.stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
   if (e != hipSuccess) return fail(PL3D_EDEVICE, "graph capture: %s", hipGetErrorString(e));
-  g = gg;
-  HIP_OK(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0));
-  HIP_OK(hipGraphDestroy(g));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(PL3D_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
   return PL3D_OK;
 }
 
 int run_substeps(pl3d_sim *s, int count) {
   if (s->timing) {
+    const LaunchSet L = single_set(s);
     for (int i = 0; i < count; i++) {
-      int rc = enqueue_substep(s);
+      int rc = enqueue_substep(L);
       if (rc) return rc;
     }
     return PL3D_OK;
   }
-  int rc = build_graph(s);
-  if (rc) return rc;
+  if (!s->graph)
+    if (int rc = capture(single_set(s), &s->graph)) return rc;
   for (int i = 0; i < count; i++) HIP_OK(hipGraphLaunch(s->graph, s->stream));
   return PL3D_OK;
 }
@@ -334,6 +375,13 @@ int check_sim(pl3d_sim *s) {
   return PL3D_OK;
 }
 
+// stepping entry points: a member of an ensemble is stepped only through its ensemble
+int check_steppable(pl3d_sim *s) {
+  if (int rc = check_sim(s)) return rc;
+  if (s->ensemble) return fail(PL3D_ESTATE, "simulation is a member of an ensemble");
+  return PL3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -350,6 +398,34 @@ const char *pl3d_version(void) { return "pourlab3d 0.1 gfx950"; }
 int pl3d_gentle_pour(double cell_mm, double seconds, pl3d_config *out) {
   if (!out || !(cell_mm > 0) || !(seconds >= 0)) return fail(PL3D_EINVAL, "bad gentle pour args");
   fill_gentle(cell_mm, seconds, out);
+  return PL3D_OK;
+}
+
+int pl3d_varied_pour(uint32_t seed, int32_t index, pl3d_config *out) {
+  if (!out || index < 0) return fail(PL3D_EINVAL, "bad varied pour arguments");
+  // one mulberry32 stream from `seed`; member `index` takes the 8 draws after those of the
+  // members before it
+  Mulberry32 rng{seed};
+  double u[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int32_t k = 0; k <= index; k++)
+    for (double &v : u) v = rng.next();
+  fill_gentle(5, 20, out);
+  const double radius = 0.05 + 0.03 * u[0];        // pot radius 5-8 cm
+  const double volume = (0.8 + 0.7 * u[1]) * 1e-3;  // fill 0.8-1.5 L
+  const double fill = volume / (PI * radius * radius);
+  out->teapot.radius = radius;
+  out->teapot.height = std::max(0.12, fill + 0.05);
+  out->teapot.bore_radius = 0.5 * (0.015 + 0.01 * u[2]);  // spout diameter 1.5-2.5 cm
+  out->teapot.inlet_height =
+      std::min(0.01 + 0.09 * u[3], out->teapot.height - 0.03);  // spout 1-10 cm up
+  out->initial_volume = volume;
+  out->motion.hold_tilt_rate = 5 + 15 * u[4];  // tilt rate 5-20 deg/s while holding
+  // cells 3-6 mm, coarsened where the liquid (half pot) would exceed ~11,000 cells so that every
+  // member stays within the one-workgroup solver's capacity with room for the poured stream
+  const double h = 0.003 + 0.003 * u[5];
+  const double hmin = std::cbrt(0.5 * volume / 11000.0);
+  out->h = std::max(h, hmin);
+  out->seed = seed + uint32_t(index) * 7919u + 1u;
   return PL3D_OK;
 }
 
@@ -547,6 +623,8 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       {(void **)&B.qD, NC * 8},
       {(void **)&B.porder, 2 * NC * 4}, {(void **)&B.pc2, 2 * NC * 4},
       {(void **)&B.nbrPlane, 2 * 3 * NC * 4}, {(void **)&B.descs, 2 * sizeof(PcgDesc)},
+      {(void **)&s->dB, sizeof(DevBufs)}, {(void **)&s->dP, sizeof(DevParams)},
+      {(void **)&s->dA, sizeof(PcgShared)},
   };
   for (auto &sl : slots) off += align256(sl.bytes);
   s->memBytes = off;
@@ -568,6 +646,31 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipHostMalloc((void **)&s->hctl, sizeof(DevCtl), hipHostMallocDefault)) != hipSuccess)
     return cleanup(fail(PL3D_EDEVICE, "stream/pinned alloc: %s", hipGetErrorString(e)));
+  // launch shapes (the member's own grids; an ensemble launches the largest over its members)
+  s->gridN = (n + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridC = (P.nc + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridF = (P.nfaces + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridFC = ((nx + P2G_TX) / P2G_TX) * ((ny + P2G_TY) / P2G_TY) *
+              ((nz + P2G_TZ) / P2G_TZ);  // bricks covering the (n+1)-sized face index space
+  s->gridExtrap = (((P.nfaces + 63) & ~63) + P.nc + STEP_THREADS - 1) / STEP_THREADS;
+  s->gridScan = P.scanBlocks;
+  P.gridN = s->gridN;
+  P.gridFC = s->gridFC;
+  {  // arguments shared by the two solves of a substep
+    PcgShared &A = s->A;
+    A.m = &B.ctl->m;
+    A.nbr = B.nbr;
+    A.diag = B.diag;
+    A.mic = B.mic;
+    A.plane = B.plane;
+    A.stride = P.nc;
+    A.planes = P.nx + P.ny + P.nz - 2;
+    A.precond = c.preconditioner == PL3D_PRECOND_JACOBI ? 2 : 0;
+    A.overflow = &B.ctl->overflow;
+    A.cellIterations = &B.ctl->cellIterations;
+    A.solveCount = &B.ctl->solveCount;
+    A.active = &B.ctl->active;
+  }
   // initial state
   std::vector<float> fx(NP), fy(NP), fz(NP), zero(NP, 0.f);
   for (int p = 0; p < n; p++) {
@@ -625,15 +728,10 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       (e = hipMemsetAsync(B.qP, 0, NC * 8, s->stream)) ||
       (e = hipMemsetAsync(B.qD, 0, NC * 8, s->stream)) ||
       (e = up(B.descs, descs, sizeof descs)) || (e = up(B.ctl, &H, sizeof(DevCtl))) ||
+      (e = up(s->dB, &B, sizeof(DevBufs))) || (e = up(s->dP, &P, sizeof(DevParams))) ||
+      (e = up(s->dA, &s->A, sizeof(PcgShared))) ||
       (e = hipStreamSynchronize(s->stream)))
     return cleanup(fail(PL3D_EDEVICE, "upload: %s", hipGetErrorString(e)));
-  s->gridN = (n + STEP_THREADS - 1) / STEP_THREADS;
-  s->gridC = (P.nc + STEP_THREADS - 1) / STEP_THREADS;
-  s->gridF = (P.nfaces + STEP_THREADS - 1) / STEP_THREADS;
-  s->gridFC = ((nx + P2G_TX) / P2G_TX) * ((ny + P2G_TY) / P2G_TY) *
-              ((nz + P2G_TZ) / P2G_TZ);  // bricks covering the (n+1)-sized face index space
-  s->gridExtrap = (((P.nfaces + 63) & ~63) + P.nc + STEP_THREADS - 1) / STEP_THREADS;
-  s->gridScan = P.scanBlocks;
   static const char *names[] = {"k_begin",   "k_count",  "k_classify_scan", "k_scatter",
                                 "k_p2g",     "k_liquid", "k_pcg",           "k_faces_post",
                                 "k_extrap",  "k_advect"};
@@ -645,8 +743,15 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   return PL3D_OK;
 }
 
+static int destroy_sim(pl3d_sim *s);
+
 int pl3d_destroy(pl3d_sim *s) {
   if (!s) return fail(PL3D_EINVAL, "null simulation handle");
+  if (s->ensemble) return fail(PL3D_ESTATE, "simulation is owned by an ensemble");
+  return destroy_sim(s);
+}
+
+static int destroy_sim(pl3d_sim *s) {
   if (s->graph) (void)hipGraphExecDestroy(s->graph);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto &p : s->pending) {
@@ -674,7 +779,7 @@ int pl3d_get_info(pl3d_sim *s, pl3d_info *out) {
 }
 
 int pl3d_step(pl3d_sim *s, double dt) {
-  if (int rc = check_sim(s)) return rc;
+  if (int rc = check_steppable(s)) return rc;
   if (!(dt > 0) || !std::isfinite(dt)) return fail(PL3D_EINVAL, "dt must be > 0");
   int rc;
   if ((rc = push_command(s, 1, 1, 0.0, dt))) return rc;
@@ -685,7 +790,7 @@ int pl3d_step(pl3d_sim *s, double dt) {
 }
 
 int pl3d_advance(pl3d_sim *s, double seconds, pl3d_diagnostics *out) {
-  if (int rc = check_sim(s)) return rc;
+  if (int rc = check_steppable(s)) return rc;
   if (!(seconds >= 0) || !std::isfinite(seconds)) return fail(PL3D_EINVAL, "bad duration");
   int rc;
   if ((rc = sync_ctl(s))) return rc;
@@ -886,7 +991,7 @@ int pl3d_read_affine(pl3d_sim *s, double *out) {
 }
 
 int pl3d_set_target(pl3d_sim *s, double angle_deg) {
-  if (int rc = check_sim(s)) return rc;
+  if (int rc = check_steppable(s)) return rc;
   if (int rc = sync_ctl(s)) return rc;
   const double a = std::max(0.0, std::min(s->cfg.motion.max_angle_deg, angle_deg));
   s->hctl->target = a * (PI / 180);
@@ -899,7 +1004,7 @@ int pl3d_set_target(pl3d_sim *s, double angle_deg) {
 }
 
 int pl3d_set_timing(pl3d_sim *s, int32_t enabled) {
-  if (int rc = check_sim(s)) return rc;
+  if (int rc = check_steppable(s)) return rc;
   if (int rc = sync_ctl(s)) return rc;
   s->timing = enabled != 0;
   for (auto &t : s->timers) {
@@ -968,6 +1073,175 @@ int pl3d_copy_probe(int64_t bytes, int32_t reps, double *ms_per_copy) {
   (void)hipEventElapsedTime(&ms, e0, e1);
   *ms_per_copy = double(ms) / reps;
   return done(PL3D_OK);
+}
+
+
+// ---- ensembles ---------------------------------------------------------------------------------
+
+static int read_ctls(pl3d_ensemble *e) {
+  for (pl3d_sim *m : e->members)
+    HIP_OK(hipMemcpyAsync(m->hctl, m->B.ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, e->stream));
+  HIP_OK(hipStreamSynchronize(e->stream));
+  return PL3D_OK;
+}
+
+static int check_ensemble(pl3d_ensemble *e) {
+  if (!e) return fail(PL3D_EINVAL, "null ensemble handle");
+  if (e->destroyed) return fail(PL3D_ESTATE, "ensemble was destroyed");
+  return PL3D_OK;
+}
+
+static void free_ensemble(pl3d_ensemble *e) {
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (pl3d_sim *m : e->members) {
+    m->ensemble = nullptr;
+    (void)destroy_sim(m);
+  }
+  if (e->mem) (void)hipFree(e->mem);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  e->destroyed = true;
+  delete e;
+}
+
+int pl3d_ensemble_create(const pl3d_config *cfgs, int32_t count, pl3d_ensemble **out) {
+  g_err.clear();
+  if (!cfgs || !out || count <= 0) return fail(PL3D_EINVAL, "bad ensemble arguments");
+  *out = nullptr;
+  for (int32_t i = 1; i < count; i++)
+    if (cfgs[i].preconditioner != cfgs[0].preconditioner)
+      return fail(PL3D_EINVAL, "ensemble members must share one preconditioner");
+  auto *e = new pl3d_ensemble();
+  for (int32_t i = 0; i < count; i++) {
+    pl3d_sim *m = nullptr;
+    const int rc = pl3d_create(&cfgs[i], &m);
+    if (rc) {
+      const std::string msg = g_err;
+      free_ensemble(e);
+      return fail(rc, "ensemble member %d: %s", int(i), msg.c_str());
+    }
+    m->ensemble = e;
+    e->members.push_back(m);
+    e->gridN = std::max(e->gridN, m->gridN);
+    e->gridC = std::max(e->gridC, m->gridC);
+    e->gridF = std::max(e->gridF, m->gridF);
+    e->gridFC = std::max(e->gridFC, m->gridFC);
+    e->gridExtrap = std::max(e->gridExtrap, m->gridExtrap);
+    e->gridScan = std::max(e->gridScan, m->gridScan);
+  }
+  e->precond = e->members[0]->A.precond;
+  const size_t K = size_t(count);
+  const size_t oB = 0, oP = align256(K * sizeof(DevBufs)),
+               oA = oP + align256(K * sizeof(DevParams)),
+               oD = oA + align256(K * sizeof(PcgShared)), total = oD + 2 * K * sizeof(PcgDesc);
+  hipError_t err;
+  if ((err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) ||
+      (err = hipMalloc(&e->mem, total))) {
+    free_ensemble(e);
+    return fail(PL3D_EDEVICE, "ensemble tables: %s", hipGetErrorString(err));
+  }
+  char *base = static_cast<char *>(e->mem);
+  e->tB = reinterpret_cast<DevBufs *>(base + oB);
+  e->tP = reinterpret_cast<DevParams *>(base + oP);
+  e->tA = reinterpret_cast<PcgShared *>(base + oA);
+  e->tD = reinterpret_cast<PcgDesc *>(base + oD);
+  std::vector<DevBufs> hb(K);
+  std::vector<DevParams> hp(K);
+  std::vector<PcgShared> ha(K);
+  for (size_t i = 0; i < K; i++) {
+    hb[i] = e->members[i]->B;
+    hp[i] = e->members[i]->P;
+    ha[i] = e->members[i]->A;
+  }
+  err = hipMemcpyAsync(e->tB, hb.data(), K * sizeof(DevBufs), hipMemcpyHostToDevice, e->stream);
+  if (!err)
+    err = hipMemcpyAsync(e->tP, hp.data(), K * sizeof(DevParams), hipMemcpyHostToDevice, e->stream);
+  if (!err)
+    err = hipMemcpyAsync(e->tA, ha.data(), K * sizeof(PcgShared), hipMemcpyHostToDevice, e->stream);
+  for (size_t i = 0; i < K && !err; i++)
+    err = hipMemcpyAsync(e->tD + 2 * i, e->members[i]->B.descs, 2 * sizeof(PcgDesc),
+                         hipMemcpyDeviceToDevice, e->stream);
+  if (!err) err = hipStreamSynchronize(e->stream);
+  if (err) {
+    free_ensemble(e);
+    return fail(PL3D_EDEVICE, "ensemble upload: %s", hipGetErrorString(err));
+  }
+  *out = e;
+  return PL3D_OK;
+}
+
+int pl3d_ensemble_destroy(pl3d_ensemble *e) {
+  if (int rc = check_ensemble(e)) return rc;
+  free_ensemble(e);
+  return PL3D_OK;
+}
+
+int pl3d_ensemble_size(pl3d_ensemble *e, int32_t *count) {
+  if (int rc = check_ensemble(e)) return rc;
+  if (!count) return fail(PL3D_EINVAL, "null out");
+  *count = int32_t(e->members.size());
+  return PL3D_OK;
+}
+
+int pl3d_ensemble_member(pl3d_ensemble *e, int32_t index, pl3d_sim **out) {
+  if (int rc = check_ensemble(e)) return rc;
+  if (!out || index < 0 || size_t(index) >= e->members.size())
+    return fail(PL3D_EINVAL, "member index %d out of range", int(index));
+  *out = e->members[size_t(index)];
+  return PL3D_OK;
+}
+
+int pl3d_ensemble_advance(pl3d_ensemble *e, double seconds, pl3d_diagnostics *out) {
+  if (int rc = check_ensemble(e)) return rc;
+  if (!(seconds >= 0) || !std::isfinite(seconds)) return fail(PL3D_EINVAL, "bad duration");
+  int rc;
+  if ((rc = read_ctls(e))) return rc;
+  const size_t K = e->members.size();
+  std::vector<double> end(K);
+  for (size_t i = 0; i < K; i++) {
+    pl3d_sim *m = e->members[i];
+    end[i] = m->hctl->time + seconds;
+    if ((rc = push_command_on(m, e->stream, 0, 0, end[i], 0.0))) return rc;
+  }
+  if (!e->graph) {
+    const LaunchSet L{e->stream, e->tB, e->tP, e->tA, e->tD, int(K), e->gridN, e->gridC,
+                      e->gridF, e->gridFC, e->gridExtrap, e->gridScan, e->precond, nullptr};
+    if ((rc = capture(L, &e->graph))) return rc;
+  }
+  for (int guard = 0; guard < 1000000; guard++) {
+    int batch = 0;
+    for (size_t i = 0; i < K; i++) {  // the member that needs the most substeps sets the batch
+      const pl3d_sim *m = e->members[i];
+      const double remain = end[i] - m->hctl->time;
+      if (!(remain > 1e-12) || m->hctl->time >= m->cfg.duration - 1e-9) continue;
+      const double dtEst = std::max(m->lastDt, 1e-5);
+      batch = std::max(batch, std::min(64, int(std::ceil(remain / dtEst)) + 1));
+    }
+    if (batch == 0) break;
+    for (int b = 0; b < batch; b++) HIP_OK(hipGraphLaunch(e->graph, e->stream));
+    if ((rc = read_ctls(e))) return rc;
+    bool any = false;
+    for (pl3d_sim *m : e->members) {
+      if (m->hctl->dt > 0) m->lastDt = m->hctl->dt;
+      any = any || m->hctl->active;
+    }
+    if (!any) break;
+  }
+  if (out)
+    for (size_t i = 0; i < K; i++)
+      if ((rc = pl3d_get_diagnostics(e->members[i], &out[i]))) return rc;
+  return PL3D_OK;
+}
+
+int pl3d_ensemble_finished(pl3d_ensemble *e, int32_t *out) {
+  if (int rc = check_ensemble(e)) return rc;
+  if (!out) return fail(PL3D_EINVAL, "null out");
+  if (int rc = read_ctls(e)) return rc;
+  for (size_t i = 0; i < e->members.size(); i++) {
+    const pl3d_sim *m = e->members[i];
+    out[i] = m->hctl->time >= m->cfg.duration - 1e-9 ? 1 : 0;
+  }
+  return PL3D_OK;
 }
 
 }  // extern "C"

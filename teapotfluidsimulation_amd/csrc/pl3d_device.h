@@ -99,6 +99,7 @@ struct DevParams {
   int nc, nu, nv, nw;
   int nfaces;
   int scanBlocks;
+  int gridN, gridFC;  // blocks of this member's parcel kernels and P2G bricks
   float h, invh;
   double hd;
   // geometry (Teapot3D, vessel3d.ts:34-59)

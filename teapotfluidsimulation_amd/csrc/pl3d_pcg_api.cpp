@@ -99,6 +99,7 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   const size_t o_pc2 = bytes; bytes += al(4 * S);
   const size_t o_nbp = bytes; bytes += al(12 * S);
   const size_t o_desc = bytes; bytes += al(sizeof(PcgDesc));
+  const size_t o_shared = bytes; bytes += al(sizeof(PcgShared));
   char *mem = nullptr;
   hipStream_t st = nullptr;
   hipError_t e;
@@ -141,20 +142,22 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   A.overflow = nullptr;
   A.cellIterations = nullptr;
   A.solveCount = nullptr;
+  A.active = nullptr;
   auto up = [&](size_t o, const void *src, size_t n) {
     return hipMemcpyAsync(mem + o, src, n, hipMemcpyHostToDevice, st);
   };
   if ((e = up(o_res, &hres, sizeof hres)) || (e = up(o_nbr, nbr.data(), 12 * S)) ||
       (e = up(o_diag, diag.data(), S)) || (e = up(o_mic, mic.data(), S)) ||
       (e = up(o_plane, plane.data(), 2 * S)) || (e = up(o_b, bl.data(), 8 * S)) ||
-      (e = up(o_desc, &D, sizeof D)))
+      (e = up(o_desc, &D, sizeof D)) || (e = up(o_shared, &A, sizeof A)))
     return done(perr(PL3D_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if ((e = hipEventCreate(&e0)) || (e = hipEventCreate(&e1)) || (e = hipEventRecord(e0, st)))
     return done(perr(PL3D_EDEVICE, std::string("events: ") + hipGetErrorString(e)));
   hipLaunchKernelGGL(A.precond == 2 ? k_pcg_jacobi : k_pcg_mic0, dim3(1),
-                     dim3(PL3D_SOLVE_THREADS), 0, st, A,
-                     reinterpret_cast<const PcgDesc *>(mem + o_desc), nullptr);
+                     dim3(PL3D_SOLVE_THREADS), 0, st,
+                     reinterpret_cast<const PcgShared *>(mem + o_shared),
+                     reinterpret_cast<const PcgDesc *>(mem + o_desc));
   (void)hipEventRecord(e1, st);
   if ((e = hipGetLastError()) != hipSuccess)
     return done(perr(PL3D_EDEVICE, std::string("launch: ") + hipGetErrorString(e)));

@@ -22,6 +22,7 @@ struct PcgShared {
   int *overflow;           // set when m exceeds PL3D_SOLVE_MMAX
   unsigned long long *cellIterations;  // += m * iterations per solve (nullptr: not counted)
   unsigned long long *solveCount;
+  const int *active;  // the substep's run flag (nullptr: always run)
 };
 
 // One solve.
@@ -42,5 +43,7 @@ struct PcgDesc {
   uint32_t *nbrPlane;
 };
 
-__global__ void k_pcg_mic0(PcgShared A, const PcgDesc *Ds, const int *activeFlag);
-__global__ void k_pcg_jacobi(PcgShared A, const PcgDesc *Ds, const int *activeFlag);
+// Grid (solves per member, members): block (x, y) runs solve Ds[y * gridDim.x + x] on the liquid
+// cells described by At[y].
+__global__ void k_pcg_mic0(const PcgShared *At, const PcgDesc *Ds);
+__global__ void k_pcg_jacobi(const PcgShared *At, const PcgDesc *Ds);

@@ -531,11 +531,11 @@ __device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const 
 // Common entry: capacity check, empty problem, then the body compiled for the smallest number of
 // register slots per thread that holds the liquid cells (fewer slots, fewer live registers).
 template <bool MIC>
-__device__ __forceinline__ void pcg_entry(SolveLds &L, const PcgShared &A,
-                                          const PcgDesc *__restrict__ Ds,
-                                          const int *__restrict__ activeFlag) {
-  if (activeFlag != nullptr && *activeFlag == 0) return;
-  const PcgDesc D = Ds[blockIdx.x];
+__device__ __forceinline__ void pcg_entry(SolveLds &L, const PcgShared *__restrict__ At,
+                                          const PcgDesc *__restrict__ Ds) {
+  const PcgShared A = At[blockIdx.y];
+  if (A.active != nullptr && *A.active == 0) return;
+  const PcgDesc D = Ds[blockIdx.y * gridDim.x + blockIdx.x];
   const int m = *A.m;
   if (m > MMAX - 1) {  // beyond capacity (one LDS slot is the zero slot): report, leave q = 0
     for (int t = threadIdx.x; t < m && t < A.stride; t += NT) D.q[t] = 0.0;
@@ -569,15 +569,16 @@ __device__ __forceinline__ void pcg_entry(SolveLds &L, const PcgShared &A,
 
 }  // namespace
 
-// One workgroup per solve.  Arguments shared by the solves of one launch in `A`, per-solve
-// arguments in `Ds[blockIdx.x]`; the preconditioner is a compile-time choice.
-__global__ __launch_bounds__(NT) void k_pcg_mic0(PcgShared A, const PcgDesc *__restrict__ Ds,
-                                                 const int *__restrict__ activeFlag) {
+// One workgroup per solve.  Arguments shared by the solves of one member in `At[blockIdx.y]`,
+// per-solve arguments in `Ds[blockIdx.y * gridDim.x + blockIdx.x]`; the preconditioner is a
+// compile-time choice.
+__global__ __launch_bounds__(NT) void k_pcg_mic0(const PcgShared *__restrict__ At,
+                                                 const PcgDesc *__restrict__ Ds) {
   __shared__ SolveLds L;
-  pcg_entry<true>(L, A, Ds, activeFlag);
+  pcg_entry<true>(L, At, Ds);
 }
-__global__ __launch_bounds__(NT) void k_pcg_jacobi(PcgShared A, const PcgDesc *__restrict__ Ds,
-                                                   const int *__restrict__ activeFlag) {
+__global__ __launch_bounds__(NT) void k_pcg_jacobi(const PcgShared *__restrict__ At,
+                                                   const PcgDesc *__restrict__ Ds) {
   __shared__ SolveLds L;
-  pcg_entry<false>(L, A, Ds, activeFlag);
+  pcg_entry<false>(L, At, Ds);
 }

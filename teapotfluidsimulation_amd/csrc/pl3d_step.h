@@ -46,14 +46,14 @@ struct DevBufs {
   PcgDesc *descs;  // 2
 };
 
-__global__ void k_begin(DevBufs B, DevParams P);
-__global__ void k_count(DevBufs B, DevParams P);
-__global__ void k_classify_scan(DevBufs B, DevParams P);
-__global__ void k_scatter(DevBufs B, DevParams P);
-__global__ void k_p2g(DevBufs B, DevParams P);
-__global__ void k_liquid(DevBufs B, DevParams P);
-__global__ void k_faces_post(DevBufs B, DevParams P);
-__global__ void k_extrap(DevBufs B, DevParams P);
-__global__ void k_advect(DevBufs B, DevParams P);
+__global__ void k_begin(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_count(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_classify_scan(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_scatter(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_p2g(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_liquid(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_faces_post(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_extrap(const DevBufs *Bt, const DevParams *Pt);
+__global__ void k_advect(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_render_positions(DevBufs B, DevParams P, float *out);
 __global__ void k_copy_probe(const float4 *a, float4 *b, size_t n4);

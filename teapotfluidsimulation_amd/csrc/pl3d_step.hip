@@ -113,6 +113,13 @@ __device__ __forceinline__ double stable_dt(const DevCtl &C, const DevParams &P)
   return fmin(P.maxSubstep, (P.cfl * P.hd) / speed);
 }
 
+// Every substep kernel serves the members of an ensemble: blockIdx.y selects the member, whose
+// buffers and parameters come from device tables (a single simulation is an ensemble of one).
+// Grids are sized for the largest member; blocks past a member's own grid leave at once.
+#define MEMBER()                           \
+  const DevBufs &B = Bt[blockIdx.y];       \
+  const DevParams &P = Pt[blockIdx.y]
+
 __device__ __forceinline__ bool is_active(const DevBufs &B) {
   return *(volatile int *)&B.ctl->active != 0;
 }
@@ -125,7 +132,8 @@ __device__ __forceinline__ long long fx40(float v) { return __double2ll_rn(doubl
 // ============================================================================= k_begin
 // One substep's scalar work: the substep length (stableDt or the caller's dt), the controller
 // (motion.ts:74), omega and the two poses; clears the per-substep counters.
-__global__ void k_begin(DevBufs B, DevParams P) {
+__global__ void k_begin(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
   for (int i = threadIdx.x; i < P.scanBlocks; i += blockDim.x) B.scanStatus[i] = 0ull;
   if (threadIdx.x != 0) return;
   DevCtl &C = *B.ctl;
@@ -166,7 +174,8 @@ __global__ void k_begin(DevBufs B, DevParams P) {
 
 // ============================================================================= k_count
 // Bin IN_POT parcels by cell (solver3d.ts:327-335 uses the same cell to mark FLUID).
-__global__ void k_count(DevBufs B, DevParams P) {
+__global__ void k_count(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
   if (!is_active(B)) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
@@ -187,8 +196,9 @@ __global__ void k_count(DevBufs B, DevParams P) {
 // AIR cell holding an IN_POT parcel, else AIR.  In the same pass, a single-pass decoupled
 // look-back scan over cells of (parcel count, FLUID) gives bin offsets and the liquid list in
 // grid index order (the order of Pcg3D's list, solver3d.ts:869).
-__global__ __launch_bounds__(STEP_THREADS) void k_classify_scan(DevBufs B, DevParams P) {
-  if (!is_active(B)) return;
+__global__ __launch_bounds__(STEP_THREADS) void k_classify_scan(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
+  if (int(blockIdx.x) >= P.scanBlocks || !is_active(B)) return;  // before taking a ticket
   __shared__ unsigned int tileId;
   __shared__ unsigned long long wsum[STEP_THREADS / 64];
   __shared__ unsigned long long prefixS;
@@ -301,7 +311,8 @@ __global__ __launch_bounds__(STEP_THREADS) void k_classify_scan(DevBufs B, DevPa
 }
 
 // ============================================================================= k_scatter
-__global__ void k_scatter(DevBufs B, DevParams P) {
+__global__ void k_scatter(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
   if (!is_active(B)) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P.n) return;
@@ -408,8 +419,9 @@ __device__ __forceinline__ void p2g_parcels(const DevBufs &B, const DevParams &P
   }
 }
 
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g(DevBufs B, DevParams P) {
-  if (!is_active(B)) return;
+__global__ __launch_bounds__(P2G_THREADS) void k_p2g(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
+  if (int(blockIdx.x) >= P.gridFC || !is_active(B)) return;
   const DevCtl &C = *B.ctl;
   __shared__ long long acc[4][P2G_TILE][2];  // u, v, w faces (w*v, w); cells (-, w)
   __shared__ int rowOff[P2G_MAXROWS + 1];
@@ -482,7 +494,8 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(DevBufs B, DevParams P) {
 // Per liquid cell: pressure rhs = -div u (divergenceRhs, solver3d.ts:473-493), the 7-point
 // operator's structure (Pcg3D.solve setup, :872-881), the density-correction rhs and the
 // interior-density statistic (densityCorrection, :643-670).
-__global__ __launch_bounds__(STEP_THREADS) void k_liquid(DevBufs B, DevParams P) {
+__global__ __launch_bounds__(STEP_THREADS) void k_liquid(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
   if (!is_active(B)) return;
   DevCtl &C = *B.ctl;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -552,7 +565,8 @@ __global__ __launch_bounds__(STEP_THREADS) void k_liquid(DevBufs B, DevParams P)
 // ============================================================================= k_faces_post
 // applyGradient (solver3d.ts:496-528): u -= q_b - q_a on faces next to liquid with no solid side
 // (pressure, in FP64 differences), and uc = -(qc_b - qc_a) (density correction, :674).
-__global__ void k_faces_post(DevBufs B, DevParams P) {
+__global__ void k_faces_post(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
   if (!is_active(B)) return;
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= P.nfaces) return;
@@ -607,7 +621,8 @@ __device__ __forceinline__ int layer1(const float *u, const uint8_t *vm, int c, 
   return n;
 }
 
-__global__ void k_extrap(DevBufs B, DevParams P) {
+__global__ void k_extrap(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
   if (!is_active(B)) return;
   int id = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = P.nx, ny = P.ny, nz = P.nz;
@@ -683,8 +698,9 @@ __global__ void k_extrap(DevBufs B, DevParams P) {
 // midpoint on the extrapolated field, density-correction displacement, SDF collision at the new
 // pose with the moving-wall normal velocity removed, symmetry reflection, LOST / DISCHARGED /
 // SPILLED.  The last block to finish closes the substep (time, steps, pose, max speed).
-__global__ __launch_bounds__(STEP_THREADS) void k_advect(DevBufs B, DevParams P) {
-  if (!is_active(B)) return;
+__global__ __launch_bounds__(STEP_THREADS) void k_advect(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  MEMBER();
+  if (int(blockIdx.x) >= P.gridN || !is_active(B)) return;  // before taking a ticket
   DevCtl &C = *B.ctl;
   __shared__ float smax[STEP_THREADS / 64];
   __shared__ bool lastBlock;
@@ -872,7 +888,7 @@ __global__ __launch_bounds__(STEP_THREADS) void k_advect(DevBufs B, DevParams P)
     atomicMax(&C.stepMaxSpeedSqBits, __float_as_uint(bm));
     __threadfence();
     const unsigned int ticket = atomicAdd(&C.doneTicket, 1u);
-    lastBlock = ticket == gridDim.x - 1;
+    lastBlock = ticket == unsigned(P.gridN) - 1u;
   }
   __syncthreads();
   if (lastBlock && threadIdx.x == 0) {  // close the substep (solver3d.ts:306-308)

@@ -27,6 +27,13 @@ def gentlePour3D(cellMm: float, seconds: float) -> dict:
     return abi.config_to_dict(c)
 
 
+def variedPour3D(seed: int, index: int) -> dict:
+    """Member `index` of the seeded family of varied pours (pl3d_varied_pour)."""
+    c = abi.Config()
+    check(load().pl3d_varied_pour(int(seed), int(index), C.byref(c)))
+    return abi.config_to_dict(c)
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
@@ -34,11 +41,15 @@ def _ptr(a):
 class Simulation3D:
     """Simulation3D (solver3d.ts:62-824) / GpuSimulation3D (gpu3d/solver.ts:89-490)."""
 
-    def __init__(self, cfg: dict):
+    def __init__(self, cfg: dict, _member_of=None, _handle=None):
         self.cfg = dict(cfg)
         self._c = abi.config_from_dict(cfg)
-        h = C.c_void_p()
-        check(load().pl3d_create(C.byref(self._c), C.byref(h)))
+        self._owner = _member_of
+        if _handle is None:
+            h = C.c_void_p()
+            check(load().pl3d_create(C.byref(self._c), C.byref(h)))
+        else:
+            h = _handle
         self._h = h
         info = abi.Info()
         check(load().pl3d_get_info(self._h, C.byref(info)))
@@ -53,7 +64,11 @@ class Simulation3D:
 
     # ------------------------------------------------------------------ lifecycle
     def destroy(self):
-        """Free the device memory (GpuSimulation3D.destroy)."""
+        """Free the device memory (GpuSimulation3D.destroy).  Members of an ensemble are freed
+        with the ensemble."""
+        if getattr(self, "_owner", None) is not None:
+            self._h = None
+            return
         if getattr(self, "_h", None):
             check(load().pl3d_destroy(self._h))
             self._h = None
@@ -249,3 +264,57 @@ class Pcg3D:
             "liquidCells": res.liquid_cells,
             "kernelMs": res.kernel_ms,
         }
+
+
+class Ensemble3D:
+    """Independent pours, possibly on different grids, stepped together on one device: each
+    substep kernel is one launch for all members.  Each member gives exactly the results of a
+    Simulation3D with the same config advanced by the same durations.  No counterpart in the
+    reference (its driver runs one Simulation3D at a time)."""
+
+    def __init__(self, cfgs):
+        cfgs = [dict(c) for c in cfgs]
+        if not cfgs:
+            raise ValueError("an ensemble needs at least one config")
+        arr = (abi.Config * len(cfgs))(*[abi.config_from_dict(c) for c in cfgs])
+        h = C.c_void_p()
+        check(load().pl3d_ensemble_create(arr, len(cfgs), C.byref(h)))
+        self._h = h
+        self.members = []
+        for i, c in enumerate(cfgs):
+            m = C.c_void_p()
+            check(load().pl3d_ensemble_member(self._h, i, C.byref(m)))
+            self.members.append(Simulation3D(c, _member_of=self, _handle=m))
+
+    def __len__(self):
+        return len(self.members)
+
+    def _handle(self):
+        if not getattr(self, "_h", None):
+            raise RuntimeError("ensemble was destroyed")
+        return self._h
+
+    def advance(self, duration: float) -> list:
+        """advance(duration) on every member; returns their Diagnostics3D dicts."""
+        out = (abi.Diagnostics * len(self.members))()
+        check(load().pl3d_ensemble_advance(self._handle(), float(duration), out))
+        return [abi.diagnostics_dict(d) for d in out]
+
+    @property
+    def finished(self) -> list:
+        out = (C.c_int32 * len(self.members))()
+        check(load().pl3d_ensemble_finished(self._handle(), out))
+        return [bool(v) for v in out]
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            for m in self.members:
+                m._h = None
+            check(load().pl3d_ensemble_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
