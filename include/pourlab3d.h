@@ -270,9 +270,11 @@ int pl3d_ensemble_size(pl3d_ensemble *ens, int32_t *count);
 /* Borrowed handle of member `index` for the query entry points (diagnostics, fractions, state,
  * stats, info).  Stepping, timing, setTarget and destroy on a member return PL3D_ESTATE. */
 int pl3d_ensemble_member(pl3d_ensemble *ens, int32_t index, pl3d_sim **out);
-/* advance(seconds) for every member (Simulation3D.advance, solver3d.ts:281-286); out, if not
- * NULL, receives `count` Diagnostics3D records. */
-int pl3d_ensemble_advance(pl3d_ensemble *ens, double seconds, pl3d_diagnostics *out);
+/* advance(seconds) for every member (Simulation3D.advance, solver3d.ts:281-286), or only for the
+ * members whose mask entry is non-zero (mask NULL: all); masked-out members take no substep.
+ * out, if not NULL, receives `count` Diagnostics3D records. */
+int pl3d_ensemble_advance(pl3d_ensemble *ens, double seconds, const int32_t *mask,
+                          pl3d_diagnostics *out);
 /* finished per member: out receives `count` flags. */
 int pl3d_ensemble_finished(pl3d_ensemble *ens, int32_t *out);
 

@@ -46,7 +46,7 @@ SIGNATURES = {
     "pl3d_ensemble_destroy": (C.c_int, [_vp]),
     "pl3d_ensemble_size": (C.c_int, [_vp, _P(_i32)]),
     "pl3d_ensemble_member": (C.c_int, [_vp, _i32, _P(_vp)]),
-    "pl3d_ensemble_advance": (C.c_int, [_vp, _d, _P(abi.Diagnostics)]),
+    "pl3d_ensemble_advance": (C.c_int, [_vp, _d, _vp, _P(abi.Diagnostics)]),
     "pl3d_ensemble_finished": (C.c_int, [_vp, _P(_i32)]),
 }
 

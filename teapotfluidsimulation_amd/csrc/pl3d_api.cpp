@@ -1191,7 +1191,8 @@ int pl3d_ensemble_member(pl3d_ensemble *e, int32_t index, pl3d_sim **out) {
   return PL3D_OK;
 }
 
-int pl3d_ensemble_advance(pl3d_ensemble *e, double seconds, pl3d_diagnostics *out) {
+int pl3d_ensemble_advance(pl3d_ensemble *e, double seconds, const int32_t *mask,
+                          pl3d_diagnostics *out) {
   if (int rc = check_ensemble(e)) return rc;
   if (!(seconds >= 0) || !std::isfinite(seconds)) return fail(PL3D_EINVAL, "bad duration");
   int rc;
@@ -1200,7 +1201,7 @@ int pl3d_ensemble_advance(pl3d_ensemble *e, double seconds, pl3d_diagnostics *ou
   std::vector<double> end(K);
   for (size_t i = 0; i < K; i++) {
     pl3d_sim *m = e->members[i];
-    end[i] = m->hctl->time + seconds;
+    end[i] = m->hctl->time + ((mask == nullptr || mask[i]) ? seconds : 0.0);
     if ((rc = push_command_on(m, e->stream, 0, 0, end[i], 0.0))) return rc;
   }
   if (!e->graph) {

@@ -294,10 +294,16 @@ class Ensemble3D:
             raise RuntimeError("ensemble was destroyed")
         return self._h
 
-    def advance(self, duration: float) -> list:
-        """advance(duration) on every member; returns their Diagnostics3D dicts."""
+    def advance(self, duration: float, mask=None) -> list:
+        """advance(duration) on every member, or on those with a true entry in `mask`; returns
+        the Diagnostics3D dicts of all members."""
         out = (abi.Diagnostics * len(self.members))()
-        check(load().pl3d_ensemble_advance(self._handle(), float(duration), out))
+        m = None
+        if mask is not None:
+            if len(mask) != len(self.members):
+                raise ValueError("mask needs one entry per member")
+            m = (C.c_int32 * len(self.members))(*[1 if v else 0 for v in mask])
+        check(load().pl3d_ensemble_advance(self._handle(), float(duration), m, out))
         return [abi.diagnostics_dict(d) for d in out]
 
     @property

@@ -46,6 +46,16 @@ def test_members_match_single_runs():
         sim.destroy()
 
 
+def test_mask_holds_members():
+    cfgs = configs()[:3]
+    ens = Ensemble3D(cfgs)
+    d1 = ens.advance(0.25, mask=[True, False, True])
+    assert d1[1]["steps"] == 0 and d1[0]["steps"] > 0 and d1[2]["steps"] > 0
+    d2 = ens.advance(0.25, mask=[False, True, False])
+    assert d2[0]["steps"] == d1[0]["steps"] and d2[1]["time"] >= 0.25 - 1e-9
+    ens.destroy()
+
+
 def test_member_handles_are_not_steppable():
     ens = Ensemble3D(configs()[:2])
     lib = load()
