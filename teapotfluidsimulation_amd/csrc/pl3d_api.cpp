@@ -153,6 +153,7 @@ struct pl3d_sim {
   DevParams *dP = nullptr;
   PcgShared *dA = nullptr;
   PcgShared A{};
+  unsigned char *mg = nullptr;  // multigrid scratch of the two solves (MG only)
   pl3d_ensemble *ensemble = nullptr;  // set while a member of an ensemble (stepped by it)
 };
 
@@ -282,8 +283,8 @@ int enqueue_substep(const LaunchSet &L) {
     hipEvent_t a = nullptr;
     if (L.timed)
       if ((rc = timed_begin(L.timed, &a))) return rc;
-    hipLaunchKernelGGL(L.precond == 2 ? k_pcg_jacobi : k_pcg_mic0, dim3(2, L.members),
-                       dim3(PL3D_SOLVE_THREADS), 0, L.stream, L.tA, L.tD);
+    hipLaunchKernelGGL(L.precond == 2 ? k_pcg_jacobi : (L.precond == 1 ? k_pcg_mg : k_pcg_mic0),
+                       dim3(2, L.members), dim3(PL3D_SOLVE_THREADS), 0, L.stream, L.tA, L.tD);
     HIP_OK(hipGetLastError());
     if (L.timed)
       if ((rc = timed_end(L.timed, 6, a))) return rc;
@@ -437,7 +438,8 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   if (!(c.h > 0) || !(c.initial_volume > 0) || !(c.cfl > 0) || !(c.max_substep > 0) ||
       c.pressure_max_iterations < 0 || !(c.teapot.radius > 0))
     return fail(PL3D_EINVAL, "invalid Config3D");
-  if (c.preconditioner != PL3D_PRECOND_MIC0 && c.preconditioner != PL3D_PRECOND_JACOBI)
+  if (c.preconditioner != PL3D_PRECOND_MIC0 && c.preconditioner != PL3D_PRECOND_JACOBI &&
+      c.preconditioner != PL3D_PRECOND_MG)
     return fail(PL3D_EINVAL, "preconditioner %d is not available", c.preconditioner);
   if (c.transfer != PL3D_TRANSFER_FLIP && c.transfer != PL3D_TRANSFER_APIC)
     return fail(PL3D_EINVAL, "transfer %d is not available", c.transfer);
@@ -625,6 +627,9 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       {(void **)&B.nbrPlane, 2 * 3 * NC * 4}, {(void **)&B.descs, 2 * sizeof(PcgDesc)},
       {(void **)&s->dB, sizeof(DevBufs)}, {(void **)&s->dP, sizeof(DevParams)},
       {(void **)&s->dA, sizeof(PcgShared)},
+      {(void **)&s->mg, c.preconditioner == PL3D_PRECOND_MG
+                            ? 2 * size_t(pl3d_mg_layout(nx, ny, nz).bytes)
+                            : 0},
   };
   for (auto &sl : slots) off += align256(sl.bytes);
   s->memBytes = off;
@@ -640,7 +645,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     return cleanup(fail(PL3D_EDEVICE, "hipMalloc(%zu): %s", off, hipGetErrorString(e)));
   off = 0;
   for (auto &sl : slots) {
-    *sl.ptr = static_cast<char *>(s->mem) + off;
+    *sl.ptr = sl.bytes > 0 ? static_cast<char *>(s->mem) + off : nullptr;
     off += align256(sl.bytes);
   }
   if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
@@ -665,11 +670,18 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     A.plane = B.plane;
     A.stride = P.nc;
     A.planes = P.nx + P.ny + P.nz - 2;
-    A.precond = c.preconditioner == PL3D_PRECOND_JACOBI ? 2 : 0;
+    A.precond = c.preconditioner == PL3D_PRECOND_JACOBI ? 2
+                : (c.preconditioner == PL3D_PRECOND_MG ? 1 : 0);
     A.overflow = &B.ctl->overflow;
     A.cellIterations = &B.ctl->cellIterations;
     A.solveCount = &B.ctl->solveCount;
     A.active = &B.ctl->active;
+    A.flags = B.flag;
+    A.lidx = B.lidx;
+    A.lcell = B.lcell;
+    A.nx = P.nx;
+    A.ny = P.ny;
+    A.nz = P.nz;
   }
   // initial state
   std::vector<float> fx(NP), fy(NP), fz(NP), zero(NP, 0.f);
@@ -705,6 +717,10 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   descs[1].porder = B.porder + NC;
   descs[1].pc2 = B.pc2 + NC;
   descs[1].nbrPlane = B.nbrPlane + 3 * NC;
+  if (s->mg) {
+    descs[0].mg = s->mg;
+    descs[1].mg = s->mg + pl3d_mg_layout(nx, ny, nz).bytes;
+  }
   DevCtl &H = *s->hctl;
   std::memset(&H, 0, sizeof(DevCtl));
   H.phase = c.motion.kind == PL3D_MOTION_STILL ? PL3D_PHASE_DONE

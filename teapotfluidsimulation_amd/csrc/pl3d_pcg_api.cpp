@@ -30,7 +30,8 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   pl3d_set_error(PL3D_OK, "");
   if (!cell || !b || !q || nx < 3 || ny < 3 || nz < 3 || max_iter < 0)
     return perr(PL3D_EINVAL, "bad arguments");
-  if (precond != PL3D_PRECOND_MIC0 && precond != PL3D_PRECOND_JACOBI)
+  if (precond != PL3D_PRECOND_MIC0 && precond != PL3D_PRECOND_JACOBI &&
+      precond != PL3D_PRECOND_MG)
     return perr(PL3D_EINVAL, "preconditioner not available");
   const size_t nc = size_t(nx) * ny * nz;
   const int nxy = nx * ny;
@@ -100,6 +101,11 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   const size_t o_nbp = bytes; bytes += al(12 * S);
   const size_t o_desc = bytes; bytes += al(sizeof(PcgDesc));
   const size_t o_shared = bytes; bytes += al(sizeof(PcgShared));
+  const size_t o_flags = bytes; bytes += al(nc);
+  const size_t o_lidx = bytes; bytes += al(4 * nc);
+  const size_t o_lcell = bytes; bytes += al(4 * S);
+  const size_t o_mg = bytes;
+  if (precond == PL3D_PRECOND_MG) bytes += al(pl3d_mg_layout(nx, ny, nz).bytes);
   char *mem = nullptr;
   hipStream_t st = nullptr;
   hipError_t e;
@@ -130,6 +136,7 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   D.porder = reinterpret_cast<int *>(mem + o_porder);
   D.pc2 = reinterpret_cast<float *>(mem + o_pc2);
   D.nbrPlane = reinterpret_cast<uint32_t *>(mem + o_nbp);
+  D.mg = precond == PL3D_PRECOND_MG ? reinterpret_cast<unsigned char *>(mem + o_mg) : nullptr;
   PcgShared A;
   A.m = &dres->m;
   A.nbr = reinterpret_cast<uint32_t *>(mem + o_nbr);
@@ -138,23 +145,32 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   A.plane = reinterpret_cast<uint16_t *>(mem + o_plane);
   A.stride = stride;
   A.planes = nx + ny + nz - 2;
-  A.precond = precond == PL3D_PRECOND_JACOBI ? 2 : 0;
+  A.precond = precond == PL3D_PRECOND_JACOBI ? 2 : (precond == PL3D_PRECOND_MG ? 1 : 0);
   A.overflow = nullptr;
   A.cellIterations = nullptr;
   A.solveCount = nullptr;
   A.active = nullptr;
+  A.flags = reinterpret_cast<const uint8_t *>(mem + o_flags);
+  A.lidx = reinterpret_cast<const int *>(mem + o_lidx);
+  A.lcell = reinterpret_cast<const int *>(mem + o_lcell);
+  A.nx = nx;
+  A.ny = ny;
+  A.nz = nz;
   auto up = [&](size_t o, const void *src, size_t n) {
     return hipMemcpyAsync(mem + o, src, n, hipMemcpyHostToDevice, st);
   };
   if ((e = up(o_res, &hres, sizeof hres)) || (e = up(o_nbr, nbr.data(), 12 * S)) ||
       (e = up(o_diag, diag.data(), S)) || (e = up(o_mic, mic.data(), S)) ||
       (e = up(o_plane, plane.data(), 2 * S)) || (e = up(o_b, bl.data(), 8 * S)) ||
-      (e = up(o_desc, &D, sizeof D)) || (e = up(o_shared, &A, sizeof A)))
+      (e = up(o_desc, &D, sizeof D)) || (e = up(o_shared, &A, sizeof A)) ||
+      (e = up(o_flags, cell, nc)) || (e = up(o_lidx, lidx.data(), 4 * nc)) ||
+      (m > 0 && (e = up(o_lcell, lcell.data(), 4 * size_t(m)))))
     return done(perr(PL3D_EDEVICE, std::string("upload: ") + hipGetErrorString(e)));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if ((e = hipEventCreate(&e0)) || (e = hipEventCreate(&e1)) || (e = hipEventRecord(e0, st)))
     return done(perr(PL3D_EDEVICE, std::string("events: ") + hipGetErrorString(e)));
-  hipLaunchKernelGGL(A.precond == 2 ? k_pcg_jacobi : k_pcg_mic0, dim3(1),
+  hipLaunchKernelGGL(A.precond == 2 ? k_pcg_jacobi : (A.precond == 1 ? k_pcg_mg : k_pcg_mic0),
+                     dim3(1),
                      dim3(PL3D_SOLVE_THREADS), 0, st,
                      reinterpret_cast<const PcgShared *>(mem + o_shared),
                      reinterpret_cast<const PcgDesc *>(mem + o_desc));

@@ -23,6 +23,11 @@ struct PcgShared {
   unsigned long long *cellIterations;  // += m * iterations per solve (nullptr: not counted)
   unsigned long long *solveCount;
   const int *active;  // the substep's run flag (nullptr: always run)
+  // grid view, used by the multigrid hierarchy
+  const uint8_t *flags;  // nx*ny*nz cell flags (AIR 0, FLUID 1, SOLID 2)
+  const int *lidx;       // grid index -> list index (-1 off the liquid)
+  const int *lcell;      // list index -> grid index
+  int nx, ny, nz;
 };
 
 // One solve.
@@ -41,7 +46,61 @@ struct PcgDesc {
   int *porder;
   float *pc2;
   uint32_t *nbrPlane;
+  unsigned char *mg;  // multigrid scratch of pl3d_mg_scratch_bytes(nx, ny, nz) (MG only)
 };
+
+// ---- multigrid (V(2,2) red-black Gauss-Seidel, up to PL3D_MG_LEVELS levels) -------------------
+#define PL3D_MG_LEVELS 4
+#define PL3D_MG_CAP1 5120  // liquid cells of coarse levels 1..3 (their corrections live in LDS)
+#define PL3D_MG_CAP2 896
+#define PL3D_MG_CAP3 128
+
+// Byte offsets of one solve's multigrid scratch (computed the same way on host and device).
+struct MgLayout {
+  int n[PL3D_MG_LEVELS][3];  // grid dims per level (level 0 = the pressure grid)
+  unsigned long long flags[PL3D_MG_LEVELS], lidx[PL3D_MG_LEVELS], lcell[PL3D_MG_LEVELS],
+      nbr[PL3D_MG_LEVELS], dc[PL3D_MG_LEVELS], child[PL3D_MG_LEVELS], b[PL3D_MG_LEVELS],
+      res[PL3D_MG_LEVELS], parent[PL3D_MG_LEVELS];
+  int cap[PL3D_MG_LEVELS];
+  unsigned long long bytes;
+};
+
+__host__ __device__ inline MgLayout pl3d_mg_layout(int nx, int ny, int nz) {
+  MgLayout L{};
+  unsigned long long o = 0;
+  auto take = [&](unsigned long long n) {
+    const unsigned long long at = o;
+    o += (n + 255ull) & ~255ull;
+    return at;
+  };
+  const int caps[PL3D_MG_LEVELS] = {16384, PL3D_MG_CAP1, PL3D_MG_CAP2, PL3D_MG_CAP3};
+  L.n[0][0] = nx;
+  L.n[0][1] = ny;
+  L.n[0][2] = nz;
+  for (int l = 0; l < PL3D_MG_LEVELS; l++) {
+    if (l > 0)
+      for (int d = 0; d < 3; d++) L.n[l][d] = (L.n[l - 1][d] + 1) / 2;
+    const unsigned long long nc =
+        (unsigned long long)L.n[l][0] * (unsigned long long)L.n[l][1] * L.n[l][2];
+    const unsigned long long cap = (unsigned long long)caps[l];
+    L.cap[l] = caps[l];
+    if (l > 0) {
+      L.flags[l] = take(nc);
+      L.lidx[l] = take(4 * nc);
+      L.lcell[l] = take(4 * cap);
+      L.nbr[l] = take(12 * cap);
+      L.dc[l] = take(cap);
+      L.child[l] = take(32 * cap);
+      L.b[l] = take(4 * cap);
+    }
+    L.res[l] = take(4 * cap);
+    L.parent[l] = take(4 * cap);
+  }
+  L.bytes = o;
+  return L;
+}
+
+__global__ void k_pcg_mg(const PcgShared *At, const PcgDesc *Ds);
 
 // Grid (solves per member, members): block (x, y) runs solve Ds[y * gridDim.x + x] on the liquid
 // cells described by At[y].

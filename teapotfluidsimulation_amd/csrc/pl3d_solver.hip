@@ -27,6 +27,8 @@
 //           whole hyperplanes i+j+k = l in increasing l (decreasing for the backward sweep) gives
 //           the same factor and the same triangular solves, with one barrier per hyperplane.
 //   Jacobi  z = r / diag, recomputed from the register-resident neighbour words where needed.
+#include <type_traits>
+
 #include "pl3d_device.h"
 #include "pl3d_solver.h"
 
@@ -103,13 +105,38 @@ struct Red {
 // neighbours here, so the stencil product needs no branches.
 #define ZSLOT (MMAX - 1)
 
+enum { PC_MIC = 0, PC_MG = 1, PC_JAC = 2 };
+
+struct MicLds {
+  int planeStart[PL3D_MAX_PLANES + 1];
+  int planeFill[PL3D_MAX_PLANES];
+  int planeRange[2];
+};
+
+#define MG_XCAP (PL3D_MG_CAP1 + PL3D_MG_CAP2 + PL3D_MG_CAP3)
+struct MgLds {
+  float X[MG_XCAP];           // corrections of coarse levels 1..3 (offsets mg_xoff)
+  int scan[NT];
+  int m[PL3D_MG_LEVELS];      // liquid cells per level
+  int nl;                     // levels in use
+  int ok;                     // 0: a coarse level exceeded its capacity; Jacobi for this solve
+  MgLayout lay;
+};
+__device__ __forceinline__ int mg_xoff(int l) {
+  return l == 1 ? 0 : (l == 2 ? PL3D_MG_CAP1 : PL3D_MG_CAP1 + PL3D_MG_CAP2);
+}
+
+struct NoLds {
+  int unused;
+};
+
+template <int PC>
 struct SolveLds {
   float s[MMAX];
   float W[MMAX];
   Red R;
-  int planeStart[PL3D_MAX_PLANES + 1];
-  int planeFill[PL3D_MAX_PLANES];
-  int planeRange[2];
+  typename std::conditional<PC == PC_MIC, MicLds,
+                            typename std::conditional<PC == PC_MG, MgLds, NoLds>::type>::type x;
 };
 
 // Block-wide reductions with alternating buffers: one barrier each; every thread adds the 16
@@ -215,6 +242,243 @@ __device__ __attribute__((noinline)) double fold_and_residual(
   return mx;
 }
 
+// ---- multigrid hierarchy and coarse V-cycle -----------------------------------------------------
+// Coarsening by 2 per axis (src/sim/multigrid.ts:62-79 in 3D): a coarse cell is SOLID on the
+// border, else AIR if any child is AIR, else FLUID if any child is FLUID, else SOLID.  Each level
+// lists its FLUID cells in grid order with 16-bit neighbour indices (AIR / SOLID sentinels), the
+// non-solid neighbour count and the red-black colour (bit 7 of dc), its up-to-8 liquid children
+// and, for the level below, each liquid cell's parent.  Restriction sums the children's
+// residuals and halves the sum (4 x their average: the 7-point operator scales with the cell size
+// squared); prolongation adds the parent's correction (piecewise constant).  Smoothing is red-black
+// Gauss-Seidel, red then black going down and black then red coming up, and the coarsest level
+// runs MG_COARSE_SWEEPS sweeps each way, so the V-cycle is a symmetric operator as CG requires.
+#define MG_COARSE_SWEEPS 8
+
+// exclusive block scan of one int per thread (through G.scan); returns this thread's offset and
+// writes the total to *total (uniform)
+__device__ int block_exclusive_scan(int *scan, int v, int *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) scan[w] = incl;
+  __syncthreads();
+  int wbase = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < NWAVES; i++) {
+    const int x = scan[i];
+    if (i < w) wbase += x;
+    tot += x;
+  }
+  __syncthreads();  // scan[] reusable
+  *total = tot;
+  return wbase + incl - v;
+}
+
+__device__ __attribute__((noinline)) void mg_setup(MgLds &G, const PcgShared &A,
+                                                   unsigned char *base, int m) {
+  if (threadIdx.x == 0) {
+    G.lay = pl3d_mg_layout(A.nx, A.ny, A.nz);
+    G.m[0] = m;
+    G.nl = 1;
+    G.ok = 1;
+  }
+  __syncthreads();
+  const MgLayout &Y = G.lay;
+  for (int l = 1; l < PL3D_MG_LEVELS; l++) {
+    const int fx = Y.n[l - 1][0], fy = Y.n[l - 1][1], fz = Y.n[l - 1][2];
+    const int cx = Y.n[l][0], cy = Y.n[l][1], cz = Y.n[l][2];
+    const int ncl = cx * cy * cz, cap = Y.cap[l];
+    const uint8_t *ff = l == 1 ? A.flags : base + Y.flags[l - 1];
+    const int *flidx = l == 1 ? A.lidx : reinterpret_cast<const int *>(base + Y.lidx[l - 1]);
+    const int *flcell = l == 1 ? A.lcell : reinterpret_cast<const int *>(base + Y.lcell[l - 1]);
+    uint8_t *cf = base + Y.flags[l];
+    int *clidx = reinterpret_cast<int *>(base + Y.lidx[l]);
+    int *clcell = reinterpret_cast<int *>(base + Y.lcell[l]);
+    // classify a contiguous chunk of coarse cells per thread and count its FLUID cells
+    const int chunk = (ncl + NT - 1) / NT;
+    const int c0 = min(ncl, int(threadIdx.x) * chunk), c1 = min(ncl, c0 + chunk);
+    int cnt = 0;
+    for (int C = c0; C < c1; C++) {
+      const int I = C % cx, J = (C / cx) % cy, K = C / (cx * cy);
+      uint8_t f = PL3D_SOLID_;
+      if (I > 0 && J > 0 && K > 0 && I < cx - 1 && J < cy - 1 && K < cz - 1) {
+        bool air = false, liq = false;
+        for (int q = 0; q < 8; q++) {
+          const int i = 2 * I + (q & 1), j = 2 * J + ((q >> 1) & 1), k = 2 * K + (q >> 2);
+          if (i >= fx || j >= fy || k >= fz) continue;
+          const uint8_t t = ff[i + fx * (j + fy * k)];
+          air = air || t == PL3D_AIR_;
+          liq = liq || t == PL3D_FLUID_;
+        }
+        f = air ? PL3D_AIR_ : (liq ? PL3D_FLUID_ : PL3D_SOLID_);
+      }
+      cf[C] = f;
+      cnt += f == PL3D_FLUID_ ? 1 : 0;
+    }
+    int ml;
+    int run = block_exclusive_scan(G.scan, cnt, &ml);
+    if (ml > cap) {  // uniform
+      if (threadIdx.x == 0) G.ok = 0;
+      break;
+    }
+    if (ml == 0) break;
+    for (int C = c0; C < c1; C++) {
+      if (cf[C] == PL3D_FLUID_) {
+        clidx[C] = run;
+        clcell[run] = C;
+        run++;
+      } else {
+        clidx[C] = -1;
+      }
+    }
+    __syncthreads();
+    uint32_t *cn = reinterpret_cast<uint32_t *>(base + Y.nbr[l]);
+    uint8_t *cdc = base + Y.dc[l];
+    int *cch = reinterpret_cast<int *>(base + Y.child[l]);
+    for (int i = threadIdx.x; i < ml; i += NT) {
+      const int C = clcell[i];
+      const int I = C % cx, J = (C / cx) % cy, K = C / (cx * cy);
+      const int offs[6] = {-1, 1, -cx, cx, -cx * cy, cx * cy};
+      uint32_t nb[6];
+      int d = 0;
+      for (int q = 0; q < 6; q++) {
+        const uint8_t t = cf[C + offs[q]];
+        d += t != PL3D_SOLID_ ? 1 : 0;
+        nb[q] = t == PL3D_FLUID_ ? uint32_t(clidx[C + offs[q]])
+                                 : (t == PL3D_AIR_ ? uint32_t(NB_AIR) : uint32_t(NB_SOLID));
+      }
+      cn[i] = nb[0] | (nb[1] << 16);
+      cn[i + cap] = nb[2] | (nb[3] << 16);
+      cn[i + 2 * cap] = nb[4] | (nb[5] << 16);
+      cdc[i] = uint8_t(d | (((I + J + K) & 1) << 7));
+      for (int q = 0; q < 8; q++) {
+        const int ii = 2 * I + (q & 1), jj = 2 * J + ((q >> 1) & 1), kk = 2 * K + (q >> 2);
+        cch[8 * i + q] = (ii < fx && jj < fy && kk < fz) ? flidx[ii + fx * (jj + fy * kk)] : -1;
+      }
+    }
+    int *fpar = reinterpret_cast<int *>(base + Y.parent[l - 1]);
+    for (int t = threadIdx.x; t < G.m[l - 1]; t += NT) {
+      const int c = flcell[t];
+      const int i = c % fx, j = (c / fx) % fy, k = c / (fx * fy);
+      fpar[t] = clidx[(i >> 1) + cx * ((j >> 1) + cy * (k >> 1))];
+    }
+    if (threadIdx.x == 0) {
+      G.m[l] = ml;
+      G.nl = l + 1;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+}
+
+// one red (c = 0) or black (c = 1) Gauss-Seidel half-sweep on coarse level l, then a barrier
+__device__ void mg_sweep(MgLds &G, unsigned char *base, int l, int c) {
+  const MgLayout &Y = G.lay;
+  const int ml = G.m[l], cap = Y.cap[l];
+  const uint32_t *cn = reinterpret_cast<const uint32_t *>(base + Y.nbr[l]);
+  const uint8_t *cdc = base + Y.dc[l];
+  const float *b = reinterpret_cast<const float *>(base + Y.b[l]);
+  float *X = G.X + mg_xoff(l);
+  for (int i = threadIdx.x; i < ml; i += NT) {
+    const int dc = cdc[i];
+    if ((dc >> 7) != c) continue;
+    const int d = dc & 7;
+    float acc = b[i];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const uint32_t w = cn[i + q * cap];
+      const int lo = int(w & 0xFFFFu), hi = int(w >> 16);
+      if (lo < int(NB_AIR)) acc += X[lo];
+      if (hi < int(NB_AIR)) acc += X[hi];
+    }
+    X[i] = d > 0 ? acc / float(d) : 0.f;
+  }
+  __syncthreads();
+}
+
+// res_l = b_l - A_l X_l, then a barrier
+__device__ void mg_residual(MgLds &G, unsigned char *base, int l) {
+  const MgLayout &Y = G.lay;
+  const int ml = G.m[l], cap = Y.cap[l];
+  const uint32_t *cn = reinterpret_cast<const uint32_t *>(base + Y.nbr[l]);
+  const uint8_t *cdc = base + Y.dc[l];
+  const float *b = reinterpret_cast<const float *>(base + Y.b[l]);
+  float *res = reinterpret_cast<float *>(base + Y.res[l]);
+  const float *X = G.X + mg_xoff(l);
+  for (int i = threadIdx.x; i < ml; i += NT) {
+    float ax = float(cdc[i] & 7) * X[i];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const uint32_t w = cn[i + q * cap];
+      const int lo = int(w & 0xFFFFu), hi = int(w >> 16);
+      if (lo < int(NB_AIR)) ax -= X[lo];
+      if (hi < int(NB_AIR)) ax -= X[hi];
+    }
+    res[i] = b[i] - ax;
+  }
+  __syncthreads();
+}
+
+// Levels 1 .. nl-1 of the V-cycle: on entry level 0's residual is in its res array; on exit
+// X of level 1 holds the coarse correction for level 0.
+__device__ __attribute__((noinline)) void mg_coarse(MgLds &G, unsigned char *base) {
+  const MgLayout &Y = G.lay;
+  const int nl = G.nl;
+  for (int l = 1; l < nl; l++) {
+    const int ml = G.m[l];
+    const float *fres = reinterpret_cast<const float *>(base + Y.res[l - 1]);
+    const int *ch = reinterpret_cast<const int *>(base + Y.child[l]);
+    float *b = reinterpret_cast<float *>(base + Y.b[l]);
+    float *X = G.X + mg_xoff(l);
+    for (int i = threadIdx.x; i < ml; i += NT) {
+      float acc = 0.f;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int c = ch[8 * i + q];
+        if (c >= 0) acc += fres[c];
+      }
+      b[i] = 0.5f * acc;
+      X[i] = 0.f;
+    }
+    __syncthreads();
+    if (l == nl - 1) {
+      for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
+        mg_sweep(G, base, l, 0);
+        mg_sweep(G, base, l, 1);
+      }
+      for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
+        mg_sweep(G, base, l, 1);
+        mg_sweep(G, base, l, 0);
+      }
+    } else {
+      for (int sw = 0; sw < 2; sw++) {
+        mg_sweep(G, base, l, 0);
+        mg_sweep(G, base, l, 1);
+      }
+      mg_residual(G, base, l);
+    }
+  }
+  for (int l = nl - 2; l >= 1; l--) {
+    const int ml = G.m[l];
+    const int *par = reinterpret_cast<const int *>(base + Y.parent[l]);
+    float *X = G.X + mg_xoff(l);
+    const float *Xc = G.X + mg_xoff(l + 1);
+    for (int i = threadIdx.x; i < ml; i += NT) {
+      const int p = par[i];
+      if (p >= 0) X[i] += Xc[p];
+    }
+    __syncthreads();
+    for (int sw = 0; sw < 2; sw++) {
+      mg_sweep(G, base, l, 1);
+      mg_sweep(G, base, l, 0);
+    }
+  }
+}
+
 // FOR_SLOTS: every slot k < ce of this thread (a uniform condition).  Slots past the last cell
 // (t >= m, `own` false) run the same branch-free code on padding: their loads use the clamped
 // index tc = min(t, m - 1), their register values are zero and their LDS writes land in unused
@@ -231,9 +495,10 @@ __device__ __attribute__((noinline)) double fold_and_residual(
     }                                                    \
   }
 
-template <bool MIC, int CPT>
-__device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const PcgDesc &D,
+template <int PC, int CPT>
+__device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, const PcgDesc &D,
                                          const int m) {
+  constexpr bool MIC = PC == PC_MIC;
   float *__restrict__ s_lds = L.s;
   float *__restrict__ W = L.W;
   Red &R = L.R;
@@ -299,8 +564,10 @@ __device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const 
   int *__restrict__ porder = D.porder;      // plane position p -> t
   float *__restrict__ pc2p = D.pc2;         // 1/e per plane position
   uint32_t *__restrict__ nbp = D.nbrPlane;  // 3 x m, plane order
-  int *planeStart = L.planeStart, *planeFill = L.planeFill;
+  int *planeStart = nullptr, *planeFill = nullptr;
   if constexpr (MIC) {
+    planeStart = L.x.planeStart;
+    planeFill = L.x.planeFill;
     nplanes = A.planes;
     for (int l = tid; l < nplanes; l += NT) planeFill[l] = 0;
     __syncthreads();
@@ -317,8 +584,8 @@ __device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const 
         acc += planeFill[l];
       }
       planeStart[nplanes] = acc;
-      L.planeRange[0] = lo;
-      L.planeRange[1] = hi;
+      L.x.planeRange[0] = lo;
+      L.x.planeRange[1] = hi;
     }
     __syncthreads();
     for (int l = tid; l < nplanes; l += NT) planeFill[l] = 0;
@@ -334,7 +601,7 @@ __device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const 
     __syncthreads();
     // factorisation sweep: W[t] holds 1/e_t (= pc_t^2) once plane(t) is done
     const float tau = 0.97f;
-    for (int l = L.planeRange[0]; l <= L.planeRange[1]; l++) {
+    for (int l = L.x.planeRange[0]; l <= L.x.planeRange[1]; l++) {
       const int p0 = planeStart[l], p1 = planeStart[l + 1];
       for (int p = p0 + tid; p < p1; p += NT) {
         const int t = porder[p];
@@ -356,13 +623,80 @@ __device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const 
     }
   }
 
+  // ---- multigrid: hierarchy, and the red-black colour of each owned slot (bit k of colmask)
+  uint32_t colmask = 0u;
+  unsigned char *mgbase = D.mg;
+  if constexpr (PC == PC_MG) {
+    mg_setup(L.x, A, mgbase, m);
+    FOR_SLOTS({
+      const int c = A.lcell[tc];
+      const int i = c % A.nx, j = (c / A.nx) % A.ny, kk = c / (A.nx * A.ny);
+      colmask |= uint32_t((i + j + kk) & 1) << k;
+    })
+  }
+  float *res0 = nullptr;
+  const int *par0 = nullptr;
+  if constexpr (PC == PC_MG) {
+    res0 = reinterpret_cast<float *>(mgbase + L.x.lay.res[0]);
+    par0 = reinterpret_cast<const int *>(mgbase + L.x.lay.parent[0]);
+  }
+
+  // level-0 red (c = 0) or black (c = 1) half-sweep in W, then a barrier
+  auto smooth0 = [&](uint32_t c) {
+    FOR_SLOTS({
+      if (((colmask >> k) & 1u) == c) {
+        const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
+        const float sum = ((W[px >> 16] + W[px & 0xFFFFu]) + (W[py >> 16] + W[py & 0xFFFFu])) +
+                          (W[pz >> 16] + W[pz & 0xFFFFu]);
+        W[t] = (r[k] + sum) * __builtin_amdgcn_rcpf(dg[k]);
+      }
+    })
+    lds_barrier();
+  };
+
   // z = M r.  MIC(0): owners write W = r, the sweeps run in place, W = z afterwards (ends with a
-  // barrier).  Jacobi: nothing to do, z is recomputed as r / diag where it is used.
+  // barrier).  MG: one V(2,2) cycle from zero, z = W (ends with a barrier).  Jacobi: nothing to
+  // do, z is recomputed as r / diag where it is used.
   auto precondition = [&]() {
-    if constexpr (MIC) {
+    if constexpr (PC == PC_MG) {
+      if (!L.x.ok) {  // hierarchy over capacity: Jacobi for this solve
+        FOR_SLOTS({ W[t] = r[k] * __builtin_amdgcn_rcpf(dg[k]); })
+        lds_barrier();
+        return;
+      }
+      // pre-smoothing: red from zero (neighbours 0), black, red, black
+      FOR_SLOTS({ W[t] = ((colmask >> k) & 1u) == 0u ? r[k] * __builtin_amdgcn_rcpf(dg[k]) : 0.f; })
+      lds_barrier();
+      smooth0(1u);
+      smooth0(0u);
+      smooth0(1u);
+      if (L.x.nl > 1) {
+        FOR_SLOTS({
+          const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
+          const float sum = ((W[px >> 16] + W[px & 0xFFFFu]) + (W[py >> 16] + W[py & 0xFFFFu])) +
+                            (W[pz >> 16] + W[pz & 0xFFFFu]);
+          if (own) res0[t] = r[k] - (dg[k] * W[t] - sum);
+        })
+        __syncthreads();  // global res0 and LDS
+        mg_coarse(L.x, mgbase);
+        const float *X1 = L.x.X;
+        FOR_SLOTS({
+          if (own) {
+            const int pa = par0[t];
+            if (pa >= 0) W[t] += X1[pa];
+          }
+        })
+        lds_barrier();
+      }
+      // post-smoothing in the reverse order: black, red, black, red
+      smooth0(1u);
+      smooth0(0u);
+      smooth0(1u);
+      smooth0(0u);
+    } else if constexpr (MIC) {
       FOR_SLOTS({ W[t] = r[k]; })
       __syncthreads();
-      const int lo = L.planeRange[0], hi = L.planeRange[1];
+      const int lo = L.x.planeRange[0], hi = L.x.planeRange[1];
       // forward: W_t = (1/e_t) (r_t + sum_{-nb} W_nb)
       for (int l = lo; l <= hi; l++) {
         for (int p = planeStart[l] + tid; p < planeStart[l + 1]; p += NT) {
@@ -394,7 +728,7 @@ __device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const 
     }
   };
   auto zval = [&](int k, int t) -> float {
-    if constexpr (MIC) {
+    if constexpr (MIC || PC == PC_MG) {
       return W[t];
     } else {
       return r[k] * __builtin_amdgcn_rcpf(dg[k]);
@@ -530,8 +864,8 @@ __device__ __forceinline__ void pcg_body(SolveLds &L, const PcgShared &A, const 
 
 // Common entry: capacity check, empty problem, then the body compiled for the smallest number of
 // register slots per thread that holds the liquid cells (fewer slots, fewer live registers).
-template <bool MIC>
-__device__ __forceinline__ void pcg_entry(SolveLds &L, const PcgShared *__restrict__ At,
+template <int PC>
+__device__ __forceinline__ void pcg_entry(SolveLds<PC> &L, const PcgShared *__restrict__ At,
                                           const PcgDesc *__restrict__ Ds) {
   const PcgShared A = At[blockIdx.y];
   if (A.active != nullptr && *A.active == 0) return;
@@ -560,11 +894,11 @@ __device__ __forceinline__ void pcg_entry(SolveLds &L, const PcgShared *__restri
   }
   const int ce = (m + NT - 1) / NT;
   if (ce <= 8)
-    pcg_body<MIC, 8>(L, A, D, m);
+    pcg_body<PC, 8>(L, A, D, m);
   else if (ce <= 12)
-    pcg_body<MIC, 12>(L, A, D, m);
+    pcg_body<PC, 12>(L, A, D, m);
   else
-    pcg_body<MIC, 16>(L, A, D, m);
+    pcg_body<PC, 16>(L, A, D, m);
 }
 
 }  // namespace
@@ -574,11 +908,16 @@ __device__ __forceinline__ void pcg_entry(SolveLds &L, const PcgShared *__restri
 // compile-time choice.
 __global__ __launch_bounds__(NT) void k_pcg_mic0(const PcgShared *__restrict__ At,
                                                  const PcgDesc *__restrict__ Ds) {
-  __shared__ SolveLds L;
-  pcg_entry<true>(L, At, Ds);
+  __shared__ SolveLds<PC_MIC> L;
+  pcg_entry<PC_MIC>(L, At, Ds);
+}
+__global__ __launch_bounds__(NT) void k_pcg_mg(const PcgShared *__restrict__ At,
+                                               const PcgDesc *__restrict__ Ds) {
+  __shared__ SolveLds<PC_MG> L;
+  pcg_entry<PC_MG>(L, At, Ds);
 }
 __global__ __launch_bounds__(NT) void k_pcg_jacobi(const PcgShared *__restrict__ At,
                                                    const PcgDesc *__restrict__ Ds) {
-  __shared__ SolveLds L;
-  pcg_entry<false>(L, At, Ds);
+  __shared__ SolveLds<PC_JAC> L;
+  pcg_entry<PC_JAC>(L, At, Ds);
 }

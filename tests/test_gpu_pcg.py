@@ -40,7 +40,7 @@ def random_domain(nx, ny, nz, seed, fluid=0.7):
     return cell, b
 
 
-@pytest.mark.parametrize("precond", ["mic0", "jacobi"])
+@pytest.mark.parametrize("precond", ["mic0", "jacobi", "mg"])
 def test_column_known_answers(precond):
     """tests/sim3d.test.ts:60-80 on the device."""
     n = 8
@@ -64,7 +64,7 @@ def test_random_domains_vs_oracle(seed, shape):
     cell, b = random_domain(nx, ny, nz, seed)
     tol = 1e-6
     qref, rref = O.pcg_solve(nx, ny, nz, cell, b, tol, 400)
-    for precond in ("mic0", "jacobi"):
+    for precond in ("mic0", "jacobi", "mg"):
         q = np.zeros(nx * ny * nz)
         r = Pcg3D(nx, ny, nz, precond).solve(cell, b, q, tol, 400)
         assert r["converged"], precond
@@ -86,7 +86,8 @@ def test_empty_and_capacity():
     assert r["iterations"] == 0 and r["converged"] and r["liquidCells"] == 0
 
 
-def test_pour_snapshot_vs_oracle():
+@pytest.mark.parametrize("precond", ["mic0", "mg"])
+def test_pour_snapshot_vs_oracle(precond):
     """(cell, rhs) of a 5 mm pour substep taken from the oracle, solved on the device."""
     s = O.OracleSim(O.gentle_pour(5, 20))
     for _ in range(30):
@@ -96,8 +97,9 @@ def test_pour_snapshot_vs_oracle():
     qref = s.grid_field("q")
     it_ref = s.last_solves()[0]
     q = np.zeros_like(b)
-    r = Pcg3D(s.nx, s.ny, s.nz, "mic0").solve(cell, b, q, 1e-6, 400)
+    r = Pcg3D(s.nx, s.ny, s.nz, precond).solve(cell, b, q, 1e-6, 400)
     assert r["converged"]
     assert r["trueResidual"] <= 1e-6 * np.abs(b).max()
-    assert abs(r["iterations"] - it_ref) <= 3
+    if precond == "mic0":
+        assert abs(r["iterations"] - it_ref) <= 3
     assert np.abs(q - qref).max() <= 1e-3 * np.abs(qref).max()
