@@ -375,7 +375,77 @@ __device__ __attribute__((noinline)) void mg_setup(MgLds &G, const PcgShared &A,
   __syncthreads();
 }
 
-// one red (c = 0) or black (c = 1) Gauss-Seidel half-sweep on coarse level l, then a barrier
+// Coarse-level cells owned by one thread (block mode: i = tid + q * NT) or one lane of wave 0
+// (wave mode: i = lane + q * 64), with their structure and right-hand side held in registers for
+// the whole V-cycle.  dc == 0 marks an unowned slot.
+#define MG_KW ((PL3D_MG_CAP3 + 63) / 64)
+template <int K>
+struct LevelRegs {
+  uint32_t nb[K][3];
+  int dc[K];
+  float b[K];
+};
+
+// structure of the owned cells and b = restriction of the finer level's residual; X = 0
+template <int K>
+__device__ __forceinline__ void lv_begin(LevelRegs<K> &R, const MgLayout &Y, unsigned char *base,
+                                         float *X, int l, int ml, int first, int stride) {
+  const uint32_t *cn = reinterpret_cast<const uint32_t *>(base + Y.nbr[l]);
+  const uint8_t *cdc = base + Y.dc[l];
+  const int *ch = reinterpret_cast<const int *>(base + Y.child[l]);
+  const float *fres = reinterpret_cast<const float *>(base + Y.res[l - 1]);
+  const int cap = Y.cap[l];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const int i = first + q * stride;
+    R.dc[q] = 0;
+    R.b[q] = 0.f;
+    if (i < ml) {
+      R.nb[q][0] = cn[i];
+      R.nb[q][1] = cn[i + cap];
+      R.nb[q][2] = cn[i + 2 * cap];
+      R.dc[q] = int(cdc[i]) | 0x100;  // bit 8: owned
+      float acc = 0.f;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const int f = ch[8 * i + c];
+        if (f >= 0) acc += fres[f];
+      }
+      R.b[q] = 0.5f * acc;
+      X[i] = 0.f;
+    }
+  }
+}
+
+// red (c = 0) or black (c = 1) half-sweep over the owned cells (no synchronisation)
+template <int K>
+__device__ __forceinline__ void lv_sweep(const LevelRegs<K> &R, float *X, int first, int stride,
+                                         int c) {
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const int dc = R.dc[q];
+    if (dc == 0 || ((dc >> 7) & 1) != c) continue;
+    float acc = R.b[q];
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+      const int lo = int(R.nb[q][w] & 0xFFFFu), hi = int(R.nb[q][w] >> 16);
+      if (lo < int(NB_AIR)) acc += X[lo];
+      if (hi < int(NB_AIR)) acc += X[hi];
+    }
+    const int d = dc & 7;
+    X[first + q * stride] = d > 0 ? acc / float(d) : 0.f;
+  }
+}
+
+// LDS ordering between the lanes of one wave (wave-mode sweeps need no workgroup barrier)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// one red (c = 0) or black (c = 1) Gauss-Seidel half-sweep on coarse level l by the whole block,
+// structure read from the level's arrays, then a barrier
 __device__ void mg_sweep(MgLds &G, unsigned char *base, int l, int c) {
   const MgLayout &Y = G.lay;
   const int ml = G.m[l], cap = Y.cap[l];
@@ -400,7 +470,7 @@ __device__ void mg_sweep(MgLds &G, unsigned char *base, int l, int c) {
   __syncthreads();
 }
 
-// res_l = b_l - A_l X_l, then a barrier
+// res_l = b_l - A_l X_l by the whole block, then a barrier
 __device__ void mg_residual(MgLds &G, unsigned char *base, int l) {
   const MgLayout &Y = G.lay;
   const int ml = G.m[l], cap = Y.cap[l];
@@ -424,16 +494,42 @@ __device__ void mg_residual(MgLds &G, unsigned char *base, int l) {
 }
 
 // Levels 1 .. nl-1 of the V-cycle: on entry level 0's residual is in its res array; on exit
-// X of level 1 holds the coarse correction for level 0.
+// X of level 1 holds the coarse correction for level 0.  A coarsest level of at most
+// 64 * MG_KW cells is smoothed by wave 0 alone, its structure in registers and without
+// workgroup barriers between half-sweeps.
 __device__ __attribute__((noinline)) void mg_coarse(MgLds &G, unsigned char *base) {
   const MgLayout &Y = G.lay;
-  const int nl = G.nl;
-  for (int l = 1; l < nl; l++) {
+  const int lc = G.nl - 1;  // coarsest level, >= 1
+  const bool wave = G.m[lc] <= 64 * MG_KW;
+  for (int l = 1; l <= lc; l++) {
     const int ml = G.m[l];
+    float *X = G.X + mg_xoff(l);
+    if (l == lc && wave) {
+      if (threadIdx.x < 64) {
+        LevelRegs<MG_KW> RW;
+        const int lane = threadIdx.x;
+        lv_begin(RW, Y, base, X, l, ml, lane, 64);
+        wave_lds_sync();
+        for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
+          lv_sweep(RW, X, lane, 64, 0);
+          wave_lds_sync();
+          lv_sweep(RW, X, lane, 64, 1);
+          wave_lds_sync();
+        }
+        for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
+          lv_sweep(RW, X, lane, 64, 1);
+          wave_lds_sync();
+          lv_sweep(RW, X, lane, 64, 0);
+          wave_lds_sync();
+        }
+      }
+      __syncthreads();
+      break;
+    }
+    // restriction: b = half the sum of the children's residuals, X = 0
     const float *fres = reinterpret_cast<const float *>(base + Y.res[l - 1]);
     const int *ch = reinterpret_cast<const int *>(base + Y.child[l]);
     float *b = reinterpret_cast<float *>(base + Y.b[l]);
-    float *X = G.X + mg_xoff(l);
     for (int i = threadIdx.x; i < ml; i += NT) {
       float acc = 0.f;
 #pragma unroll
@@ -445,7 +541,7 @@ __device__ __attribute__((noinline)) void mg_coarse(MgLds &G, unsigned char *bas
       X[i] = 0.f;
     }
     __syncthreads();
-    if (l == nl - 1) {
+    if (l == lc) {
       for (int sw = 0; sw < MG_COARSE_SWEEPS; sw++) {
         mg_sweep(G, base, l, 0);
         mg_sweep(G, base, l, 1);
@@ -462,7 +558,7 @@ __device__ __attribute__((noinline)) void mg_coarse(MgLds &G, unsigned char *bas
       mg_residual(G, base, l);
     }
   }
-  for (int l = nl - 2; l >= 1; l--) {
+  for (int l = lc - 1; l >= 1; l--) {
     const int ml = G.m[l];
     const int *par = reinterpret_cast<const int *>(base + Y.parent[l]);
     float *X = G.X + mg_xoff(l);

@@ -312,3 +312,59 @@ def test_apic_is_not_flip():
     assert np.abs(runs[0][1]).max() == 0
     assert np.abs(runs[1][1]).max() > 0
     assert np.abs(runs[0][0]["vy"] - runs[1][0]["vy"]).max() > 1e-6
+
+
+def _pour_snapshot(cell_mm=10, steps=40):
+    s = O.OracleSim(O.gentle_pour(cell_mm, 20))
+    for _ in range(steps):
+        s.step(s.stableDt())
+    return s.nx, s.ny, s.nz, s.grid_field("cell").astype(np.uint8), s.grid_field("rhs_p")
+
+
+def test_mg_vcycle_is_symmetric_positive_definite():
+    """CG needs a symmetric positive definite preconditioner (tests/multigrid.test.ts:21-47 checks
+    the 2D V-cycle the same way): <M r1, r2> = <r1, M r2> and <M r, r> > 0 on random vectors over
+    a pour's liquid cells.  A smoother run in the same colour order both ways, or a restriction
+    that is not a multiple of the prolongation's transpose, breaks the first."""
+    from oracle.mg3d import Multigrid3D
+    nx, ny, nz, cell, _ = _pour_snapshot()
+    mg = Multigrid3D(nx, ny, nz, cell)
+    assert len(mg.levels) >= 3
+    rng = np.random.default_rng(5)
+    m = mg.levels[0]["L"].size
+    for _ in range(3):
+        r1, r2 = rng.standard_normal(m), rng.standard_normal(m)
+        a, b = mg.vcycle(r1) @ r2, r1 @ mg.vcycle(r2)
+        assert abs(a - b) <= 1e-10 * (abs(a) + abs(b))
+        assert mg.vcycle(r1) @ r1 > 0
+
+
+def test_mg_coarsening_rule_known_answer():
+    """multigrid.ts:62-79 in 3D on a hand-made 6x6x6 box: a coarse cell with an AIR child is AIR,
+    one with FLUID and SOLID children only is FLUID, the coarse border is SOLID."""
+    from oracle.mg3d import coarsen
+    n = 6
+    cell = np.full((n, n, n), 2, np.uint8)
+    cell[1:5, 1:5, 1:5] = 1
+    cell[3, 3, 3] = 0  # one air cell in the fine block of coarse cell (1,1,1)
+    cx, cy, cz, c = coarsen(n, n, n, cell.ravel())
+    assert (cx, cy, cz) == (3, 3, 3)
+    c = c.reshape(3, 3, 3)
+    assert c[1, 1, 1] == 0  # its children include the AIR cell (3,3,3)
+    assert (c[0] == 2).all() and (c[:, 0] == 2).all() and (c[:, :, 0] == 2).all()
+    cell[3, 3, 3] = 1
+    _, _, _, c2 = coarsen(n, n, n, cell.ravel())
+    assert c2.reshape(3, 3, 3)[1, 1, 1] == 1
+
+
+def test_mg_pcg_converges_and_beats_jacobi():
+    """MG-PCG meets the stop rule in far fewer iterations than Jacobi PCG on a pour snapshot, and
+    its solution satisfies A q = b to the tolerance (closed check with the oracle's operator)."""
+    from oracle.mg3d import mg_pcg
+    nx, ny, nz, cell, b = _pour_snapshot()
+    q, it, ok = mg_pcg(nx, ny, nz, cell, b, 1e-6, 400)
+    assert ok and it < 40
+    aq = O.apply_laplacian(nx, ny, nz, cell, q)
+    assert np.abs(b - aq)[cell == 1].max() <= 1e-6 * np.abs(b[cell == 1]).max()
+    _, rref = O.pcg_solve(nx, ny, nz, cell, b, 1e-6, 400)
+    assert it <= rref.iterations + 2  # no worse than the reference's MIC(0)
