@@ -31,6 +31,11 @@ HBM_PEAK_GBS = 8000.0
 # updates and max|r| 24 + forward sweep 13 + backward sweep and z.r 17 + s update 12; Jacobi drops
 # the sweeps and reads r, diag and writes z (9).
 BYTES_PER_CELL_ITER = {"mic0": 75.0, "jacobi": 54.0}
+# HBM bytes per k_pcg_jacobi launch on the default workload (4 mm, APIC, Jacobi-PCG), measured
+# with rocprofv3 FETCH_SIZE and WRITE_SIZE and calibrated on the library's copy kernel (FETCH x
+# 2.0, WRITE x 1.0): profiles/pour4mm_apic_jacobi_hbm_traffic.md.  The solve keeps its vectors in
+# LDS and registers, so this is far below the algorithmic bytes.
+MEASURED_TRAFFIC = {("apic", "jacobi", 4.0): 1.433e6}
 # Published reference figure for this metric: the reference's WebGPU FP32 4 mm pour, 0.87x real
 # time on an RTX A2000 Laptop (benchmarks/gpu3d.json:97-137).
 REFERENCE_WEBGPU_4MM = 0.87
@@ -257,8 +262,11 @@ def main():
             unit_note = "no byte model for this kernel"
         if bytes_per_launch is not None:
             achieved = bytes_per_launch / avg_s / 1e9
+            traffic = MEASURED_TRAFFIC.get((a.transfer, a.precond, a.cell))
             roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                        "traffic_source": "profiles/pour4mm_apic_jacobi_hbm_traffic.md"
+                        if traffic else None,
                         "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch":
                         bytes_per_launch, "bytes_model": unit_note}
 
