@@ -608,11 +608,12 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   const int ce = (m + NT - 1) / NT;
   const int tb = ((tid >> 6) * ce) * 64 + (tid & 63);
 
-  // registers per slot: r, x_lo, the diagonal, and the six neighbour slots as two 16-bit LDS
-  // indices per word (missing neighbours -> ZSLOT).  The words are made opaque at the top of each
-  // iteration so the compiler cannot hoist their decoding out of the loop (that would keep six
-  // addresses per slot live).
-  float r[CPT], xlo[CPT], dg[CPT];
+  // registers per slot: r, x_lo, and the six neighbour slots as two 14-bit LDS indices per word
+  // (bits 0-13 and 16-29; missing neighbours -> ZSLOT).  The diagonal (0..6) rides in the spare
+  // bits 14-15 of word 0 and bit 14 of word 1, which saves a register per slot.  The words are
+  // made opaque at the top of each iteration so the compiler cannot hoist their decoding out of
+  // the loop (that would keep six addresses per slot live).
+  float r[CPT], xlo[CPT];
   uint32_t nbc[CPT][3];
   double bmax = 0.0;
   auto local_word = [](uint32_t w) -> uint32_t {
@@ -620,24 +621,31 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     return (lo < NB_AIR ? lo : uint32_t(ZSLOT)) | ((hi < NB_AIR ? hi : uint32_t(ZSLOT)) << 16);
   };
   constexpr uint32_t Z2 = uint32_t(ZSLOT) | (uint32_t(ZSLOT) << 16);
+  // padding slots: diagonal 6 (= 2 | 1 << 2), no coupling, r = 0
+  constexpr uint32_t D6W0 = 2u << 14, D6W1 = 1u << 14;
 #pragma unroll
   for (int k = 0; k < CPT; k++) {
     r[k] = xlo[k] = 0.f;
-    dg[k] = 6.f;
-    nbc[k][0] = nbc[k][1] = nbc[k][2] = Z2;
+    nbc[k][0] = Z2 | D6W0;
+    nbc[k][1] = Z2 | D6W1;
+    nbc[k][2] = Z2;
   }
   FOR_SLOTS({
     const uint32_t wx = nbx[tc], wy = nby[tc], wz = nbz[tc];
     const double bl = b[tc];
-    dg[k] = own ? diag_of(wx, wy, wz) : 6.f;  // padding: 6, no coupling, r = 0
-    nbc[k][0] = own ? local_word(wx) : Z2;
-    nbc[k][1] = own ? local_word(wy) : Z2;
+    const uint32_t d = own ? uint32_t(diag_of(wx, wy, wz)) : 6u;
+    nbc[k][0] = (own ? local_word(wx) : Z2) | ((d & 3u) << 14);
+    nbc[k][1] = (own ? local_word(wy) : Z2) | ((d >> 2) << 14);
     nbc[k][2] = own ? local_word(wz) : Z2;
     const double bt = own ? bl : 0.0;
     r[k] = float(bt);
     bmax = fmax(bmax, fabs(bt));
     if (own) xhi[t] = 0.0;
   })
+  // the diagonal of slot k, and the two LDS indices of a packed word
+  auto dgv = [&](int k) -> float {
+    return float(((nbc[k][0] >> 14) & 3u) | (((nbc[k][1] >> 14) & 1u) << 2));
+  };
   if (tid == 0) {
     s_lds[ZSLOT] = 0.f;
     W[ZSLOT] = 0.f;
@@ -742,9 +750,9 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     FOR_SLOTS({
       if (((colmask >> k) & 1u) == c) {
         const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
-        const float sum = ((W[px >> 16] + W[px & 0xFFFFu]) + (W[py >> 16] + W[py & 0xFFFFu])) +
-                          (W[pz >> 16] + W[pz & 0xFFFFu]);
-        W[t] = (r[k] + sum) * __builtin_amdgcn_rcpf(dg[k]);
+        const float sum = ((W[px >> 16] + W[px & 0x3FFFu]) + (W[py >> 16] + W[py & 0x3FFFu])) +
+                          (W[pz >> 16] + W[pz & 0x3FFFu]);
+        W[t] = (r[k] + sum) * __builtin_amdgcn_rcpf(dgv(k));
       }
     })
     lds_barrier();
@@ -756,12 +764,12 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   auto precondition = [&]() {
     if constexpr (PC == PC_MG) {
       if (!L.x.ok) {  // hierarchy over capacity: Jacobi for this solve
-        FOR_SLOTS({ W[t] = r[k] * __builtin_amdgcn_rcpf(dg[k]); })
+        FOR_SLOTS({ W[t] = r[k] * __builtin_amdgcn_rcpf(dgv(k)); })
         lds_barrier();
         return;
       }
       // pre-smoothing: red from zero (neighbours 0), black, red, black
-      FOR_SLOTS({ W[t] = ((colmask >> k) & 1u) == 0u ? r[k] * __builtin_amdgcn_rcpf(dg[k]) : 0.f; })
+      FOR_SLOTS({ W[t] = ((colmask >> k) & 1u) == 0u ? r[k] * __builtin_amdgcn_rcpf(dgv(k)) : 0.f; })
       lds_barrier();
       smooth0(1u);
       smooth0(0u);
@@ -769,9 +777,9 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
       if (L.x.nl > 1) {
         FOR_SLOTS({
           const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
-          const float sum = ((W[px >> 16] + W[px & 0xFFFFu]) + (W[py >> 16] + W[py & 0xFFFFu])) +
-                            (W[pz >> 16] + W[pz & 0xFFFFu]);
-          if (own) res0[t] = r[k] - (dg[k] * W[t] - sum);
+          const float sum = ((W[px >> 16] + W[px & 0x3FFFu]) + (W[py >> 16] + W[py & 0x3FFFu])) +
+                            (W[pz >> 16] + W[pz & 0x3FFFu]);
+          if (own) res0[t] = r[k] - (dgv(k) * W[t] - sum);
         })
         __syncthreads();  // global res0 and LDS
         mg_coarse(L.x, mgbase);
@@ -827,7 +835,7 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     if constexpr (MIC || PC == PC_MG) {
       return W[t];
     } else {
-      return r[k] * __builtin_amdgcn_rcpf(dg[k]);
+      return r[k] * __builtin_amdgcn_rcpf(dgv(k));
     }
   };
 
@@ -872,9 +880,9 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     FOR_SLOTS({  // A s = diag s_t - sum of the liquid neighbours (missing ones read the zero slot)
       const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
       const float st = s_lds[t];
-      const float acc = dg[k] * st - (((s_lds[px >> 16] + s_lds[px & 0xFFFFu]) +
-                                       (s_lds[py >> 16] + s_lds[py & 0xFFFFu])) +
-                                      (s_lds[pz >> 16] + s_lds[pz & 0xFFFFu]));
+      const float acc = dgv(k) * st - (((s_lds[px >> 16] + s_lds[px & 0x3FFFu]) +
+                                       (s_lds[py >> 16] + s_lds[py & 0x3FFFu])) +
+                                      (s_lds[pz >> 16] + s_lds[pz & 0x3FFFu]));
       W[t] = acc;  // A s, read back by the owner only
       sas_part += double(st) * double(acc);
     })
