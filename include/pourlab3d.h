@@ -278,6 +278,39 @@ int pl3d_ensemble_advance(pl3d_ensemble *ens, double seconds, const int32_t *mas
 /* finished per member: out receives `count` flags. */
 int pl3d_ensemble_finished(pl3d_ensemble *ens, int32_t *out);
 
+/* ---- the standalone Poisson problem set ------------------------------------------------------ */
+
+/* `problems` pressure-Poisson problems on n^3 grids (the pour's operator: 7-point Laplacian on FLUID
+ * cells, AIR = p 0 ghost, SOLID = Neumann, solver3d.ts:872-881; rhs = minus the MAC divergence of a
+ * face velocity field, :473-493), generated on the device from seeds seed0 + k: a tilted, bumpy
+ * free surface inside a 2-cell cylinder wall, and a box-filtered Gaussian velocity field.  Build-
+ * only: the reference has no such set.  Solved together by a batched Jacobi-PCG. */
+typedef struct pl3d_poisson pl3d_poisson;
+#define PL3D_POISSON_FP32 0    /* FP32 vectors, stop on the recursive residual */
+#define PL3D_POISSON_REFINED 1 /* FP32 vectors + FP64 solution and true-residual refinement */
+#define PL3D_POISSON_FP64 2    /* FP64 vectors: the reference solve */
+typedef struct pl3d_poisson_result {
+  int32_t iterations;
+  int32_t converged;
+  double residual;      /* max |r| of the solver's own residual at exit */
+  double true_residual; /* max |b - A x| recomputed in FP64 */
+  double bnorm;         /* max |b| */
+  double error_vs_fp64; /* max |x - x_fp64| / max |x_fp64|, or -1 before an FP64 solve */
+  int64_t liquid_cells;
+  double kernel_ms; /* device time of the whole batched solve (all problems) */
+} pl3d_poisson_result;
+int pl3d_poisson_create(int32_t n, int32_t problems, uint32_t seed0, pl3d_poisson **out);
+int pl3d_poisson_destroy(pl3d_poisson *ps);
+/* flags (n^3 bytes), rhs (n^3 floats) and FLUID count of problem k; any pointer may be NULL */
+int pl3d_poisson_problem(pl3d_poisson *ps, int32_t k, uint8_t *flags, float *rhs,
+                         int64_t *liquid_cells);
+/* Solve every problem to max|r| <= max(tol max|b|, 1e-14) (solver3d.ts:910) at one precision;
+ * out receives `problems` results.  The solution is kept per precision. */
+int pl3d_poisson_solve(pl3d_poisson *ps, int32_t precision, double tol, int32_t max_iter,
+                       pl3d_poisson_result *out);
+/* the kept solution of problem k at a precision, n^3 doubles (0 off the liquid) */
+int pl3d_poisson_solution(pl3d_poisson *ps, int32_t precision, int32_t k, double *x);
+
 /* ---- standalone pressure solve (Pcg3D) ------------------------------------------------------ */
 
 /* Pcg3D(nx, ny, nz).solve(cell, b, q, tol, maxIter) (solver3d.ts:858-937): solves the 7-point

@@ -13,6 +13,7 @@ from .sim3d import (  # noqa: F401
     SOLID,
     SPILLED,
     Pcg3D,
+    PoissonSet,
     Simulation3D,
     gentlePour3D,
     variedPour3D,

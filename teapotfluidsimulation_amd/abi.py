@@ -138,6 +138,22 @@ class SolveResult(C.Structure):
     ]
 
 
+class PoissonResult(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("residual", C.c_double),
+        ("true_residual", C.c_double),
+        ("bnorm", C.c_double),
+        ("error_vs_fp64", C.c_double),
+        ("liquid_cells", C.c_int64),
+        ("kernel_ms", C.c_double),
+    ]
+
+
+POISSON_PRECISIONS = {"fp32": 0, "refined": 1, "fp64": 2}
+
+
 def diagnostics_dict(d: Diagnostics) -> dict:
     """Diagnostics3D as the reference returns it (solver3d.ts:796-814), camelCase keys."""
     return {

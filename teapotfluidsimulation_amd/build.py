@@ -22,7 +22,8 @@ NODE_ADDON = os.path.join(NODE_DIR, "pourlab3d.node")
 NODE_INCLUDES = ["/usr/include/node", "/usr/local/include/node"]
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
-SOURCES = ["pl3d_solver.hip", "pl3d_step.hip", "pl3d_api.cpp", "pl3d_pcg_api.cpp"]
+SOURCES = ["pl3d_solver.hip", "pl3d_step.hip", "pl3d_api.cpp", "pl3d_pcg_api.cpp",
+           "pl3d_poisson.hip"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

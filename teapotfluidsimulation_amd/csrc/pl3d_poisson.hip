@@ -1,0 +1,855 @@
+// pl3d_poisson.hip — the standalone pressure-Poisson problem set: P seeded free-surface problems on
+// n^3 grids, generated on the device and solved together by a batched PCG whose kernels stream every
+// problem per launch (the grid's y dimension is the problem).
+//
+// Operator: the pour's pressure system, 7-point Laplacian on FLUID cells, AIR = Dirichlet 0 ghost,
+// SOLID = Neumann (src/sim3d/solver3d.ts:872-881, :939-953); rhs = minus the MAC divergence of a
+// face velocity field in cell units (solver3d.ts:473-493).  The set itself is build-only.
+//
+// Generator, problem k (seed s = seed0 + k), y vertical:
+//   surface  h(x, z) = (0.3 + 0.3 u0) n + tx (x - n/2) + tz (z - n/2)
+//                      + sum_b a_b exp(-((x - cx_b)^2 + (z - cz_b)^2) / (2 * 36)), its parameters
+//            drawn on the host from mulberry32(s) (rng.ts:5-14)
+//   flags    SOLID on the box border and at radius >= n/2 - 3 from the vertical axis (a 2-cell
+//            wall of outer radius n/2 - 1, and outside it); FLUID below h; else AIR
+//   velocity N(0, 1) per face from a counter-based hash (mulberry32 step of the state
+//            s * 0x9E3779B1 + index * 0x85EBCA77), Box-Muller, then a separable 3x3x3 box filter
+//            per component with clamped indices; all in FP64, rhs stored as FP32
+//
+// Solver: Jacobi-preconditioned CG, one lockstep iteration for all problems = five launches
+// (stencil product + dot partials, per-problem reduction for alpha, vector update + dot and max
+// partials, reduction for beta and the stop test, direction update).  Dot products are FP64
+// partial sums reduced in a fixed order (deterministic).  Precisions:
+//   FP32     FP32 vectors, stop on the recursive residual
+//   REFINED  FP32 vectors holding increments of an FP64 solution; every 25 iterations and at the
+//            stop test the increments are folded into the FP64 solution and r = b - A x is
+//            recomputed in FP64 (reliable update, restart when the residuals differ by > 1 %);
+//            stop on the FP64 true residual
+//   FP64     FP64 vectors (the reference solve)
+// Stop rule: max|r| <= max(tol * max|b|, 1e-14) per problem (solver3d.ts:910, :923-925).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pourlab3d.h"
+
+int pl3d_set_error(int code, const std::string &msg);
+
+namespace {
+
+constexpr uint8_t AIR = 0, FLUID = 1, SOLID = 2;
+constexpr int PS_THREADS = 256;
+constexpr int PS_CPT = 4;  // cells per thread
+constexpr int PS_CPB = PS_THREADS * PS_CPT;
+constexpr int PS_REFRESH = 25;
+constexpr int PS_MAX_BUMPS = 8;
+
+int perr(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  return pl3d_set_error(code, buf);
+}
+
+#define PS_HIP(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return perr(PL3D_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct Surface {  // per problem, drawn on the host
+  double H, tx, tz;
+  int nb, pad;
+  double cx[PS_MAX_BUMPS], cz[PS_MAX_BUMPS], a[PS_MAX_BUMPS];
+};
+
+struct Scal {  // per-problem solver scalars, device resident
+  double sigma, alpha, bmax, target, rmax, gap, trueRes;
+  int iters, active, converged, checkTrue, restart, pad;
+};
+
+uint32_t mulberry_next(uint32_t &a) {  // rng.ts:5-14
+  a += 0x6d2b79f5u;
+  uint32_t t = a;
+  t = (t ^ (t >> 15)) * (t | 1u);
+  t ^= t + (t ^ (t >> 7)) * (t | 61u);
+  return t ^ (t >> 14);
+}
+
+Surface draw_surface(int n, uint32_t seed) {
+  uint32_t a = seed;
+  auto u = [&]() { return double(mulberry_next(a)) / 4294967296.0; };
+  Surface S{};
+  S.H = (0.3 + 0.3 * u()) * n;
+  S.tx = 0.2 * (u() - 0.5);
+  S.tz = 0.2 * (u() - 0.5);
+  S.nb = 4 + int(std::floor(5 * u()));
+  for (int b = 0; b < S.nb; b++) {
+    S.cx[b] = u() * n;
+    S.cz[b] = u() * n;
+    S.a[b] = 8.0 * (2.0 * u() - 1.0);
+  }
+  return S;
+}
+
+// ------------------------------------------------------------------------------- generator kernels
+
+__device__ __forceinline__ double hash_uniform(uint32_t seed, uint64_t idx) {
+  uint32_t a = uint32_t(uint64_t(seed) * 0x9E3779B1ull + idx * 0x85EBCA77ull);
+  a += 0x6d2b79f5u;
+  uint32_t t = a;
+  t = (t ^ (t >> 15)) * (t | 1u);
+  t ^= t + (t ^ (t >> 7)) * (t | 61u);
+  return double(t ^ (t >> 14)) / 4294967296.0;
+}
+
+// flags of one problem: one thread per cell
+__global__ void k_ps_flags(uint8_t *flags, const Surface *surf, int n) {
+  const int64_t N = int64_t(n) * n * n;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const Surface &S = *surf;
+  const int i = int(c % n), j = int((c / n) % n), k = int(c / (int64_t(n) * n));
+  uint8_t f;
+  const double x = i + 0.5, y = j + 0.5, z = k + 0.5, half = 0.5 * n;
+  const double r = sqrt((x - half) * (x - half) + (z - half) * (z - half));
+  if (i == 0 || j == 0 || k == 0 || i == n - 1 || j == n - 1 || k == n - 1 || r >= half - 3) {
+    f = SOLID;
+  } else {
+    double h = S.H + S.tx * (x - half) + S.tz * (z - half);
+    for (int b = 0; b < S.nb; b++)
+      h += S.a[b] * exp(-((x - S.cx[b]) * (x - S.cx[b]) + (z - S.cz[b]) * (z - S.cz[b])) / 72.0);
+    f = y < h ? FLUID : AIR;
+  }
+  flags[c] = f;
+}
+
+// raw N(0,1) on every face of one problem (u, v, w faces concatenated)
+__global__ void k_ps_normals(double *raw, uint32_t seed, int64_t count) {
+  const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= count) return;
+  const double u1 = hash_uniform(seed, 2 * uint64_t(f));
+  const double u2 = hash_uniform(seed, 2 * uint64_t(f) + 1);
+  raw[f] = sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * M_PI * u2);
+}
+
+// one pass of the separable box filter along axis `ax` of one component grid (gx, gy, gz)
+__global__ void k_ps_box(const double *in, double *out, int gx, int gy, int gz, int ax) {
+  const int64_t cnt = int64_t(gx) * gy * gz;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= cnt) return;
+  const int i = int(c % gx), j = int((c / gx) % gy), k = int(c / (int64_t(gx) * gy));
+  const int g[3] = {gx, gy, gz};
+  const int64_t stride[3] = {1, gx, int64_t(gx) * gy};
+  const int pos = ax == 0 ? i : (ax == 1 ? j : k);
+  const int lo = max(pos - 1, 0), hi = min(pos + 1, g[ax] - 1);
+  const double a = in[c + (lo - pos) * stride[ax]], b = in[c], d = in[c + (hi - pos) * stride[ax]];
+  out[c] = (a + b + d) * (1.0 / 3.0);
+}
+
+// rhs of one problem from the filtered faces: -(du + dv + dw) on FLUID cells, FP32
+__global__ void k_ps_rhs(const double *u, const double *v, const double *w, const uint8_t *flags,
+                         float *b, int n) {
+  const int64_t N = int64_t(n) * n * n;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  if (flags[c] != FLUID) {
+    b[c] = 0.f;
+    return;
+  }
+  const int i = int(c % n), j = int((c / n) % n), k = int(c / (int64_t(n) * n));
+  const int64_t iu = i + int64_t(n + 1) * (j + int64_t(n) * k);
+  const int64_t iv = i + int64_t(n) * (j + int64_t(n + 1) * k);
+  const int64_t iw = i + int64_t(n) * (j + int64_t(n) * k);
+  const double div = (u[iu + 1] - u[iu]) + (v[iv + n] - v[iv]) + (w[iw + int64_t(n) * n] - w[iw]);
+  b[c] = float(-div);
+}
+
+// ------------------------------------------------------------------------------- solver kernels
+
+// non-solid neighbour count and the six neighbour offsets of a FLUID cell (never on the border)
+__device__ __forceinline__ int ps_diag(const uint8_t *fl, int64_t c, const int64_t off[6]) {
+  int d = 0;
+#pragma unroll
+  for (int q = 0; q < 6; q++) d += fl[c + off[q]] != SOLID ? 1 : 0;
+  return d;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < PS_THREADS / 64; w++) t += sh[w];
+  return t;  // valid in thread 0
+}
+
+__device__ __forceinline__ double block_max_d(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < PS_THREADS / 64; w++) t = fmax(t, sh[w]);
+  return t;
+}
+
+#define PS_CELLS(...)                                                              \
+  for (int q = 0; q < PS_CPT; q++) {                                               \
+    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x; \
+    if (c < N && fl[c] == FLUID) {                                                 \
+      __VA_ARGS__                                                                  \
+    }                                                                              \
+  }
+
+// As = A s, partial s.As per block
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_spmv(const uint8_t *flags, const T *sv, T *Asv,
+                                                        double *partial, const Scal *sc, int n,
+                                                        int bpp) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint8_t *fl = flags + base;
+  const T *s = sv + base;
+  T *As = Asv + base;
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  double acc = 0.0;
+  PS_CELLS({
+    T sum = T(0);
+    int d = 0;
+    _Pragma("unroll") for (int o = 0; o < 6; o++) {
+      const uint8_t f = fl[c + off[o]];
+      d += f != SOLID ? 1 : 0;
+      if (f == FLUID) sum += s[c + off[o]];
+    }
+    const T v = T(d) * s[c] - sum;
+    As[c] = v;
+    acc += double(s[c]) * double(v);
+  })
+  const double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) partial[int64_t(p) * bpp + blockIdx.x] = t;
+}
+
+// per problem: alpha = sigma / s.As (one block per problem, fixed-order sum)
+__global__ __launch_bounds__(PS_THREADS) void k_ps_alpha(const double *partial, Scal *sc, int bpp) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.x;
+  if (!sc[p].active) return;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < bpp; i += PS_THREADS) acc += partial[int64_t(p) * bpp + i];
+  const double sas = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) {
+    if (!(sas > 0.0)) {  // breakdown (solver3d.ts:916)
+      sc[p].active = 0;
+    } else {
+      sc[p].alpha = sc[p].sigma / sas;
+    }
+  }
+}
+
+// x += alpha s, r -= alpha As; partial z.r (z = r / diag) and max|r|
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_update(const uint8_t *flags, T *xv, T *rv,
+                                                          const T *sv, const T *Asv,
+                                                          double *partial, double *partial2,
+                                                          const Scal *sc, int n, int bpp) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint8_t *fl = flags + base;
+  T *x = xv + base, *r = rv + base;
+  const T *s = sv + base, *As = Asv + base;
+  const T alpha = T(sc[p].alpha);
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  double acc = 0.0, mx = 0.0;
+  PS_CELLS({
+    x[c] += alpha * s[c];
+    const T rr = r[c] - alpha * As[c];
+    r[c] = rr;
+    const T z = rr / T(ps_diag(fl, c, off));
+    acc += double(z) * double(rr);
+    mx = fmax(mx, fabs(double(rr)));
+  })
+  const double t = block_sum_d(acc, sh);
+  __syncthreads();
+  const double m = block_max_d(mx, sh);
+  if (threadIdx.x == 0) {
+    partial[int64_t(p) * bpp + blockIdx.x] = t;
+    partial2[int64_t(p) * bpp + blockIdx.x] = m;
+  }
+}
+
+// per problem: sigma' and max|r|; stop test; beta -> alpha slot (alpha is consumed already)
+__global__ __launch_bounds__(PS_THREADS) void k_ps_beta(const double *partial,
+                                                        const double *partial2, Scal *sc, int bpp,
+                                                        int refined) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.x;
+  if (!sc[p].active) return;
+  double acc = 0.0, mx = 0.0;
+  for (int i = threadIdx.x; i < bpp; i += PS_THREADS) {
+    acc += partial[int64_t(p) * bpp + i];
+    mx = fmax(mx, partial2[int64_t(p) * bpp + i]);
+  }
+  const double sn = block_sum_d(acc, sh);
+  __syncthreads();
+  const double rmax = block_max_d(mx, sh);
+  if (threadIdx.x == 0) {
+    Scal &S = sc[p];
+    S.iters++;
+    S.rmax = rmax;
+    S.alpha = sn / S.sigma;  // beta for k_ps_dir
+    S.sigma = sn;
+    if (rmax <= S.target) {
+      if (refined) {
+        S.checkTrue = 1;  // confirm on the FP64 true residual
+      } else {
+        S.converged = 1;
+        S.active = 0;
+      }
+    }
+  }
+}
+
+// Direction update.  mode 0: s = z + beta s (beta in the alpha slot after k_ps_beta); mode 1:
+// s = z for the problems flagged to restart; mode 2: s = z for every active problem (start; the
+// old s is not read, it may hold anything).  Inactive problems are untouched.
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_dir(const uint8_t *flags, const T *rv, T *sv,
+                                                       const Scal *sc, int n, int mode) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  if (mode == 1 && !sc[p].restart) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint8_t *fl = flags + base;
+  const T *r = rv + base;
+  T *s = sv + base;
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  if (mode == 0) {
+    const T beta = T(sc[p].alpha);
+    PS_CELLS({ s[c] = r[c] / T(ps_diag(fl, c, off)) + beta * s[c]; })
+  } else {
+    PS_CELLS({ s[c] = r[c] / T(ps_diag(fl, c, off)); })
+  }
+}
+
+// REFINED: x_hi += x_lo, x_lo = 0, r = b - A x_hi in FP64 (stored FP32), partial max|r| and
+// max|r - r_recursive|
+__global__ __launch_bounds__(PS_THREADS) void k_ps_refresh(const uint8_t *flags, const float *bv,
+                                                           float *xlo, double *xhi, float *rv,
+                                                           double *partial, double *partial2,
+                                                           const Scal *sc, int n, int bpp,
+                                                           int fold) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint8_t *fl = flags + base;
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  double *xh = xhi + base;
+  float *xl = xlo + base;
+  if (fold) {
+    PS_CELLS({
+      xh[c] += double(xl[c]);
+      xl[c] = 0.f;
+    })
+    return;
+  }
+  float *r = rv + base;
+  const float *b = bv + base;
+  double mx = 0.0, gap = 0.0;
+  PS_CELLS({
+    double sum = 0.0;
+    int d = 0;
+    _Pragma("unroll") for (int o = 0; o < 6; o++) {
+      const uint8_t f = fl[c + off[o]];
+      d += f != SOLID ? 1 : 0;
+      if (f == FLUID) sum += xh[c + off[o]];
+    }
+    const double rt = double(b[c]) - (double(d) * xh[c] - sum);
+    gap = fmax(gap, fabs(rt - double(r[c])));
+    r[c] = float(rt);
+    mx = fmax(mx, fabs(rt));
+  })
+  const double m = block_max_d(mx, sh);
+  __syncthreads();
+  const double g = block_max_d(gap, sh);
+  if (threadIdx.x == 0) {
+    partial[int64_t(p) * bpp + blockIdx.x] = m;
+    partial2[int64_t(p) * bpp + blockIdx.x] = g;
+  }
+}
+
+// REFINED: per problem verdict of a refresh; restart flag when the residuals drifted apart
+__global__ __launch_bounds__(PS_THREADS) void k_ps_refresh_verdict(const double *partial,
+                                                                   const double *partial2,
+                                                                   Scal *sc, int bpp) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.x;
+  if (!sc[p].active) return;
+  double mx = 0.0, gp = 0.0;
+  for (int i = threadIdx.x; i < bpp; i += PS_THREADS) {
+    mx = fmax(mx, partial[int64_t(p) * bpp + i]);
+    gp = fmax(gp, partial2[int64_t(p) * bpp + i]);
+  }
+  const double rt = block_max_d(mx, sh);
+  __syncthreads();
+  const double g = block_max_d(gp, sh);
+  if (threadIdx.x == 0) {
+    Scal &S = sc[p];
+    S.trueRes = rt;
+    S.gap = g;
+    S.checkTrue = 0;
+    S.restart = g > 0.01 * S.rmax ? 1 : 0;
+    if (rt <= S.target) {
+      S.converged = 1;
+      S.active = 0;
+    }
+  }
+}
+
+// sigma = z.r after a restart (only problems flagged for it)
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_zr(const uint8_t *flags, const T *rv,
+                                                      double *partial, const Scal *sc, int n,
+                                                      int bpp, int onlyRestart) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.y;
+  if (!sc[p].active || (onlyRestart && !sc[p].restart)) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint8_t *fl = flags + base;
+  const T *r = rv + base;
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  double acc = 0.0;
+  PS_CELLS({ acc += double(r[c]) * double(r[c]) / double(ps_diag(fl, c, off)); })
+  const double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) partial[int64_t(p) * bpp + blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(PS_THREADS) void k_ps_sigma(const double *partial, Scal *sc, int bpp,
+                                                         int onlyRestart) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.x;
+  if (!sc[p].active || (onlyRestart && !sc[p].restart)) return;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < bpp; i += PS_THREADS) acc += partial[int64_t(p) * bpp + i];
+  const double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) {
+    sc[p].sigma = t;
+    sc[p].restart = 0;
+  }
+}
+
+// start: r = b, x = 0 (and x_hi = 0), per-block max|b|
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_init(const uint8_t *flags, const float *bv,
+                                                        T *xv, T *rv, double *xhi,
+                                                        double *partial, int n, int bpp) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.y;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint8_t *fl = flags + base;
+  double mx = 0.0;
+  for (int q = 0; q < PS_CPT; q++) {
+    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;
+    if (c >= N) continue;
+    const bool liq = fl[c] == FLUID;
+    const float bb = liq ? bv[base + c] : 0.f;
+    rv[base + c] = T(bb);
+    xv[base + c] = T(0);
+    if (xhi) xhi[base + c] = 0.0;
+    mx = fmax(mx, fabs(double(bb)));
+  }
+  const double m = block_max_d(mx, sh);
+  if (threadIdx.x == 0) partial[int64_t(p) * bpp + blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(PS_THREADS) void k_ps_start(const double *partial, Scal *sc, int bpp,
+                                                         double tol) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.x;
+  double mx = 0.0;
+  for (int i = threadIdx.x; i < bpp; i += PS_THREADS) mx = fmax(mx, partial[int64_t(p) * bpp + i]);
+  const double b = block_max_d(mx, sh);
+  if (threadIdx.x == 0) {
+    Scal S{};
+    S.bmax = b;
+    S.target = fmax(tol * b, 1e-14);
+    S.rmax = b;
+    S.active = b >= 1e-14 ? 1 : 0;  // solver3d.ts:882: an all-zero rhs converges at once
+    S.converged = S.active ? 0 : 1;
+    sc[p] = S;
+  }
+}
+
+// solution of one precision as FP64 on the grid (0 off the liquid)
+template <class T>
+__global__ void k_ps_store(const uint8_t *flags, const T *x, const double *xhi, double *out,
+                           int64_t total) {
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= total) return;
+  out[c] = flags[c] == FLUID ? (xhi ? xhi[c] + double(x[c]) : double(x[c])) : 0.0;
+}
+
+// per problem: max|b - A x| in FP64 and max|x - x_ref|, max|x_ref| (x_ref may be null)
+__global__ __launch_bounds__(PS_THREADS) void k_ps_check(const uint8_t *flags, const float *bv,
+                                                         const double *xv, const double *refv,
+                                                         double *partial, int n, int bpp) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.y;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint8_t *fl = flags + base;
+  const double *x = xv + base;
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  double res = 0.0, err = 0.0, mref = 0.0;
+  PS_CELLS({
+    double sum = 0.0;
+    int d = 0;
+    _Pragma("unroll") for (int o = 0; o < 6; o++) {
+      const uint8_t f = fl[c + off[o]];
+      d += f != SOLID ? 1 : 0;
+      if (f == FLUID) sum += x[c + off[o]];
+    }
+    res = fmax(res, fabs(double(bv[base + c]) - (double(d) * x[c] - sum)));
+    if (refv) {
+      err = fmax(err, fabs(x[c] - refv[base + c]));
+      mref = fmax(mref, fabs(refv[base + c]));
+    }
+  })
+  const double a = block_max_d(res, sh);
+  __syncthreads();
+  const double b = block_max_d(err, sh);
+  __syncthreads();
+  const double c2 = block_max_d(mref, sh);
+  if (threadIdx.x == 0) {
+    partial[3 * (int64_t(p) * bpp + blockIdx.x)] = a;
+    partial[3 * (int64_t(p) * bpp + blockIdx.x) + 1] = b;
+    partial[3 * (int64_t(p) * bpp + blockIdx.x) + 2] = c2;
+  }
+}
+
+__global__ void k_ps_liquid(const uint8_t *flags, unsigned long long *count, int64_t N) {
+  const int p = blockIdx.y;
+  unsigned long long c = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < N;
+       i += int64_t(gridDim.x) * blockDim.x)
+    c += flags[int64_t(p) * N + i] == FLUID ? 1ull : 0ull;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(&count[p], c);
+}
+
+}  // namespace
+
+struct pl3d_poisson {
+  int n = 0, P = 0, bpp = 0;
+  int64_t N = 0;
+  uint32_t seed0 = 0;
+  hipStream_t stream = nullptr;
+  uint8_t *flags = nullptr;
+  float *b = nullptr;
+  std::vector<int64_t> liquid;
+  // work
+  void *x = nullptr, *r = nullptr, *s = nullptr, *As = nullptr;  // float or double per solve
+  size_t workElem = 0;                                           // element size of the work set
+  double *xhi = nullptr;
+  double *partial = nullptr, *partial2 = nullptr;
+  Scal *sc = nullptr;
+  double *sol[3] = {nullptr, nullptr, nullptr};
+  bool destroyed = false;
+};
+
+namespace {
+
+void free_work(pl3d_poisson *ps) {
+  for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (ps->xhi) (void)hipFree(ps->xhi);
+  ps->xhi = nullptr;
+  ps->workElem = 0;
+}
+
+int alloc_work(pl3d_poisson *ps, size_t elem, bool hi) {
+  const size_t cells = size_t(ps->P) * size_t(ps->N);
+  if (ps->workElem != elem) {
+    free_work(ps);
+    for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As}) PS_HIP(hipMalloc(p, cells * elem));
+    ps->workElem = elem;
+  }
+  if (hi && !ps->xhi) PS_HIP(hipMalloc((void **)&ps->xhi, cells * 8));
+  return PL3D_OK;
+}
+
+int check_ps(pl3d_poisson *ps) {
+  if (!ps) return perr(PL3D_EINVAL, "null Poisson set handle");
+  if (ps->destroyed) return perr(PL3D_ESTATE, "Poisson set was destroyed");
+  return PL3D_OK;
+}
+
+template <class T>
+int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *kms) {
+  const int n = ps->n, P = ps->P, bpp = ps->bpp;
+  hipStream_t st = ps->stream;
+  T *x = static_cast<T *>(ps->x), *r = static_cast<T *>(ps->r), *s = static_cast<T *>(ps->s),
+    *As = static_cast<T *>(ps->As);
+  const dim3 grid(bpp, P), blk(PS_THREADS);
+  hipEvent_t e0, e1;
+  PS_HIP(hipEventCreate(&e0));
+  PS_HIP(hipEventCreate(&e1));
+  PS_HIP(hipEventRecord(e0, st));
+  hipLaunchKernelGGL(k_ps_init<T>, grid, blk, 0, st, ps->flags, ps->b, x, r,
+                     refined ? ps->xhi : nullptr, ps->partial, n, bpp);
+  hipLaunchKernelGGL(k_ps_start, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, tol);
+  hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->flags, r, s, ps->sc, n, 2);  // s = z
+  hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->flags, r, ps->partial, ps->sc, n, bpp, 0);
+  hipLaunchKernelGGL(k_ps_sigma, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, 0);
+  PS_HIP(hipGetLastError());
+  std::vector<Scal> h(P);
+  int sinceRefresh = 0;
+  for (int it = 1; it <= maxIter; it++) {
+    hipLaunchKernelGGL(k_ps_spmv<T>, grid, blk, 0, st, ps->flags, s, As, ps->partial, ps->sc, n,
+                       bpp);
+    hipLaunchKernelGGL(k_ps_alpha, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp);
+    hipLaunchKernelGGL(k_ps_update<T>, grid, blk, 0, st, ps->flags, x, r, s, As, ps->partial,
+                       ps->partial2, ps->sc, n, bpp);
+    hipLaunchKernelGGL(k_ps_beta, dim3(P), blk, 0, st, ps->partial, ps->partial2, ps->sc, bpp,
+                       refined ? 1 : 0);
+    hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->flags, r, s, ps->sc, n, 0);
+    PS_HIP(hipGetLastError());
+    sinceRefresh++;
+    const bool poll = it % 5 == 0 || it == maxIter;
+    bool anyCheck = false, anyActive = true;
+    if (poll) {
+      PS_HIP(hipMemcpyAsync(h.data(), ps->sc, sizeof(Scal) * P, hipMemcpyDeviceToHost, st));
+      PS_HIP(hipStreamSynchronize(st));
+      anyActive = false;
+      for (const Scal &S : h) {
+        anyActive = anyActive || S.active;
+        anyCheck = anyCheck || (S.active && S.checkTrue);
+      }
+      if (!anyActive) break;
+    }
+    if constexpr (sizeof(T) == 4) {
+      if (refined && (sinceRefresh >= PS_REFRESH || anyCheck || it == maxIter)) {
+        sinceRefresh = 0;
+        float *xf = reinterpret_cast<float *>(x), *rf = reinterpret_cast<float *>(r);
+        hipLaunchKernelGGL(k_ps_refresh, grid, blk, 0, st, ps->flags, ps->b, xf, ps->xhi, rf,
+                           ps->partial, ps->partial2, ps->sc, n, bpp, 1);
+        hipLaunchKernelGGL(k_ps_refresh, grid, blk, 0, st, ps->flags, ps->b, xf, ps->xhi, rf,
+                           ps->partial, ps->partial2, ps->sc, n, bpp, 0);
+        hipLaunchKernelGGL(k_ps_refresh_verdict, dim3(P), blk, 0, st, ps->partial, ps->partial2,
+                           ps->sc, bpp);
+        // restart where flagged: s = z, sigma = z.r
+        hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->flags, r, s, ps->sc, n, 1);
+        hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->flags, r, ps->partial, ps->sc, n,
+                           bpp, 1);
+        hipLaunchKernelGGL(k_ps_sigma, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, 1);
+        PS_HIP(hipGetLastError());
+      }
+    }
+  }
+  PS_HIP(hipEventRecord(e1, st));
+  PS_HIP(hipEventSynchronize(e1));
+  PS_HIP(hipEventElapsedTime(kms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return PL3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pl3d_poisson_create(int32_t n, int32_t problems, uint32_t seed0, pl3d_poisson **out) {
+  pl3d_set_error(PL3D_OK, "");
+  if (!out || n < 8 || n > 1024 || problems < 1 || problems > 65535)
+    return perr(PL3D_EINVAL, "bad Poisson set arguments");
+  *out = nullptr;
+  auto *ps = new pl3d_poisson();
+  ps->n = n;
+  ps->P = problems;
+  ps->N = int64_t(n) * n * n;
+  ps->bpp = int((ps->N + PS_CPB - 1) / PS_CPB);
+  ps->seed0 = seed0;
+  ps->liquid.assign(problems, 0);
+  auto fail = [&](int rc) {
+    pl3d_poisson_destroy(ps);
+    return rc;
+  };
+  const size_t cells = size_t(problems) * size_t(ps->N);
+  const int64_t nf = int64_t(n + 1) * n * n;  // faces per component
+  double *raw = nullptr, *tmp = nullptr;
+  Surface *dsurf = nullptr;
+  unsigned long long *dcount = nullptr;
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&ps->stream, hipStreamNonBlocking)) ||
+      (e = hipMalloc((void **)&ps->flags, cells)) || (e = hipMalloc((void **)&ps->b, cells * 4)) ||
+      (e = hipMalloc((void **)&ps->partial, size_t(problems) * ps->bpp * 3 * 8)) ||
+      (e = hipMalloc((void **)&ps->partial2, size_t(problems) * ps->bpp * 8)) ||
+      (e = hipMalloc((void **)&ps->sc, sizeof(Scal) * problems)) ||
+      (e = hipMalloc((void **)&raw, 3 * nf * 8)) || (e = hipMalloc((void **)&tmp, 3 * nf * 8)) ||
+      (e = hipMalloc((void **)&dsurf, sizeof(Surface) * problems)) ||
+      (e = hipMalloc((void **)&dcount, 8 * size_t(problems))))
+    return fail(perr(PL3D_EDEVICE, "Poisson set allocation: %s", hipGetErrorString(e)));
+  std::vector<Surface> surf(problems);
+  for (int p = 0; p < problems; p++) surf[p] = draw_surface(n, seed0 + uint32_t(p));
+  hipStream_t st = ps->stream;
+  int err = PL3D_OK;
+  if ((e = hipMemcpyAsync(dsurf, surf.data(), sizeof(Surface) * problems, hipMemcpyHostToDevice,
+                          st)) ||
+      (e = hipMemsetAsync(dcount, 0, 8 * size_t(problems), st)))
+    err = perr(PL3D_EDEVICE, "Poisson set upload: %s", hipGetErrorString(e));
+  const int T = 256;
+  for (int p = 0; p < problems && !err; p++) {
+    uint8_t *fl = ps->flags + size_t(p) * ps->N;
+    hipLaunchKernelGGL(k_ps_flags, dim3((ps->N + T - 1) / T), dim3(T), 0, st, fl, dsurf + p, n);
+    hipLaunchKernelGGL(k_ps_normals, dim3((3 * nf + T - 1) / T), dim3(T), 0, st, raw,
+                       seed0 + uint32_t(p), 3 * nf);
+    // separable box filter: x, y, z passes per component, ping-pong raw <-> tmp
+    const int dims[3][3] = {{n + 1, n, n}, {n, n + 1, n}, {n, n, n + 1}};
+    for (int cpt = 0; cpt < 3; cpt++) {
+      double *a = raw + cpt * nf, *b2 = tmp + cpt * nf;
+      for (int ax = 0; ax < 3; ax++) {
+        hipLaunchKernelGGL(k_ps_box, dim3((nf + T - 1) / T), dim3(T), 0, st, a, b2, dims[cpt][0],
+                           dims[cpt][1], dims[cpt][2], ax);
+        std::swap(a, b2);
+      }
+      // three passes: the result is in the tmp half of this component
+    }
+    hipLaunchKernelGGL(k_ps_rhs, dim3((ps->N + T - 1) / T), dim3(T), 0, st, tmp, tmp + nf,
+                       tmp + 2 * nf, fl, ps->b + size_t(p) * ps->N, n);
+    if ((e = hipGetLastError())) err = perr(PL3D_EDEVICE, "generator: %s", hipGetErrorString(e));
+  }
+  if (!err) {
+    hipLaunchKernelGGL(k_ps_liquid, dim3(64, problems), dim3(T), 0, st, ps->flags, dcount, ps->N);
+    std::vector<unsigned long long> cnt(problems);
+    if ((e = hipMemcpyAsync(cnt.data(), dcount, 8 * size_t(problems), hipMemcpyDeviceToHost,
+                            st)) ||
+        (e = hipStreamSynchronize(st)))
+      err = perr(PL3D_EDEVICE, "generator: %s", hipGetErrorString(e));
+    for (int p = 0; p < problems; p++) ps->liquid[p] = int64_t(cnt[p]);
+  }
+  (void)hipFree(raw);
+  (void)hipFree(tmp);
+  (void)hipFree(dsurf);
+  (void)hipFree(dcount);
+  if (err) return fail(err);
+  *out = ps;
+  return PL3D_OK;
+}
+
+int pl3d_poisson_destroy(pl3d_poisson *ps) {
+  if (int rc = check_ps(ps)) return rc;
+  if (ps->stream) (void)hipStreamSynchronize(ps->stream);
+  free_work(ps);
+  for (void *p : {(void *)ps->flags, (void *)ps->b, (void *)ps->partial, (void *)ps->partial2,
+                  (void *)ps->sc, (void *)ps->sol[0], (void *)ps->sol[1], (void *)ps->sol[2]})
+    if (p) (void)hipFree(p);
+  if (ps->stream) (void)hipStreamDestroy(ps->stream);
+  ps->destroyed = true;
+  delete ps;
+  return PL3D_OK;
+}
+
+int pl3d_poisson_problem(pl3d_poisson *ps, int32_t k, uint8_t *flags, float *rhs,
+                         int64_t *liquid_cells) {
+  if (int rc = check_ps(ps)) return rc;
+  if (k < 0 || k >= ps->P) return perr(PL3D_EINVAL, "problem %d out of range", int(k));
+  const size_t off = size_t(k) * ps->N;
+  if (flags)
+    PS_HIP(hipMemcpyAsync(flags, ps->flags + off, ps->N, hipMemcpyDeviceToHost, ps->stream));
+  if (rhs)
+    PS_HIP(hipMemcpyAsync(rhs, ps->b + off, ps->N * 4, hipMemcpyDeviceToHost, ps->stream));
+  PS_HIP(hipStreamSynchronize(ps->stream));
+  if (liquid_cells) *liquid_cells = ps->liquid[k];
+  return PL3D_OK;
+}
+
+int pl3d_poisson_solve(pl3d_poisson *ps, int32_t precision, double tol, int32_t max_iter,
+                       pl3d_poisson_result *out) {
+  if (int rc = check_ps(ps)) return rc;
+  if (precision < PL3D_POISSON_FP32 || precision > PL3D_POISSON_FP64 || !(tol > 0) ||
+      max_iter < 1)
+    return perr(PL3D_EINVAL, "bad Poisson solve arguments");
+  const bool refined = precision == PL3D_POISSON_REFINED;
+  const bool f64 = precision == PL3D_POISSON_FP64;
+  if (int rc = alloc_work(ps, f64 ? 8 : 4, refined)) return rc;
+  float kms = 0.f;
+  int rc = f64 ? run_solve<double>(ps, false, tol, max_iter, &kms)
+               : run_solve<float>(ps, refined, tol, max_iter, &kms);
+  if (rc) return rc;
+  // keep the solution (FP64 on the grid) and check it
+  const size_t cells = size_t(ps->P) * size_t(ps->N);
+  if (!ps->sol[precision]) PS_HIP(hipMalloc((void **)&ps->sol[precision], cells * 8));
+  const int T = 256;
+  const dim3 g1((cells + T - 1) / T);
+  if (f64)
+    hipLaunchKernelGGL(k_ps_store<double>, g1, dim3(T), 0, ps->stream, ps->flags,
+                       static_cast<const double *>(ps->x), nullptr, ps->sol[precision],
+                       int64_t(cells));
+  else
+    hipLaunchKernelGGL(k_ps_store<float>, g1, dim3(T), 0, ps->stream, ps->flags,
+                       static_cast<const float *>(ps->x), refined ? ps->xhi : nullptr,
+                       ps->sol[precision], int64_t(cells));
+  const double *ref = ps->sol[PL3D_POISSON_FP64];
+  hipLaunchKernelGGL(k_ps_check, dim3(ps->bpp, ps->P), dim3(PS_THREADS), 0, ps->stream, ps->flags,
+                     ps->b, ps->sol[precision], ref, ps->partial, ps->n, ps->bpp);
+  PS_HIP(hipGetLastError());
+  std::vector<double> part(size_t(ps->P) * ps->bpp * 3);
+  std::vector<Scal> h(ps->P);
+  PS_HIP(hipMemcpyAsync(part.data(), ps->partial, part.size() * 8, hipMemcpyDeviceToHost,
+                        ps->stream));
+  PS_HIP(hipMemcpyAsync(h.data(), ps->sc, sizeof(Scal) * ps->P, hipMemcpyDeviceToHost,
+                        ps->stream));
+  PS_HIP(hipStreamSynchronize(ps->stream));
+  if (out) {
+    for (int p = 0; p < ps->P; p++) {
+      double res = 0, err = 0, mref = 0;
+      for (int i = 0; i < ps->bpp; i++) {
+        const size_t o = 3 * (size_t(p) * ps->bpp + i);
+        res = std::max(res, part[o]);
+        err = std::max(err, part[o + 1]);
+        mref = std::max(mref, part[o + 2]);
+      }
+      pl3d_poisson_result &R = out[p];
+      std::memset(&R, 0, sizeof R);
+      R.iterations = h[p].iters;
+      R.converged = h[p].converged;
+      R.residual = h[p].rmax;
+      R.true_residual = res;
+      R.bnorm = h[p].bmax;
+      R.error_vs_fp64 = ref ? (mref > 0 ? err / mref : err) : -1.0;
+      R.liquid_cells = ps->liquid[p];
+      R.kernel_ms = kms;
+    }
+  }
+  return PL3D_OK;
+}
+
+int pl3d_poisson_solution(pl3d_poisson *ps, int32_t precision, int32_t k, double *x) {
+  if (int rc = check_ps(ps)) return rc;
+  if (precision < 0 || precision > 2 || !ps->sol[precision] || k < 0 || k >= ps->P || !x)
+    return perr(PL3D_EINVAL, "no such solution");
+  PS_HIP(hipMemcpyAsync(x, ps->sol[precision] + size_t(k) * ps->N, ps->N * 8,
+                        hipMemcpyDeviceToHost, ps->stream));
+  PS_HIP(hipStreamSynchronize(ps->stream));
+  return PL3D_OK;
+}
+
+}  // extern "C"

@@ -324,3 +324,56 @@ class Ensemble3D:
             self.destroy()
         except Exception:
             pass
+
+
+class PoissonSet:
+    """The standalone pressure-Poisson set: `problems` seeded free-surface problems on n^3 grids
+    (the pour's operator and divergence right-hand side), generated and solved on the device by a
+    batched Jacobi-PCG at FP32, FP32 + FP64 refinement, or FP64.  Build-only (no reference
+    counterpart)."""
+
+    def __init__(self, n: int, problems: int, seed0: int = 1000):
+        self.n, self.problems, self.seed0 = int(n), int(problems), int(seed0)
+        h = C.c_void_p()
+        check(load().pl3d_poisson_create(self.n, self.problems, self.seed0, C.byref(h)))
+        self._h = h
+
+    def _handle(self):
+        if not getattr(self, "_h", None):
+            raise RuntimeError("Poisson set was destroyed")
+        return self._h
+
+    def problem(self, k: int):
+        """(flags uint8[n^3], rhs float32[n^3], liquid cells) of problem k."""
+        N = self.n ** 3
+        fl = np.zeros(N, np.uint8)
+        b = np.zeros(N, np.float32)
+        m = C.c_int64()
+        check(load().pl3d_poisson_problem(self._handle(), int(k), _ptr(fl), _ptr(b), C.byref(m)))
+        return fl, b, m.value
+
+    def solve(self, precision: str = "refined", tol: float = 1e-6, maxIter: int = 5000) -> list:
+        out = (abi.PoissonResult * self.problems)()
+        check(load().pl3d_poisson_solve(self._handle(), abi.POISSON_PRECISIONS[precision],
+                                        float(tol), int(maxIter), out))
+        return [{"iterations": r.iterations, "converged": bool(r.converged),
+                 "residual": r.residual, "trueResidual": r.true_residual, "bnorm": r.bnorm,
+                 "errorVsFp64": r.error_vs_fp64, "liquidCells": int(r.liquid_cells),
+                 "kernelMs": r.kernel_ms} for r in out]
+
+    def solution(self, precision: str, k: int) -> np.ndarray:
+        x = np.zeros(self.n ** 3, np.float64)
+        check(load().pl3d_poisson_solution(self._handle(), abi.POISSON_PRECISIONS[precision],
+                                           int(k), _ptr(x)))
+        return x
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            check(load().pl3d_poisson_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

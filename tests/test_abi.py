@@ -32,7 +32,8 @@ def test_library_exports_every_declared_symbol():
 def test_record_layout_matches_c():
     recs = {"pl3d_teapot_config": abi.TeapotConfig, "pl3d_motion_config": abi.MotionConfig,
             "pl3d_config": abi.Config, "pl3d_diagnostics": abi.Diagnostics,
-            "pl3d_info": abi.Info, "pl3d_solve_result": abi.SolveResult}
+            "pl3d_info": abi.Info, "pl3d_solve_result": abi.SolveResult,
+            "pl3d_poisson_result": abi.PoissonResult}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for cname, py in recs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

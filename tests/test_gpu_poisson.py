@@ -1,0 +1,70 @@
+"""The standalone Poisson set on the device against oracle/poisson3d.py.
+
+Bars, each stated at its assertion:
+- generator: flags bit-identical to the FP64 oracle's; rhs equal to the oracle's FP32 rhs within
+  one FP32 rounding (relative 1e-6 of max|b|: the device sums the separable box filter in another
+  order);
+- FP64 solve: max|x - x_direct| <= 1e-6 max|x_direct| at tol 1e-10 against a sparse direct solve;
+- refined solve: its FP64 true residual meets the stop rule max|b - Ax| <= tol max|b|;
+- FP32 solve: converged on its own residual; its true residual is reported (FP32 drift), and its
+  error against the FP64 solve is larger than the refined solve's.
+"""
+import numpy as np
+import pytest
+
+from oracle import poisson3d as PS
+from teapotfluidsimulation_amd import PoissonSet
+
+pytestmark = pytest.mark.gpu
+
+
+def test_generator_matches_oracle():
+    n, P, seed0 = 24, 3, 1000
+    ps = PoissonSet(n, P, seed0)
+    for k in range(P):
+        fl, b, m = ps.problem(k)
+        fl_ref = PS.flags(n, seed0 + k)
+        assert np.array_equal(fl, fl_ref), k
+        assert m == int((fl_ref == 1).sum())
+        b_ref = PS.rhs(n, seed0 + k, fl_ref)
+        assert np.abs(b - b_ref).max() <= 1e-6 * np.abs(b_ref).max(), k
+    ps.destroy()
+
+
+def test_precisions_against_direct_solve():
+    n, P, seed0 = 24, 3, 2000
+    ps = PoissonSet(n, P, seed0)
+    r64 = ps.solve("fp64", 1e-10, 2000)
+    rref = ps.solve("refined", 1e-6, 2000)
+    r32 = ps.solve("fp32", 1e-6, 2000)
+    for k in range(P):
+        fl, b, m = ps.problem(k)
+        x_direct = PS.solve(n, fl, b)
+        x64 = ps.solution("fp64", k)
+        assert r64[k]["converged"]
+        assert np.abs(x64 - x_direct).max() <= 1e-6 * np.abs(x_direct).max(), k
+        bn = np.abs(b).max()
+        assert rref[k]["converged"] and rref[k]["trueResidual"] <= 1e-6 * bn, rref[k]
+        A, L = PS.operator(n, fl)
+        xr = ps.solution("refined", k)
+        assert np.abs(A @ xr[L] - b[L].astype(np.float64)).max() <= 1e-6 * bn
+        assert r32[k]["converged"]
+        assert 0 <= rref[k]["errorVsFp64"] <= 1e-4
+        print(k, m, "iterations fp64/refined/fp32", r64[k]["iterations"], rref[k]["iterations"],
+              r32[k]["iterations"], "true residual / max|b| refined", rref[k]["trueResidual"] / bn,
+              "fp32", r32[k]["trueResidual"] / bn, "error vs fp64 refined",
+              rref[k]["errorVsFp64"], "fp32", r32[k]["errorVsFp64"], flush=True)
+    ps.destroy()
+
+
+def test_empty_rhs_and_bad_arguments():
+    ps = PoissonSet(8, 2, 5)  # an 8^3 grid: the wall leaves at most the four central columns
+    r = ps.solve("fp32", 1e-6, 50)
+    for k, x in enumerate(r):
+        fl, b, m = ps.problem(k)
+        assert x["converged"]
+        if m == 0 or np.abs(b).max() < 1e-14:
+            assert x["iterations"] == 0
+    with pytest.raises(RuntimeError):
+        ps.solve("fp64", -1.0, 10)
+    ps.destroy()

@@ -368,3 +368,37 @@ def test_mg_pcg_converges_and_beats_jacobi():
     assert np.abs(b - aq)[cell == 1].max() <= 1e-6 * np.abs(b[cell == 1]).max()
     _, rref = O.pcg_solve(nx, ny, nz, cell, b, 1e-6, 400)
     assert it <= rref.iterations + 2  # no worse than the reference's MIC(0)
+
+
+def test_poisson_generator_oracle():
+    """oracle/poisson3d.py: its mulberry32 equals the C++ oracle's (pinned by the recording), the
+    separable clamped box filter equals a direct 27-point clamped mean, the flags give a liquid
+    region inside the wall and under the surface, and a direct solve satisfies A x = b."""
+    from oracle import poisson3d as PS
+    r = PS.Mulberry32(1000)
+    assert np.array_equal(np.array([r() for _ in range(16)]), O.rng(1000, 16))
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((5, 6, 7))
+    direct = np.zeros_like(a)
+    for k in range(5):
+        for j in range(6):
+            for i in range(7):
+                acc = 0.0
+                for dk in (-1, 0, 1):
+                    for dj in (-1, 0, 1):
+                        for di in (-1, 0, 1):
+                            acc += a[min(max(k + dk, 0), 4), min(max(j + dj, 0), 5),
+                                     min(max(i + di, 0), 6)]
+                direct[k, j, i] = acc / 27
+    assert np.allclose(PS.box3(a), direct, atol=1e-14)
+    n = 24
+    fl = PS.flags(n, 1000)
+    f3 = fl.reshape(n, n, n)
+    assert (f3[0] == 2).all() and (f3[:, 0] == 2).all() and (f3[:, :, 0] == 2).all()
+    m = int((fl == 1).sum())
+    assert 0.05 * n ** 3 < m < 0.6 * n ** 3
+    b = PS.rhs(n, 1000, fl)
+    assert np.all(b[fl != 1] == 0) and np.abs(b[fl == 1]).max() > 0
+    x = PS.solve(n, fl, b)
+    A, L = PS.operator(n, fl)
+    assert np.abs(A @ x[L] - b[L]).max() < 1e-10
