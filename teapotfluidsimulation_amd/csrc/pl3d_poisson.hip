@@ -173,15 +173,27 @@ __global__ void k_ps_rhs(const double *u, const double *v, const double *w, cons
   b[c] = float(-div);
 }
 
+// per-cell info word of one problem (see PS_CELLS)
+__global__ void k_ps_info(const uint8_t *__restrict__ fl, uint16_t *__restrict__ info, int n) {
+  const int64_t N = int64_t(n) * n * n;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  if (fl[c] != FLUID) {
+    info[c] = 0;
+    return;
+  }
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  uint32_t w = 0x8000u, d = 0;
+  for (int o = 0; o < 6; o++) {
+    const uint8_t f = fl[c + off[o]];
+    d += f != SOLID ? 1u : 0u;
+    w |= f == FLUID ? (1u << o) : 0u;
+  }
+  info[c] = uint16_t(w | (d << 8));
+}
+
 // ------------------------------------------------------------------------------- solver kernels
 
-// non-solid neighbour count and the six neighbour offsets of a FLUID cell (never on the border)
-__device__ __forceinline__ int ps_diag(const uint8_t *fl, int64_t c, const int64_t off[6]) {
-  int d = 0;
-#pragma unroll
-  for (int q = 0; q < 6; q++) d += fl[c + off[q]] != SOLID ? 1 : 0;
-  return d;
-}
 
 __device__ __forceinline__ double block_sum_d(double v, double *sh) {
 #pragma unroll
@@ -205,40 +217,86 @@ __device__ __forceinline__ double block_max_d(double v, double *sh) {
   return t;
 }
 
-#define PS_CELLS(...)                                                              \
-  for (int q = 0; q < PS_CPT; q++) {                                               \
-    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x; \
-    if (c < N && fl[c] == FLUID) {                                                 \
-      __VA_ARGS__                                                                  \
-    }                                                                              \
+// Each thread takes PS_CPT cells, PS_THREADS apart.  The cells' info words are loaded first, all
+// together; the body then runs for the FLUID cells with `c` (cell index) and `inf` (its word:
+// bit 15 FLUID, bits 0-5 the -x,+x,-y,+y,-z,+z neighbours that are FLUID, bits 8-10 the
+// non-solid neighbour count) in scope.  FLUID cells are never on the border, so all six
+// neighbours of c exist and may be loaded unconditionally.
+#define PS_CELLS(...)                                                                     \
+  {                                                                                       \
+    uint32_t inf_[PS_CPT];                                                                \
+    _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
+      const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+      inf_[q] = c < N ? uint32_t(info[c]) : 0u;                                           \
+    }                                                                                     \
+    _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
+      const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+      const uint32_t inf = inf_[q];                                                       \
+      if (inf & 0x8000u) {                                                                \
+        __VA_ARGS__                                                                       \
+      }                                                                                   \
+    }                                                                                     \
   }
+#define PS_DIAG(inf) int(((inf) >> 8) & 7u)
+#define PS_NB(inf, o) (((inf) >> (o)) & 1u)
+
+// Two-phase cell loop of the streaming kernels: the PS_CPT info words of this thread, then every
+// value load of all its cells at once (a non-FLUID cell loads from the interior cell (1,1,1),
+// whose neighbours all exist, and its results are discarded), then the FLUID cells' stores.  Two
+// dependent memory round trips per thread instead of one per cell.
+#define PS_LOAD_INFO()                                                                  \
+  uint32_t inf[PS_CPT];                                                                 \
+  int64_t cc[PS_CPT];                                                                   \
+  _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
+    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+    inf[q] = c < N ? uint32_t(info[c]) : 0u;                                            \
+  }                                                                                     \
+  _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
+    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+    cc[q] = (inf[q] & 0x8000u) ? c : safe;                                              \
+  }                                                                                     \
+  /* a wave with no FLUID cell (air or solid regions) skips its loads and stores */     \
+  const bool waveLiquid = __any(((inf[0] | inf[1] | inf[2] | inf[3]) & 0x8000u) != 0u);
 
 // As = A s, partial s.As per block
 template <class T>
-__global__ __launch_bounds__(PS_THREADS) void k_ps_spmv(const uint8_t *flags, const T *sv, T *Asv,
-                                                        double *partial, const Scal *sc, int n,
+__global__ __launch_bounds__(PS_THREADS) void k_ps_spmv(const uint16_t *__restrict__ infov,
+                                                        const T *__restrict__ sv,
+                                                        T *__restrict__ Asv,
+                                                        double *__restrict__ partial,
+                                                        const Scal *__restrict__ sc, int n,
                                                         int bpp) {
   __shared__ double sh[PS_THREADS / 64];
   const int p = blockIdx.y;
   if (!sc[p].active) return;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const uint8_t *fl = flags + base;
+  const int64_t safe = 1 + n + int64_t(n) * n;
+  const uint16_t *info = infov + base;
   const T *s = sv + base;
   T *As = Asv + base;
   const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  PS_LOAD_INFO();
   double acc = 0.0;
-  PS_CELLS({
-    T sum = T(0);
-    int d = 0;
-    _Pragma("unroll") for (int o = 0; o < 6; o++) {
-      const uint8_t f = fl[c + off[o]];
-      d += f != SOLID ? 1 : 0;
-      if (f == FLUID) sum += s[c + off[o]];
+  if (waveLiquid) {
+    T s0[PS_CPT], nb[PS_CPT][6];
+#pragma unroll
+    for (int q = 0; q < PS_CPT; q++) {
+      s0[q] = s[cc[q]];
+#pragma unroll
+      for (int o = 0; o < 6; o++) nb[q][o] = s[cc[q] + off[o]];
     }
-    const T v = T(d) * s[c] - sum;
-    As[c] = v;
-    acc += double(s[c]) * double(v);
-  })
+#pragma unroll
+    for (int q = 0; q < PS_CPT; q++) {
+      T sum = T(0);
+#pragma unroll
+      for (int o = 0; o < 6; o++) sum += PS_NB(inf[q], o) ? nb[q][o] : T(0);
+      const T v = T(PS_DIAG(inf[q])) * s0[q] - sum;
+      if (inf[q] & 0x8000u) {
+        As[cc[q]] = v;
+        acc += double(s0[q]) * double(v);
+      }
+    }
+  }
   const double t = block_sum_d(acc, sh);
   if (threadIdx.x == 0) partial[int64_t(p) * bpp + blockIdx.x] = t;
 }
@@ -262,28 +320,46 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_alpha(const double *partial, 
 
 // x += alpha s, r -= alpha As; partial z.r (z = r / diag) and max|r|
 template <class T>
-__global__ __launch_bounds__(PS_THREADS) void k_ps_update(const uint8_t *flags, T *xv, T *rv,
-                                                          const T *sv, const T *Asv,
-                                                          double *partial, double *partial2,
-                                                          const Scal *sc, int n, int bpp) {
+__global__ __launch_bounds__(PS_THREADS) void k_ps_update(const uint16_t *__restrict__ infov,
+                                                          T *__restrict__ xv, T *__restrict__ rv,
+                                                          const T *__restrict__ sv,
+                                                          const T *__restrict__ Asv,
+                                                          double *__restrict__ partial,
+                                                          double *__restrict__ partial2,
+                                                          const Scal *__restrict__ sc, int n,
+                                                          int bpp) {
   __shared__ double sh[PS_THREADS / 64];
   const int p = blockIdx.y;
   if (!sc[p].active) return;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const uint8_t *fl = flags + base;
+  const int64_t safe = 1 + n + int64_t(n) * n;
+  const uint16_t *info = infov + base;
   T *x = xv + base, *r = rv + base;
   const T *s = sv + base, *As = Asv + base;
   const T alpha = T(sc[p].alpha);
-  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  PS_LOAD_INFO();
   double acc = 0.0, mx = 0.0;
-  PS_CELLS({
-    x[c] += alpha * s[c];
-    const T rr = r[c] - alpha * As[c];
-    r[c] = rr;
-    const T z = rr / T(ps_diag(fl, c, off));
-    acc += double(z) * double(rr);
-    mx = fmax(mx, fabs(double(rr)));
-  })
+  if (waveLiquid) {
+    T xs[PS_CPT], rs[PS_CPT], ss[PS_CPT], as[PS_CPT];
+#pragma unroll
+    for (int q = 0; q < PS_CPT; q++) {
+      xs[q] = x[cc[q]];
+      rs[q] = r[cc[q]];
+      ss[q] = s[cc[q]];
+      as[q] = As[cc[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < PS_CPT; q++) {
+      if (inf[q] & 0x8000u) {
+        x[cc[q]] = xs[q] + alpha * ss[q];
+        const T rr = rs[q] - alpha * as[q];
+        r[cc[q]] = rr;
+        const T z = rr / T(PS_DIAG(inf[q]));
+        acc += double(z) * double(rr);
+        mx = fmax(mx, fabs(double(rr)));
+      }
+    }
+  }
   const double t = block_sum_d(acc, sh);
   __syncthreads();
   const double m = block_max_d(mx, sh);
@@ -327,38 +403,51 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_beta(const double *partial,
 
 // Direction update.  mode 0: s = z + beta s (beta in the alpha slot after k_ps_beta); mode 1:
 // s = z for the problems flagged to restart; mode 2: s = z for every active problem (start; the
-// old s is not read, it may hold anything).  Inactive problems are untouched.
+// old s is not used, it may hold anything).  Inactive problems are untouched.
 template <class T>
-__global__ __launch_bounds__(PS_THREADS) void k_ps_dir(const uint8_t *flags, const T *rv, T *sv,
-                                                       const Scal *sc, int n, int mode) {
+__global__ __launch_bounds__(PS_THREADS) void k_ps_dir(const uint16_t *__restrict__ infov,
+                                                       const T *__restrict__ rv,
+                                                       T *__restrict__ sv,
+                                                       const Scal *__restrict__ sc, int n,
+                                                       int mode) {
   const int p = blockIdx.y;
   if (!sc[p].active) return;
   if (mode == 1 && !sc[p].restart) return;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const uint8_t *fl = flags + base;
+  const int64_t safe = 1 + n + int64_t(n) * n;
+  const uint16_t *info = infov + base;
   const T *r = rv + base;
   T *s = sv + base;
-  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
-  if (mode == 0) {
-    const T beta = T(sc[p].alpha);
-    PS_CELLS({ s[c] = r[c] / T(ps_diag(fl, c, off)) + beta * s[c]; })
-  } else {
-    PS_CELLS({ s[c] = r[c] / T(ps_diag(fl, c, off)); })
+  const T beta = mode == 0 ? T(sc[p].alpha) : T(0);
+  PS_LOAD_INFO();
+  if (!waveLiquid) return;
+  T rs[PS_CPT], ss[PS_CPT];
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) {
+    rs[q] = r[cc[q]];
+    ss[q] = mode == 0 ? s[cc[q]] : T(0);
   }
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++)
+    if (inf[q] & 0x8000u) s[cc[q]] = rs[q] / T(PS_DIAG(inf[q])) + beta * ss[q];
 }
 
 // REFINED: x_hi += x_lo, x_lo = 0, r = b - A x_hi in FP64 (stored FP32), partial max|r| and
 // max|r - r_recursive|
-__global__ __launch_bounds__(PS_THREADS) void k_ps_refresh(const uint8_t *flags, const float *bv,
-                                                           float *xlo, double *xhi, float *rv,
-                                                           double *partial, double *partial2,
-                                                           const Scal *sc, int n, int bpp,
-                                                           int fold) {
+__global__ __launch_bounds__(PS_THREADS) void k_ps_refresh(const uint16_t *__restrict__ infov,
+                                                           const float *__restrict__ bv,
+                                                           float *__restrict__ xlo,
+                                                           double *__restrict__ xhi,
+                                                           float *__restrict__ rv,
+                                                           double *__restrict__ partial,
+                                                           double *__restrict__ partial2,
+                                                           const Scal *__restrict__ sc, int n,
+                                                           int bpp, int fold) {
   __shared__ double sh[PS_THREADS / 64];
   const int p = blockIdx.y;
   if (!sc[p].active) return;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const uint8_t *fl = flags + base;
+  const uint16_t *info = infov + base;
   const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
   double *xh = xhi + base;
   float *xl = xlo + base;
@@ -373,14 +462,11 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_refresh(const uint8_t *flags,
   const float *b = bv + base;
   double mx = 0.0, gap = 0.0;
   PS_CELLS({
+    double nb[6];
+    _Pragma("unroll") for (int o = 0; o < 6; o++) nb[o] = xh[c + off[o]];
     double sum = 0.0;
-    int d = 0;
-    _Pragma("unroll") for (int o = 0; o < 6; o++) {
-      const uint8_t f = fl[c + off[o]];
-      d += f != SOLID ? 1 : 0;
-      if (f == FLUID) sum += xh[c + off[o]];
-    }
-    const double rt = double(b[c]) - (double(d) * xh[c] - sum);
+    _Pragma("unroll") for (int o = 0; o < 6; o++) sum += PS_NB(inf, o) ? nb[o] : 0.0;
+    const double rt = double(b[c]) - (double(PS_DIAG(inf)) * xh[c] - sum);
     gap = fmax(gap, fabs(rt - double(r[c])));
     r[c] = float(rt);
     mx = fmax(mx, fabs(rt));
@@ -424,18 +510,19 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_refresh_verdict(const double 
 
 // sigma = z.r after a restart (only problems flagged for it)
 template <class T>
-__global__ __launch_bounds__(PS_THREADS) void k_ps_zr(const uint8_t *flags, const T *rv,
-                                                      double *partial, const Scal *sc, int n,
+__global__ __launch_bounds__(PS_THREADS) void k_ps_zr(const uint16_t *__restrict__ infov,
+                                                      const T *__restrict__ rv,
+                                                      double *__restrict__ partial,
+                                                      const Scal *__restrict__ sc, int n,
                                                       int bpp, int onlyRestart) {
   __shared__ double sh[PS_THREADS / 64];
   const int p = blockIdx.y;
   if (!sc[p].active || (onlyRestart && !sc[p].restart)) return;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const uint8_t *fl = flags + base;
+  const uint16_t *info = infov + base;
   const T *r = rv + base;
-  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
   double acc = 0.0;
-  PS_CELLS({ acc += double(r[c]) * double(r[c]) / double(ps_diag(fl, c, off)); })
+  PS_CELLS({ acc += double(r[c]) * double(r[c]) / double(PS_DIAG(inf)); })
   const double t = block_sum_d(acc, sh);
   if (threadIdx.x == 0) partial[int64_t(p) * bpp + blockIdx.x] = t;
 }
@@ -506,25 +593,23 @@ __global__ void k_ps_store(const uint8_t *flags, const T *x, const double *xhi, 
 }
 
 // per problem: max|b - A x| in FP64 and max|x - x_ref|, max|x_ref| (x_ref may be null)
-__global__ __launch_bounds__(PS_THREADS) void k_ps_check(const uint8_t *flags, const float *bv,
-                                                         const double *xv, const double *refv,
-                                                         double *partial, int n, int bpp) {
+__global__ __launch_bounds__(PS_THREADS) void k_ps_check(const uint16_t *__restrict__ infov,
+                                                         const float *__restrict__ bv,
+                                                         const double *__restrict__ xv,
+                                                         const double *__restrict__ refv,
+                                                         double *__restrict__ partial, int n,
+                                                         int bpp) {
   __shared__ double sh[PS_THREADS / 64];
   const int p = blockIdx.y;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const uint8_t *fl = flags + base;
+  const uint16_t *info = infov + base;
   const double *x = xv + base;
   const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
   double res = 0.0, err = 0.0, mref = 0.0;
   PS_CELLS({
     double sum = 0.0;
-    int d = 0;
-    _Pragma("unroll") for (int o = 0; o < 6; o++) {
-      const uint8_t f = fl[c + off[o]];
-      d += f != SOLID ? 1 : 0;
-      if (f == FLUID) sum += x[c + off[o]];
-    }
-    res = fmax(res, fabs(double(bv[base + c]) - (double(d) * x[c] - sum)));
+    _Pragma("unroll") for (int o = 0; o < 6; o++) sum += PS_NB(inf, o) ? x[c + off[o]] : 0.0;
+    res = fmax(res, fabs(double(bv[base + c]) - (double(PS_DIAG(inf)) * x[c] - sum)));
     if (refv) {
       err = fmax(err, fabs(x[c] - refv[base + c]));
       mref = fmax(mref, fabs(refv[base + c]));
@@ -561,6 +646,7 @@ struct pl3d_poisson {
   uint32_t seed0 = 0;
   hipStream_t stream = nullptr;
   uint8_t *flags = nullptr;
+  uint16_t *info = nullptr;  // per-cell info words (see PS_CELLS)
   float *b = nullptr;
   std::vector<int64_t> liquid;
   // work
@@ -616,21 +702,21 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
   hipLaunchKernelGGL(k_ps_init<T>, grid, blk, 0, st, ps->flags, ps->b, x, r,
                      refined ? ps->xhi : nullptr, ps->partial, n, bpp);
   hipLaunchKernelGGL(k_ps_start, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, tol);
-  hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->flags, r, s, ps->sc, n, 2);  // s = z
-  hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->flags, r, ps->partial, ps->sc, n, bpp, 0);
+  hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 2);  // s = z
+  hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n, bpp, 0);
   hipLaunchKernelGGL(k_ps_sigma, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, 0);
   PS_HIP(hipGetLastError());
   std::vector<Scal> h(P);
   int sinceRefresh = 0;
   for (int it = 1; it <= maxIter; it++) {
-    hipLaunchKernelGGL(k_ps_spmv<T>, grid, blk, 0, st, ps->flags, s, As, ps->partial, ps->sc, n,
+    hipLaunchKernelGGL(k_ps_spmv<T>, grid, blk, 0, st, ps->info, s, As, ps->partial, ps->sc, n,
                        bpp);
     hipLaunchKernelGGL(k_ps_alpha, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp);
-    hipLaunchKernelGGL(k_ps_update<T>, grid, blk, 0, st, ps->flags, x, r, s, As, ps->partial,
+    hipLaunchKernelGGL(k_ps_update<T>, grid, blk, 0, st, ps->info, x, r, s, As, ps->partial,
                        ps->partial2, ps->sc, n, bpp);
     hipLaunchKernelGGL(k_ps_beta, dim3(P), blk, 0, st, ps->partial, ps->partial2, ps->sc, bpp,
                        refined ? 1 : 0);
-    hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->flags, r, s, ps->sc, n, 0);
+    hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 0);
     PS_HIP(hipGetLastError());
     sinceRefresh++;
     const bool poll = it % 5 == 0 || it == maxIter;
@@ -649,15 +735,15 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
       if (refined && (sinceRefresh >= PS_REFRESH || anyCheck || it == maxIter)) {
         sinceRefresh = 0;
         float *xf = reinterpret_cast<float *>(x), *rf = reinterpret_cast<float *>(r);
-        hipLaunchKernelGGL(k_ps_refresh, grid, blk, 0, st, ps->flags, ps->b, xf, ps->xhi, rf,
+        hipLaunchKernelGGL(k_ps_refresh, grid, blk, 0, st, ps->info, ps->b, xf, ps->xhi, rf,
                            ps->partial, ps->partial2, ps->sc, n, bpp, 1);
-        hipLaunchKernelGGL(k_ps_refresh, grid, blk, 0, st, ps->flags, ps->b, xf, ps->xhi, rf,
+        hipLaunchKernelGGL(k_ps_refresh, grid, blk, 0, st, ps->info, ps->b, xf, ps->xhi, rf,
                            ps->partial, ps->partial2, ps->sc, n, bpp, 0);
         hipLaunchKernelGGL(k_ps_refresh_verdict, dim3(P), blk, 0, st, ps->partial, ps->partial2,
                            ps->sc, bpp);
         // restart where flagged: s = z, sigma = z.r
-        hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->flags, r, s, ps->sc, n, 1);
-        hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->flags, r, ps->partial, ps->sc, n,
+        hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 1);
+        hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n,
                            bpp, 1);
         hipLaunchKernelGGL(k_ps_sigma, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, 1);
         PS_HIP(hipGetLastError());
@@ -700,6 +786,7 @@ int pl3d_poisson_create(int32_t n, int32_t problems, uint32_t seed0, pl3d_poisso
   hipError_t e;
   if ((e = hipStreamCreateWithFlags(&ps->stream, hipStreamNonBlocking)) ||
       (e = hipMalloc((void **)&ps->flags, cells)) || (e = hipMalloc((void **)&ps->b, cells * 4)) ||
+      (e = hipMalloc((void **)&ps->info, cells * 2)) ||
       (e = hipMalloc((void **)&ps->partial, size_t(problems) * ps->bpp * 3 * 8)) ||
       (e = hipMalloc((void **)&ps->partial2, size_t(problems) * ps->bpp * 8)) ||
       (e = hipMalloc((void **)&ps->sc, sizeof(Scal) * problems)) ||
@@ -719,6 +806,8 @@ int pl3d_poisson_create(int32_t n, int32_t problems, uint32_t seed0, pl3d_poisso
   for (int p = 0; p < problems && !err; p++) {
     uint8_t *fl = ps->flags + size_t(p) * ps->N;
     hipLaunchKernelGGL(k_ps_flags, dim3((ps->N + T - 1) / T), dim3(T), 0, st, fl, dsurf + p, n);
+    hipLaunchKernelGGL(k_ps_info, dim3((ps->N + T - 1) / T), dim3(T), 0, st, fl,
+                       ps->info + size_t(p) * ps->N, n);
     hipLaunchKernelGGL(k_ps_normals, dim3((3 * nf + T - 1) / T), dim3(T), 0, st, raw,
                        seed0 + uint32_t(p), 3 * nf);
     // separable box filter: x, y, z passes per component, ping-pong raw <-> tmp
@@ -758,7 +847,7 @@ int pl3d_poisson_destroy(pl3d_poisson *ps) {
   if (int rc = check_ps(ps)) return rc;
   if (ps->stream) (void)hipStreamSynchronize(ps->stream);
   free_work(ps);
-  for (void *p : {(void *)ps->flags, (void *)ps->b, (void *)ps->partial, (void *)ps->partial2,
+  for (void *p : {(void *)ps->flags, (void *)ps->info, (void *)ps->b, (void *)ps->partial, (void *)ps->partial2,
                   (void *)ps->sc, (void *)ps->sol[0], (void *)ps->sol[1], (void *)ps->sol[2]})
     if (p) (void)hipFree(p);
   if (ps->stream) (void)hipStreamDestroy(ps->stream);
@@ -808,7 +897,7 @@ int pl3d_poisson_solve(pl3d_poisson *ps, int32_t precision, double tol, int32_t 
                        static_cast<const float *>(ps->x), refined ? ps->xhi : nullptr,
                        ps->sol[precision], int64_t(cells));
   const double *ref = ps->sol[PL3D_POISSON_FP64];
-  hipLaunchKernelGGL(k_ps_check, dim3(ps->bpp, ps->P), dim3(PS_THREADS), 0, ps->stream, ps->flags,
+  hipLaunchKernelGGL(k_ps_check, dim3(ps->bpp, ps->P), dim3(PS_THREADS), 0, ps->stream, ps->info,
                      ps->b, ps->sol[precision], ref, ps->partial, ps->n, ps->bpp);
   PS_HIP(hipGetLastError());
   std::vector<double> part(size_t(ps->P) * ps->bpp * 3);
