@@ -249,6 +249,22 @@ int pl3d_read_affine(pl3d_sim *sim, double *out);
 /* Manual tilt target, degrees (PourController.setTarget, sim/motion.ts:40-44). */
 int pl3d_set_target(pl3d_sim *sim, double angle_deg);
 
+/* ---- recordings ----------------------------------------------------------------------------- */
+
+/* Frames of a pour3d.bin.gz recording (src/view3d/format.ts:1-76, scripts/record3d.ts:16-68),
+ * encoded on the device: every stride-th parcel, positions quantised to `quantum` metres.  The
+ * header, gzip and the frame schedule stay with the host (record3d.py, node/scripts/record3d.js).
+ * pl3d_destroy (pl3d_ensemble_destroy for a member) fails with PL3D_ESTATE while a recorder of
+ * the simulation is live. */
+typedef struct pl3d_recorder pl3d_recorder;
+int pl3d_recorder_create(pl3d_sim *sim, int32_t stride, double quantum, pl3d_recorder **out);
+/* number of kept parcels, and their origin layers (count bytes; may be NULL) */
+int pl3d_recorder_parcels(pl3d_recorder *rec, int32_t *count, uint8_t *origin);
+/* the current state as the next frame: 3*count 16-bit words, u16 values for the first frame and
+ * int16 differences to the previous frame after it (encodeRecording, format.ts:46-76) */
+int pl3d_recorder_capture(pl3d_recorder *rec, uint16_t *out);
+int pl3d_recorder_destroy(pl3d_recorder *rec);
+
 /* ---- ensembles ------------------------------------------------------------------------------ */
 
 /* An ensemble of independent pours, possibly on different grids, stepped together: every kernel of

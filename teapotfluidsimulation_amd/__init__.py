@@ -14,6 +14,7 @@ from .sim3d import (  # noqa: F401
     SPILLED,
     Pcg3D,
     PoissonSet,
+    Recorder3D,
     Simulation3D,
     gentlePour3D,
     variedPour3D,

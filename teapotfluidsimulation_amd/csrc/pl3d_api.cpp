@@ -155,6 +155,7 @@ struct pl3d_sim {
   PcgShared A{};
   unsigned char *mg = nullptr;  // multigrid scratch of the two solves (MG only)
   pl3d_ensemble *ensemble = nullptr;  // set while a member of an ensemble (stepped by it)
+  int recorders = 0;                  // live pl3d_recorder objects reading this simulation
 };
 
 // An ensemble: independent pours whose substeps run as one launch per kernel (blockIdx.y selects
@@ -764,6 +765,7 @@ static int destroy_sim(pl3d_sim *s);
 int pl3d_destroy(pl3d_sim *s) {
   if (!s) return fail(PL3D_EINVAL, "null simulation handle");
   if (s->ensemble) return fail(PL3D_ESTATE, "simulation is owned by an ensemble");
+  if (s->recorders) return fail(PL3D_ESTATE, "destroy the simulation's recorders first");
   return destroy_sim(s);
 }
 
@@ -1188,6 +1190,8 @@ int pl3d_ensemble_create(const pl3d_config *cfgs, int32_t count, pl3d_ensemble *
 
 int pl3d_ensemble_destroy(pl3d_ensemble *e) {
   if (int rc = check_ensemble(e)) return rc;
+  for (pl3d_sim *m : e->members)
+    if (m->recorders) return fail(PL3D_ESTATE, "destroy the members' recorders first");
   free_ensemble(e);
   return PL3D_OK;
 }
@@ -1258,6 +1262,76 @@ int pl3d_ensemble_finished(pl3d_ensemble *e, int32_t *out) {
     const pl3d_sim *m = e->members[i];
     out[i] = m->hctl->time >= m->cfg.duration - 1e-9 ? 1 : 0;
   }
+  return PL3D_OK;
+}
+
+
+// ---- recordings (scripts/record3d.ts, src/view3d/format.ts) -------------------------------------
+
+struct pl3d_recorder {
+  pl3d_sim *sim = nullptr;
+  int stride = 1, count = 0, frames = 0;
+  double quantum = 0.00025;
+  int *prev = nullptr;        // device, 3 count
+  uint16_t *out = nullptr;    // device, 3 count
+  bool destroyed = false;
+};
+
+int pl3d_recorder_create(pl3d_sim *s, int32_t stride, double quantum, pl3d_recorder **out) {
+  if (int rc = check_sim(s)) return rc;
+  if (!out || stride < 1 || !(quantum > 0)) return fail(PL3D_EINVAL, "bad recorder arguments");
+  auto *r = new pl3d_recorder();
+  r->sim = s;
+  r->stride = stride;
+  r->quantum = quantum;
+  r->count = (s->P.n + stride - 1) / stride;
+  hipError_t e;
+  if ((e = hipMalloc((void **)&r->prev, size_t(3) * r->count * 4)) ||
+      (e = hipMalloc((void **)&r->out, size_t(3) * r->count * 2))) {
+    if (r->prev) (void)hipFree(r->prev);
+    delete r;
+    return fail(PL3D_EDEVICE, "recorder allocation: %s", hipGetErrorString(e));
+  }
+  s->recorders++;
+  *out = r;
+  return PL3D_OK;
+}
+
+int pl3d_recorder_parcels(pl3d_recorder *r, int32_t *count, uint8_t *origin) {
+  if (!r || r->destroyed) return fail(PL3D_EINVAL, "bad recorder handle");
+  if (int rc = check_sim(r->sim)) return rc;
+  if (count) *count = r->count;
+  if (origin) {
+    std::vector<uint8_t> all(size_t(r->sim->P.n));
+    HIP_OK(hipMemcpyAsync(all.data(), r->sim->B.origin, all.size(), hipMemcpyDeviceToHost,
+                          r->sim->stream));
+    HIP_OK(hipStreamSynchronize(r->sim->stream));
+    for (int k = 0; k < r->count; k++) origin[k] = all[size_t(k) * r->stride];
+  }
+  return PL3D_OK;
+}
+
+int pl3d_recorder_capture(pl3d_recorder *r, uint16_t *out) {
+  if (!r || r->destroyed || !out) return fail(PL3D_EINVAL, "bad recorder arguments");
+  pl3d_sim *s = r->sim;
+  if (int rc = check_sim(s)) return rc;
+  hipLaunchKernelGGL(k_record, dim3((r->count + 255) / 256), dim3(256), 0, s->stream, s->B,
+                     s->P.n, r->stride, r->count, r->quantum, r->frames == 0 ? 1 : 0, r->prev,
+                     r->out);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpyAsync(out, r->out, size_t(3) * r->count * 2, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  r->frames++;
+  return PL3D_OK;
+}
+
+int pl3d_recorder_destroy(pl3d_recorder *r) {
+  if (!r || r->destroyed) return fail(PL3D_EINVAL, "bad recorder handle");
+  if (r->prev) (void)hipFree(r->prev);
+  if (r->out) (void)hipFree(r->out);
+  r->sim->recorders--;
+  r->destroyed = true;
+  delete r;
   return PL3D_OK;
 }
 

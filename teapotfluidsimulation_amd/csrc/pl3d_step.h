@@ -56,4 +56,6 @@ __global__ void k_faces_post(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_extrap(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_advect(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_render_positions(DevBufs B, DevParams P, float *out);
+__global__ void k_record(DevBufs B, int n, int stride, int count, double quantum, int first,
+                         int *prev, uint16_t *out);
 __global__ void k_copy_probe(const float4 *a, float4 *b, size_t n4);

@@ -903,6 +903,41 @@ __global__ __launch_bounds__(STEP_THREADS) void k_advect(const DevBufs *__restri
   }
 }
 
+// ============================================================================= recording frames
+// encodeRecording (src/view3d/format.ts:46-76) for the kept parcels p = k * stride of the current
+// state: Math.round(Float32 position / quantum) clamped to [0, 65535]; frame 0 stores the values
+// (u16), later frames the differences to the previous frame clamped to int16, and the running
+// state advances by the clamped difference so that decoding reproduces it.
+__global__ void k_record(DevBufs B, int n, int stride, int count, double quantum, int first,
+                         int *__restrict__ prev, uint16_t *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int p = k * stride;
+  if (p >= n) return;
+  const float pos[3] = {B.x[p], B.y[p], B.z[p]};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double v = double(pos[d]) / quantum;
+    int cur;
+    if (v != v) {
+      cur = 0;  // Math.round(NaN) -> NaN -> 0 in an Int32Array
+    } else {
+      double r = floor(v);
+      if (v - r >= 0.5) r += 1.0;  // Math.round: halves go up
+      cur = int(fmin(65535.0, fmax(0.0, r)));
+    }
+    const int i = 3 * k + d;
+    if (first) {
+      out[i] = uint16_t(cur);
+      prev[i] = cur;
+    } else {
+      const int delta = max(-32768, min(32767, cur - prev[i]));
+      out[i] = uint16_t(int16_t(delta));
+      prev[i] += delta;
+    }
+  }
+}
+
 // ============================================================================= render positions
 // writePositions (gpu3d/shaders.ts:779-801): xyz of the simulated half, then its mirror image
 // about the symmetry plane z = h.

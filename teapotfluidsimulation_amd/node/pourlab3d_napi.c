@@ -423,9 +423,8 @@ static napi_value js_destroy(napi_env env, napi_callback_info info) {
   if (!args(env, info, 1, argv)) return NULL;
   CHECK_NAPI(napi_get_value_external(env, argv[0], (void **)&b));
   if (b && b->sim) {
-    pl3d_sim *s = b->sim;
+    CHECK_PL3D(pl3d_destroy(b->sim));  /* fails, leaving the handle live, while a recorder is */
     b->sim = NULL;
-    CHECK_PL3D(pl3d_destroy(s));
   }
   return NULL;
 }
@@ -632,6 +631,114 @@ static napi_value js_set_target(napi_env env, napi_callback_info info) {
   return NULL;
 }
 
+/* ------------------------------------------------------------------ recordings
+ * A recorder handle holds a reference to its simulation's handle, so the simulation cannot be
+ * collected before the recorder is. */
+typedef struct {
+  pl3d_recorder *rec;
+  napi_ref sim_ref;
+} recorder_box;
+
+static void finalize_recorder(napi_env env, void *data, void *hint) {
+  (void)hint;
+  recorder_box *b = (recorder_box *)data;
+  if (b->rec) pl3d_recorder_destroy(b->rec);
+  if (b->sim_ref) napi_delete_reference(env, b->sim_ref);
+  free(b);
+}
+
+static int get_recorder(napi_env env, napi_value v, recorder_box **out) {
+  recorder_box *b = NULL;
+  napi_valuetype t;
+  if (napi_typeof(env, v, &t) != napi_ok || t != napi_external ||
+      napi_get_value_external(env, v, (void **)&b) != napi_ok || !b) {
+    napi_throw_type_error(env, NULL, "pourlab3d: expected a recorder handle");
+    return 0;
+  }
+  if (!b->rec) {
+    napi_throw_error(env, NULL, "pourlab3d: recorder was destroyed");
+    return 0;
+  }
+  *out = b;
+  return 1;
+}
+
+/* recorderCreate(h, stride, quantum) -> recorder handle (scripts/record3d.ts:16-19) */
+static napi_value js_recorder_create(napi_env env, napi_callback_info info) {
+  napi_value argv[3], ext;
+  pl3d_sim *sim;
+  double stride, quantum;
+  pl3d_recorder *rec = NULL;
+  if (!args(env, info, 3, argv) || !get_sim(env, argv[0], &sim) ||
+      !get_double(env, argv[1], &stride, "stride") || !get_double(env, argv[2], &quantum, "quantum"))
+    return NULL;
+  CHECK_PL3D(pl3d_recorder_create(sim, (int32_t)stride, quantum, &rec));
+  recorder_box *b = (recorder_box *)calloc(1, sizeof *b);
+  if (!b) {
+    pl3d_recorder_destroy(rec);
+    napi_throw_error(env, NULL, "pourlab3d: out of host memory");
+    return NULL;
+  }
+  b->rec = rec;
+  if (napi_create_reference(env, argv[0], 1, &b->sim_ref) != napi_ok ||
+      napi_create_external(env, b, finalize_recorder, NULL, &ext) != napi_ok) {
+    finalize_recorder(env, b, NULL);
+    napi_throw_error(env, NULL, "pourlab3d: cannot wrap the recorder");
+    return NULL;
+  }
+  return ext;
+}
+
+/* recorderParcels(r) -> {count, origin: Uint8Array(count)} */
+static napi_value js_recorder_parcels(napi_env env, napi_callback_info info) {
+  napi_value argv[1], o, cv;
+  recorder_box *b;
+  int32_t count = 0;
+  void *p;
+  if (!args(env, info, 1, argv) || !get_recorder(env, argv[0], &b)) return NULL;
+  CHECK_PL3D(pl3d_recorder_parcels(b->rec, &count, NULL));
+  napi_value a = new_typed(env, napi_uint8_array, 1, (size_t)count, &p);
+  if (!a) return NULL;
+  CHECK_PL3D(pl3d_recorder_parcels(b->rec, NULL, (uint8_t *)p));
+  CHECK_NAPI(napi_create_object(env, &o));
+  CHECK_NAPI(napi_create_int32(env, count, &cv));
+  if (!set_val(env, o, "count", cv) || !set_val(env, o, "origin", a)) return NULL;
+  return o;
+}
+
+/* recorderCapture(r, count) -> Uint16Array(3 count): the next frame's words (u16 values for the
+ * first frame, int16 differences after it; encodeRecording, format.ts:56-74) */
+static napi_value js_recorder_capture(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  recorder_box *b;
+  double count;
+  void *p;
+  if (!args(env, info, 2, argv) || !get_recorder(env, argv[0], &b) ||
+      !get_double(env, argv[1], &count, "count"))
+    return NULL;
+  napi_value a = new_typed(env, napi_uint16_array, 2, 3 * (size_t)count, &p);
+  if (!a) return NULL;
+  CHECK_PL3D(pl3d_recorder_capture(b->rec, (uint16_t *)p));
+  return a;
+}
+
+static napi_value js_recorder_destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  recorder_box *b = NULL;
+  if (!args(env, info, 1, argv)) return NULL;
+  CHECK_NAPI(napi_get_value_external(env, argv[0], (void **)&b));
+  if (b && b->rec) {
+    pl3d_recorder *r = b->rec;
+    b->rec = NULL;
+    CHECK_PL3D(pl3d_recorder_destroy(r));
+  }
+  if (b && b->sim_ref) {
+    CHECK_NAPI(napi_delete_reference(env, b->sim_ref));
+    b->sim_ref = NULL;
+  }
+  return NULL;
+}
+
 /* pcgSolve(nx, ny, nz, cell: Uint8Array, b: Float64Array, q: Float64Array, tol, maxIter,
  * preconditioner) -> {iterations, converged, residual, trueResidual, liquidCells, kernelMs}
  * (Pcg3D.solve, solver3d.ts:858-937) */
@@ -702,6 +809,10 @@ static napi_value init(napi_env env, napi_value exports) {
       {"readAffine", js_read_affine},
       {"setTarget", js_set_target},
       {"pcgSolve", js_pcg_solve},
+      {"recorderCreate", js_recorder_create},
+      {"recorderParcels", js_recorder_parcels},
+      {"recorderCapture", js_recorder_capture},
+      {"recorderDestroy", js_recorder_destroy},
   };
   for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
     napi_value f;

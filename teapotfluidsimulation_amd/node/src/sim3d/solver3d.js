@@ -179,6 +179,31 @@ class Simulation3D {
   }
 }
 
+class Recorder3D {
+  // Device-side frame encoder of a recording (capture/encode of scripts/record3d.ts:23-36 and
+  // encodeRecording, format.ts:46-76): every `stride`-th parcel, quantised to `quantum` metres.
+  // capture() returns the next frame's Uint16Array(3P): u16 values for the first frame, int16
+  // differences after it.  The simulation cannot be destroyed while its recorder is live.
+  constructor(sim, stride = 3, quantum = 0.00025) {
+    this._lib = load();
+    this.stride = stride;
+    this.quantum = quantum;
+    this._r = this._lib.recorderCreate(sim._h, stride, quantum);
+    const p = this._lib.recorderParcels(this._r);
+    this.parcels = p.count;
+    this.origin = p.origin;
+  }
+
+  capture() {
+    return this._lib.recorderCapture(this._r, this.parcels);
+  }
+
+  destroy() {
+    if (this._r) this._lib.recorderDestroy(this._r);
+    this._r = null;
+  }
+}
+
 class Pcg3D {
   // Pcg3D (solver3d.ts:827-974): matrix-free 7-point PCG on the FLUID cells, on the GPU.
   constructor(nx, ny, nz, preconditioner = 'mic0') {
@@ -194,4 +219,4 @@ class Pcg3D {
   }
 }
 
-module.exports = { Simulation3D, Pcg3D, IN_POT, DISCHARGED, SPILLED, LOST, AIR, FLUID, SOLID, STAT };
+module.exports = { Simulation3D, Recorder3D, Pcg3D, IN_POT, DISCHARGED, SPILLED, LOST, AIR, FLUID, SOLID, STAT };

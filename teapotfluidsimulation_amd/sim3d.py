@@ -266,6 +266,53 @@ class Pcg3D:
         }
 
 
+class Recorder3D:
+    """Device-side frame encoder of a recording (the capture/encode half of scripts/record3d.ts
+    and encodeRecording, src/view3d/format.ts:46-76): every `stride`-th parcel, positions
+    quantised to `quantum` metres; capture() returns the next frame's 3P 16-bit words (u16
+    values for the first frame, int16 differences after it) as they go into the file."""
+
+    def __init__(self, sim: "Simulation3D", stride: int = 3, quantum: float = 0.00025):
+        self.sim = sim
+        self.stride = int(stride)
+        self.quantum = float(quantum)
+        h = C.c_void_p()
+        check(load().pl3d_recorder_create(sim._handle(), self.stride, self.quantum, C.byref(h)))
+        self._h = h
+        cnt = C.c_int32()
+        check(load().pl3d_recorder_parcels(self._h, C.byref(cnt), None))
+        self.parcels = cnt.value
+        self.frames = 0
+
+    def _handle(self):
+        if not getattr(self, "_h", None):
+            raise RuntimeError("recorder was destroyed")
+        return self._h
+
+    @property
+    def origin(self) -> np.ndarray:
+        out = np.zeros(self.parcels, np.uint8)
+        check(load().pl3d_recorder_parcels(self._handle(), None, _ptr(out)))
+        return out
+
+    def capture(self) -> np.ndarray:
+        out = np.zeros(3 * self.parcels, np.uint16)
+        check(load().pl3d_recorder_capture(self._handle(), _ptr(out)))
+        self.frames += 1
+        return out
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            check(load().pl3d_recorder_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class Ensemble3D:
     """Independent pours, possibly on different grids, stepped together on one device: each
     substep kernel is one launch for all members.  Each member gives exactly the results of a

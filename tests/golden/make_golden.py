@@ -6,6 +6,8 @@ Reads (as data only; nothing of the reference is imported or executed):
                                                    mL per layer per frame, pivotWorld, domain) and
                                                    pour3d_5mm_frames.npz (origin band and quantised
                                                    positions of every 3rd parcel at t = 0, 0.1, ...)
+                                                   and pour3d_5mm_bytes.json (lengths and SHA-256
+                                                   of the header bytes and the raw file)
 The recording layout is u32 headerLength | header JSON | u8 origin[P] | u16 frame0[3P] |
 i16 delta[F-1][3P], little endian, positions quantised to header.quantum metres
 (src/view3d/format.ts:5-7).
@@ -13,6 +15,7 @@ i16 delta[F-1][3P], little endian, positions quantised to header.quantum metres
 Usage: python tests/golden/make_golden.py [/root/reference]
 """
 import gzip
+import hashlib
 import json
 import os
 import struct
@@ -53,13 +56,18 @@ def main(ref="/root/reference"):
             kept[k] = cur.copy()
     with open(os.path.join(HERE, "pour3d_5mm_header.json"), "w") as f:
         json.dump(header, f, indent=1)
+    # the header's exact bytes (JSON.stringify output) and the whole file's digest, so a writer
+    # can be checked byte for byte against what the reference script wrote
+    with open(os.path.join(HERE, "pour3d_5mm_bytes.json"), "w") as f:
+        json.dump({"headerLength": hl, "headerSha256": hashlib.sha256(raw[4:4 + hl]).hexdigest(),
+                   "rawLength": len(raw), "rawSha256": hashlib.sha256(raw).hexdigest()}, f, indent=1)
     np.savez_compressed(
         os.path.join(HERE, "pour3d_5mm_frames.npz"),
         origin=np.array(origin),
         frames=np.array(sorted(kept)),
         positions=np.stack([kept[k].reshape(P, 3).astype(np.uint16) for k in sorted(kept)]),
     )
-    print("wrote pins.json, pour3d_5mm_header.json, pour3d_5mm_frames.npz;", len(kept), "frames")
+    print("wrote pins.json, pour3d_5mm_header.json, pour3d_5mm_frames.npz, pour3d_5mm_bytes.json;", len(kept), "frames")
 
 
 if __name__ == "__main__":

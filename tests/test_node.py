@@ -21,7 +21,8 @@ pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
 
 EXPORTS = ["version", "gentlePour", "create", "destroy", "info", "step", "advance", "diagnostics",
            "stableDt", "finished", "dischargedFractions", "stats", "readState", "dischargeOrder",
-           "readParticles", "readRenderPositions", "readAffine", "setTarget", "pcgSolve"]
+           "readParticles", "readRenderPositions", "readAffine", "setTarget", "pcgSolve",
+           "recorderCreate", "recorderParcels", "recorderCapture", "recorderDestroy"]
 
 
 def node_json(script: str):
@@ -68,3 +69,31 @@ console.log(JSON.stringify(res));
     assert out["handle"].startswith("TypeError") and "handle" in out["handle"]
     assert out["sizes"].startswith("RangeError")
     assert out["types"].startswith("TypeError") and "cell" in out["types"]
+
+
+def test_recording_writer_and_reader_agree_with_python_host(tmp_path):
+    """src/view3d/format.js writes the same bytes as record3d.encode_recording for the same header
+    and frame words, and its decodeRecording reads them back as the Python reader does."""
+    import numpy as np
+    from teapotfluidsimulation_amd import record3d as R
+    rng = np.random.default_rng(3)
+    P = 7
+    h = {"version": 1, "frames": 3, "parcels": P, "fps": 10, "quantum": 0.00025, "cellMm": 5,
+         "pivot": [0, 0.06], "angles": [0, 1e-7, 12.5], "label": "t"}
+    origin = rng.integers(0, 3, P).astype(np.uint8)
+    words = [rng.integers(0, 65536, 3 * P).astype(np.uint16) for _ in range(3)]
+    raw = R.encode_recording(h, origin, words)
+    spec = tmp_path / "in.json"
+    spec.write_text(json.dumps({"header": h, "origin": origin.tolist(),
+                                "words": [w.tolist() for w in words]}))
+    out = tmp_path / "out.bin"
+    script = (f"const f=require('./src/view3d/format');const s=JSON.parse(require('fs')"
+              f".readFileSync('{spec}','utf8'));const b=f.writeRecording(s.header,"
+              f"Uint8Array.from(s.origin),s.words.map(w=>Uint16Array.from(w)));"
+              f"require('fs').writeFileSync('{out}',b);const d=f.decodeRecording(b);"
+              f"console.log(JSON.stringify({{frames:d.header.frames,last:Array.from(d.positions[2])}}))")
+    js = node_json(script)
+    assert out.read_bytes() == raw
+    _, _, pos = R.decode_recording(raw)
+    assert js["frames"] == 3
+    assert np.array_equal(np.array(js["last"], np.float32), pos[2])
