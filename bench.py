@@ -29,8 +29,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per liquid cell per PCG iteration (FP32 storage): MIC(0) = stencil 9 + vector
 # updates and max|r| 24 + forward sweep 13 + backward sweep and z.r 17 + s update 12; Jacobi drops
-# the sweeps and reads r, diag and writes z (9).
-BYTES_PER_CELL_ITER = {"mic0": 75.0, "jacobi": 54.0}
+# the sweeps and reads r, diag and writes z (9); Chebyshev of degree d adds a d write to the first
+# step (4) and, for each further step, the stencil on z with its flag (5), r (4), d read and
+# written (8) and z written (4); MG V(2,2) over 4 levels about 89 plus the CG vector work 53.
+BYTES_PER_CELL_ITER = {"mic0": 75.0, "jacobi": 54.0, "chebyshev": 58.0 + 21.0 * (6 - 1), "mg": 142.0}
 # HBM bytes per k_pcg_jacobi launch on the default workload (4 mm, APIC, Jacobi-PCG), measured
 # with rocprofv3 FETCH_SIZE and WRITE_SIZE and calibrated on the library's copy kernel (FETCH x
 # 2.0, WRITE x 1.0): profiles/pour4mm_apic_jacobi_hbm_traffic.md.  The solve keeps its vectors in
@@ -256,7 +258,7 @@ def main():
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cell", type=float, default=4.0)
-    ap.add_argument("--precond", default="jacobi", choices=["jacobi", "mic0"])
+    ap.add_argument("--precond", default="jacobi", choices=["jacobi", "mic0", "chebyshev", "mg"])
     ap.add_argument("--transfer", default=None, choices=["apic", "flip"],
                     help="default: apic for the 4 mm pour, flip for --ensemble (APIC is not "
                          "stable in the fast-tilting varied pours)")

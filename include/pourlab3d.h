@@ -82,6 +82,11 @@ extern "C" {
 #define PL3D_PRECOND_MIC0 0
 #define PL3D_PRECOND_MG 1
 #define PL3D_PRECOND_JACOBI 2
+/* a fixed Chebyshev polynomial in D^-1 A, the preconditioner of the reference's GPU solver
+ * (gpu3d/shaders.ts:12-15, :588-620) */
+#define PL3D_PRECOND_CHEBYSHEV 3
+#define PL3D_CHEB_DEGREE_DEFAULT 6
+#define PL3D_CHEB_LOWER_DEFAULT 0.05
 
 /* particle-to-grid transfer (reference: FLIP/PIC blend only, solver3d.ts:607-624) */
 #define PL3D_TRANSFER_FLIP 0
@@ -116,7 +121,7 @@ typedef struct pl3d_motion_config {
   double settle_time;
 } pl3d_motion_config;
 
-/* Config3D (sim3d/solver3d.ts:25-43) plus two device-only choices at the end */
+/* Config3D (sim3d/solver3d.ts:25-43) plus device-only choices at the end */
 typedef struct pl3d_config {
   pl3d_teapot_config teapot;
   pl3d_motion_config motion;
@@ -134,6 +139,8 @@ typedef struct pl3d_config {
   double duration;
   int32_t preconditioner; /* PL3D_PRECOND_* (device solver only) */
   int32_t transfer;       /* PL3D_TRANSFER_* */
+  int32_t cheb_degree;    /* Chebyshev preconditioner degree, 0: PL3D_CHEB_DEGREE_DEFAULT */
+  double cheb_lower;      /* its interval [cheb_lower, 2] of D^-1 A, 0: PL3D_CHEB_LOWER_DEFAULT */
 } pl3d_config;
 
 /* Diagnostics3D (sim3d/solver3d.ts:45-60) */
@@ -333,7 +340,7 @@ int pl3d_poisson_solution(pl3d_poisson *ps, int32_t precision, int32_t k, double
  * Poisson system on the FLUID cells of `cell` (AIR = p 0 ghost, SOLID = Neumann), host arrays of
  * nx*ny*nz entries, stop rule max|r| <= max(tol * max|b|, 1e-14).  The solve runs on the device in
  * FP32 with FP64 dot products and FP64 true-residual refinement; q receives the solution (0 off the
- * liquid).  precond is a PL3D_PRECOND_* value. */
+ * liquid).  precond is a PL3D_PRECOND_* value (Chebyshev with the default degree and interval). */
 int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *b,
                    double *q, double tol, int32_t max_iter, int32_t precond,
                    pl3d_solve_result *out);

@@ -16,7 +16,7 @@ IN_POT, DISCHARGED, SPILLED, LOST = 0, 1, 2, 3
 
 MOTION_KINDS = {"still": 0, "pour": 1, "tilt": 2, "manual": 3}
 PHASES = ["wait", "tilt-in", "hold", "return", "rest", "done", "manual"]
-PRECONDITIONERS = {"mic0": 0, "mg": 1, "jacobi": 2}
+PRECONDITIONERS = {"mic0": 0, "mg": 1, "jacobi": 2, "chebyshev": 3}
 TRANSFERS = {"flip": 0, "apic": 1}
 
 STAT = {
@@ -88,6 +88,8 @@ class Config(C.Structure):
         ("duration", C.c_double),
         ("preconditioner", C.c_int32),
         ("transfer", C.c_int32),
+        ("cheb_degree", C.c_int32),
+        ("cheb_lower", C.c_double),
     ]
 
 
@@ -218,6 +220,8 @@ def config_to_dict(c: Config) -> dict:
         "duration": c.duration,
         "preconditioner": {v: k for k, v in PRECONDITIONERS.items()}.get(c.preconditioner, "mic0"),
         "transfer": {v: k for k, v in TRANSFERS.items()}.get(c.transfer, "flip"),
+        "chebDegree": c.cheb_degree,
+        "chebLower": c.cheb_lower,
     }
 
 
@@ -260,4 +264,6 @@ def config_from_dict(d: dict) -> Config:
     c.duration = d["duration"]
     c.preconditioner = PRECONDITIONERS[d.get("preconditioner", "mic0")]
     c.transfer = TRANSFERS[d.get("transfer", "flip")]
+    c.cheb_degree = int(d.get("chebDegree", 0))
+    c.cheb_lower = float(d.get("chebLower", 0.0))
     return c

@@ -284,8 +284,7 @@ int enqueue_substep(const LaunchSet &L) {
     hipEvent_t a = nullptr;
     if (L.timed)
       if ((rc = timed_begin(L.timed, &a))) return rc;
-    hipLaunchKernelGGL(L.precond == 2 ? k_pcg_jacobi : (L.precond == 1 ? k_pcg_mg : k_pcg_mic0),
-                       dim3(2, L.members), dim3(PL3D_SOLVE_THREADS), 0, L.stream, L.tA, L.tD);
+    hipLaunchKernelGGL(pcg_kernel(L.precond), dim3(2, L.members), dim3(PL3D_SOLVE_THREADS), 0, L.stream, L.tA, L.tD);
     HIP_OK(hipGetLastError());
     if (L.timed)
       if ((rc = timed_end(L.timed, 6, a))) return rc;
@@ -440,8 +439,10 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       c.pressure_max_iterations < 0 || !(c.teapot.radius > 0))
     return fail(PL3D_EINVAL, "invalid Config3D");
   if (c.preconditioner != PL3D_PRECOND_MIC0 && c.preconditioner != PL3D_PRECOND_JACOBI &&
-      c.preconditioner != PL3D_PRECOND_MG)
+      c.preconditioner != PL3D_PRECOND_MG && c.preconditioner != PL3D_PRECOND_CHEBYSHEV)
     return fail(PL3D_EINVAL, "preconditioner %d is not available", c.preconditioner);
+  if (c.cheb_degree < 0 || c.cheb_degree > 64 || !(c.cheb_lower >= 0.0 && c.cheb_lower < 2.0))
+    return fail(PL3D_EINVAL, "Chebyshev degree must be in 0..64 and its lower bound in [0, 2)");
   if (c.transfer != PL3D_TRANSFER_FLIP && c.transfer != PL3D_TRANSFER_APIC)
     return fail(PL3D_EINVAL, "transfer %d is not available", c.transfer);
   auto *s = new pl3d_sim();
@@ -671,8 +672,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     A.plane = B.plane;
     A.stride = P.nc;
     A.planes = P.nx + P.ny + P.nz - 2;
-    A.precond = c.preconditioner == PL3D_PRECOND_JACOBI ? 2
-                : (c.preconditioner == PL3D_PRECOND_MG ? 1 : 0);
+    A.precond = c.preconditioner;
     A.overflow = &B.ctl->overflow;
     A.cellIterations = &B.ctl->cellIterations;
     A.solveCount = &B.ctl->solveCount;
@@ -718,6 +718,10 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   descs[1].porder = B.porder + NC;
   descs[1].pc2 = B.pc2 + NC;
   descs[1].nbrPlane = B.nbrPlane + 3 * NC;
+  for (PcgDesc &d : descs) {
+    d.chebDegree = c.cheb_degree > 0 ? c.cheb_degree : PL3D_CHEB_DEGREE_DEFAULT;
+    d.chebLower = float(c.cheb_lower > 0.0 ? c.cheb_lower : PL3D_CHEB_LOWER_DEFAULT);
+  }
   if (s->mg) {
     descs[0].mg = s->mg;
     descs[1].mg = s->mg + pl3d_mg_layout(nx, ny, nz).bytes;

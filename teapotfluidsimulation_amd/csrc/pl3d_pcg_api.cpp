@@ -31,7 +31,7 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   if (!cell || !b || !q || nx < 3 || ny < 3 || nz < 3 || max_iter < 0)
     return perr(PL3D_EINVAL, "bad arguments");
   if (precond != PL3D_PRECOND_MIC0 && precond != PL3D_PRECOND_JACOBI &&
-      precond != PL3D_PRECOND_MG)
+      precond != PL3D_PRECOND_MG && precond != PL3D_PRECOND_CHEBYSHEV)
     return perr(PL3D_EINVAL, "preconditioner not available");
   const size_t nc = size_t(nx) * ny * nz;
   const int nxy = nx * ny;
@@ -137,6 +137,8 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   D.pc2 = reinterpret_cast<float *>(mem + o_pc2);
   D.nbrPlane = reinterpret_cast<uint32_t *>(mem + o_nbp);
   D.mg = precond == PL3D_PRECOND_MG ? reinterpret_cast<unsigned char *>(mem + o_mg) : nullptr;
+  D.chebDegree = PL3D_CHEB_DEGREE_DEFAULT;
+  D.chebLower = float(PL3D_CHEB_LOWER_DEFAULT);
   PcgShared A;
   A.m = &dres->m;
   A.nbr = reinterpret_cast<uint32_t *>(mem + o_nbr);
@@ -145,7 +147,7 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   A.plane = reinterpret_cast<uint16_t *>(mem + o_plane);
   A.stride = stride;
   A.planes = nx + ny + nz - 2;
-  A.precond = precond == PL3D_PRECOND_JACOBI ? 2 : (precond == PL3D_PRECOND_MG ? 1 : 0);
+  A.precond = precond;
   A.overflow = nullptr;
   A.cellIterations = nullptr;
   A.solveCount = nullptr;
@@ -169,8 +171,7 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if ((e = hipEventCreate(&e0)) || (e = hipEventCreate(&e1)) || (e = hipEventRecord(e0, st)))
     return done(perr(PL3D_EDEVICE, std::string("events: ") + hipGetErrorString(e)));
-  hipLaunchKernelGGL(A.precond == 2 ? k_pcg_jacobi : (A.precond == 1 ? k_pcg_mg : k_pcg_mic0),
-                     dim3(1),
+  hipLaunchKernelGGL(pcg_kernel(A.precond), dim3(1),
                      dim3(PL3D_SOLVE_THREADS), 0, st,
                      reinterpret_cast<const PcgShared *>(mem + o_shared),
                      reinterpret_cast<const PcgDesc *>(mem + o_desc));

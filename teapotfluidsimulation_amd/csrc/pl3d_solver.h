@@ -18,7 +18,7 @@ struct PcgShared {
   const uint16_t *plane;   // i + j + k
   int stride;              // capacity of the list arrays (>= m)
   int planes;              // number of hyperplanes (nx + ny + nz - 2)
-  int precond;             // 0 = MIC(0), 2 = Jacobi
+  int precond;             // PL3D_PRECOND_*: 0 MIC(0), 1 MG, 2 Jacobi, 3 Chebyshev
   int *overflow;           // set when m exceeds PL3D_SOLVE_MMAX
   unsigned long long *cellIterations;  // += m * iterations per solve (nullptr: not counted)
   unsigned long long *solveCount;
@@ -47,6 +47,8 @@ struct PcgDesc {
   float *pc2;
   uint32_t *nbrPlane;
   unsigned char *mg;  // multigrid scratch of pl3d_mg_scratch_bytes(nx, ny, nz) (MG only)
+  int chebDegree;     // Chebyshev only: polynomial degree (>= 1) and lower bound of the interval
+  float chebLower;    // [lower, 2] of D^-1 A it targets
 };
 
 // ---- multigrid (V(2,2) red-black Gauss-Seidel, up to PL3D_MG_LEVELS levels) -------------------
@@ -106,3 +108,12 @@ __global__ void k_pcg_mg(const PcgShared *At, const PcgDesc *Ds);
 // cells described by At[y].
 __global__ void k_pcg_mic0(const PcgShared *At, const PcgDesc *Ds);
 __global__ void k_pcg_jacobi(const PcgShared *At, const PcgDesc *Ds);
+__global__ void k_pcg_cheb(const PcgShared *At, const PcgDesc *Ds);
+
+// The PCG kernel of a preconditioner (PL3D_PRECOND_* value).
+inline decltype(&k_pcg_jacobi) pcg_kernel(int precond) {
+  return precond == 3 ? k_pcg_cheb
+         : precond == 2 ? k_pcg_jacobi
+         : precond == 1 ? k_pcg_mg
+                        : k_pcg_mic0;
+}

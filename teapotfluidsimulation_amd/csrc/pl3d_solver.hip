@@ -27,6 +27,9 @@
 //           whole hyperplanes i+j+k = l in increasing l (decreasing for the backward sweep) gives
 //           the same factor and the same triangular solves, with one barrier per hyperplane.
 //   Jacobi  z = r / diag, recomputed from the register-resident neighbour words where needed.
+//   Chebyshev  a fixed polynomial in D^-1 A (degree and lower spectral bound per solve; the
+//           reference's GPU solver uses degree 6 on [0.05, 2], gpu3d/shaders.ts:12-15): only
+//           stencil products and barriers, no reductions.
 #include <type_traits>
 
 #include "pl3d_device.h"
@@ -105,7 +108,7 @@ struct Red {
 // neighbours here, so the stencil product needs no branches.
 #define ZSLOT (MMAX - 1)
 
-enum { PC_MIC = 0, PC_MG = 1, PC_JAC = 2 };
+enum { PC_MIC = 0, PC_MG = 1, PC_JAC = 2, PC_CHEB = 3 };
 
 struct MicLds {
   int planeStart[PL3D_MAX_PLANES + 1];
@@ -758,11 +761,51 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     lds_barrier();
   };
 
+  // Chebyshev: the polynomial's recurrence constants (fixed for the solve, so the preconditioner
+  // is one fixed SPD operator and CG stays valid)
+  const int chebDeg = D.chebDegree;
+  const float chebTheta = 0.5f * (2.0f + D.chebLower), chebDelta = 0.5f * (2.0f - D.chebLower);
+
   // z = M r.  MIC(0): owners write W = r, the sweeps run in place, W = z afterwards (ends with a
-  // barrier).  MG: one V(2,2) cycle from zero, z = W (ends with a barrier).  Jacobi: nothing to
-  // do, z is recomputed as r / diag where it is used.
+  // barrier).  MG: one V(2,2) cycle from zero, z = W (ends with a barrier).  Chebyshev: deg steps
+  // of the Chebyshev iteration for D^-1 A on [lower, 2] from z = 0, z = W (ends with a barrier).
+  // Jacobi: nothing to do, z is recomputed as r / diag where it is used.
   auto precondition = [&]() {
-    if constexpr (PC == PC_MG) {
+    if constexpr (PC == PC_CHEB) {
+      // step 0: z = d = D^-1 r / theta; then for each further degree, with res = r - A z from
+      // the z held in W:  d <- (rho' rho) d + (2 rho' / delta) D^-1 res,  z <- z + d,
+      // rho' = 1 / (2 sigma1 - rho), sigma1 = theta / delta, rho_0 = 1 / sigma1.  The same
+      // recurrence as the reference's GPU preconditioner (gpu3d/shaders.ts:588-620), with res
+      // recomputed from z instead of carried, so only d needs registers.
+      const float sigma1 = chebTheta / chebDelta;
+      float rho = 1.0f / sigma1;
+      float dreg[CPT];
+      FOR_SLOTS({
+        const float d = r[k] * __builtin_amdgcn_rcpf(dgv(k) * chebTheta);
+        dreg[k] = d;
+        W[t] = d;
+      })
+      lds_barrier();
+      for (int deg = 1; deg < chebDeg; deg++) {
+        // opaque words: keep the neighbour decoding inside the loop (see the PCG loop)
+#pragma unroll
+        for (int k = 0; k < CPT; k++)
+          asm volatile("" : "+v"(nbc[k][0]), "+v"(nbc[k][1]), "+v"(nbc[k][2]));
+        const float rhoN = 1.0f / (2.0f * sigma1 - rho);
+        const float c1 = rhoN * rho, c2 = 2.0f * rhoN / chebDelta;
+        FOR_SLOTS({
+          const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
+          const float az = dgv(k) * W[t] - (((W[px >> 16] + W[px & 0x3FFFu]) +
+                                             (W[py >> 16] + W[py & 0x3FFFu])) +
+                                            (W[pz >> 16] + W[pz & 0x3FFFu]));
+          dreg[k] = c1 * dreg[k] + c2 * (r[k] - az) * __builtin_amdgcn_rcpf(dgv(k));
+        })
+        lds_barrier();  // every read of this z done
+        FOR_SLOTS({ W[t] += dreg[k]; })
+        lds_barrier();
+        rho = rhoN;
+      }
+    } else if constexpr (PC == PC_MG) {
       if (!L.x.ok) {  // hierarchy over capacity: Jacobi for this solve
         FOR_SLOTS({ W[t] = r[k] * __builtin_amdgcn_rcpf(dgv(k)); })
         lds_barrier();
@@ -832,7 +875,7 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     }
   };
   auto zval = [&](int k, int t) -> float {
-    if constexpr (MIC || PC == PC_MG) {
+    if constexpr (MIC || PC == PC_MG || PC == PC_CHEB) {
       return W[t];
     } else {
       return r[k] * __builtin_amdgcn_rcpf(dgv(k));
@@ -1024,4 +1067,9 @@ __global__ __launch_bounds__(NT) void k_pcg_jacobi(const PcgShared *__restrict__
                                                    const PcgDesc *__restrict__ Ds) {
   __shared__ SolveLds<PC_JAC> L;
   pcg_entry<PC_JAC>(L, At, Ds);
+}
+__global__ __launch_bounds__(NT) void k_pcg_cheb(const PcgShared *__restrict__ At,
+                                                 const PcgDesc *__restrict__ Ds) {
+  __shared__ SolveLds<PC_CHEB> L;
+  pcg_entry<PC_CHEB>(L, At, Ds);
 }

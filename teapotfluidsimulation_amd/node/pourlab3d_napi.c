@@ -121,7 +121,7 @@ static int get_opt_str(napi_env env, napi_value obj, const char *key, char *buf,
 }
 
 static const char *MOTION_KINDS[] = {"still", "pour", "tilt", "manual"};
-static const char *PRECONDS[] = {"mic0", "mg", "jacobi"};
+static const char *PRECONDS[] = {"mic0", "mg", "jacobi", "chebyshev"};
 static const char *TRANSFERS[] = {"flip", "apic"};
 
 static int name_index(const char *const *names, int n, const char *s) {
@@ -133,7 +133,8 @@ static int name_index(const char *const *names, int n, const char *s) {
 /* ------------------------------------------------------------------ Config3D <-> pl3d_config */
 
 /* Config3D (sim3d/solver3d.ts:25-43) as a JS object -> pl3d_config.  Device-only keys
- * 'preconditioner' and 'transfer' are optional strings. */
+ * 'preconditioner' and 'transfer' are optional strings, 'chebDegree' and 'chebLower' optional
+ * numbers (0 or absent: the library's defaults). */
 static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
   napi_value t, m, pv, x;
   double d;
@@ -205,10 +206,20 @@ static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
   {
     int r = get_opt_str(env, o, "preconditioner", buf, sizeof buf);
     if (r < 0) return 0;
-    c->preconditioner = r == 0 ? PL3D_PRECOND_MIC0 : name_index(PRECONDS, 3, buf);
+    c->preconditioner = r == 0 ? PL3D_PRECOND_MIC0 : name_index(PRECONDS, 4, buf);
     if (c->preconditioner < 0) {
-      napi_throw_type_error(env, NULL, "Config3D: preconditioner must be 'mic0', 'mg' or 'jacobi'");
+      napi_throw_type_error(env, NULL,
+                            "Config3D: preconditioner must be 'mic0', 'mg', 'jacobi' or 'chebyshev'");
       return 0;
+    }
+    bool has = false;
+    if (napi_has_named_property(env, o, "chebDegree", &has) == napi_ok && has) {
+      if (!get_num(env, o, "chebDegree", &d)) return 0;
+      c->cheb_degree = (int32_t)d;
+    }
+    if (napi_has_named_property(env, o, "chebLower", &has) == napi_ok && has) {
+      if (!get_num(env, o, "chebLower", &d)) return 0;
+      c->cheb_lower = d;
     }
     r = get_opt_str(env, o, "transfer", buf, sizeof buf);
     if (r < 0) return 0;
@@ -761,9 +772,9 @@ static napi_value js_pcg_solve(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, NULL, "pourlab3d: preconditioner must be a string");
     return NULL;
   }
-  const int pc = name_index(PRECONDS, 3, buf);
+  const int pc = name_index(PRECONDS, 4, buf);
   if (pc < 0) {
-    napi_throw_type_error(env, NULL, "pourlab3d: preconditioner must be 'mic0', 'mg' or 'jacobi'");
+    napi_throw_type_error(env, NULL, "pourlab3d: preconditioner must be 'mic0', 'mg', 'jacobi' or 'chebyshev'");
     return NULL;
   }
   const size_t cells = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];

@@ -1,6 +1,6 @@
 """Full pour on the GPU, as benchmarks/run3d.ts:9-38 drives the CPU solver.
 
-Usage: python -m teapotfluidsimulation_amd.run3d [cellMm] [seconds] [--precond mic0|jacobi]
+Usage: python -m teapotfluidsimulation_amd.run3d [cellMm] [seconds] [--precond mic0|jacobi|mg|chebyshev]
 Prints run3d.json-shaped JSON (result, dischargedFractions, wallSeconds, history).
 """
 import argparse

@@ -17,7 +17,7 @@ def _row(pins, cell):
 
 @pytest.mark.parametrize("cell,vol_tol,share_tol,precond", [
     (5, 0.01, 0.02, "mic0"), (4, 0.003, 0.013, "mic0"), (4, 0.003, 0.013, "jacobi"),
-    (4, 0.003, 0.013, "mg")])
+    (4, 0.003, 0.013, "mg"), (4, 0.003, 0.013, "chebyshev")])
 def test_full_pour_matches_cpu_reference(pins, cell, vol_tol, share_tol, precond):
     ref = _row(pins, cell)
     out = run_pour(cell, 20, precond)

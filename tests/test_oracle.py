@@ -370,6 +370,64 @@ def test_mg_pcg_converges_and_beats_jacobi():
     assert it <= rref.iterations + 2  # no worse than the reference's MIC(0)
 
 
+def test_chebyshev_preconditioner_closed_form():
+    """oracle/cheb3d.py's recurrence (gpu3d/shaders.ts:588-620) equals the closed form
+    z = q(D^-1 A) D^-1 r, q(x) = (1 - T_k((theta - x)/delta) / T_k(theta/delta)) / x, computed by
+    eigendecomposition of D^-1/2 A D^-1/2 on a pour snapshot's liquid cells; the spectrum of
+    D^-1 A lies in (0, 2] and q is positive there, so the preconditioner is SPD.  Degree 1 is
+    Jacobi scaled by 1/theta.  A wrong rho update, sign or D^-1 placement fails the first check."""
+    from oracle.cheb3d import cheb_apply, cheb_poly
+    from oracle.mg3d import level_operator
+    nx, ny, nz, cell, _ = _pour_snapshot()
+    # a sub-box of the snapshot keeps the dense eigenproblem small
+    c3 = cell.reshape(nz, ny, nx).copy()
+    c3[:, :, nx // 2:] = 2
+    c3[:, ny // 2:, :] = np.where(c3[:, ny // 2:, :] == 1, 0, c3[:, ny // 2:, :])
+    cell2 = c3.ravel()
+    _, _, A, dg, _ = level_operator(nx, ny, nz, cell2)
+    m = dg.size
+    assert 200 < m < 4000
+    Dh = 1.0 / np.sqrt(dg)
+    S = (Dh[:, None] * A.toarray()) * Dh[None, :]
+    lam, V = np.linalg.eigh(S)
+    assert lam.min() > 0 and lam.max() <= 2 + 1e-12
+    rng = np.random.default_rng(11)
+    for degree, lower in ((6, 0.05), (3, 0.2), (9, 0.01)):
+        assert np.all(cheb_poly(lam, degree, lower) > 0)
+        r = rng.standard_normal(m)
+        z = cheb_apply(A, dg, r, degree, lower)
+        zc = Dh * (V @ (cheb_poly(lam, degree, lower) * (V.T @ (Dh * r))))
+        assert np.abs(z - zc).max() <= 1e-10 * np.abs(zc).max(), (degree, lower)
+    r = rng.standard_normal(m)
+    assert np.allclose(cheb_apply(A, dg, r, 1, 0.05), r / (dg * 1.025), rtol=1e-15)
+
+
+def test_chebyshev_pcg_known_answers():
+    """The 8^3 column answers of tests/sim3d.test.ts:60-80 for the Chebyshev-PCG, its solution
+    against A q = b on a pour snapshot, and fewer iterations than the reference's MIC(0)."""
+    from oracle.cheb3d import cheb_pcg
+    n = 8
+    cell = np.zeros(n ** 3, np.uint8)
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                border = i in (0, n - 1) or j in (0, n - 1) or k in (0, n - 1)
+                cell[i + n * (j + n * k)] = 2 if border else (1 if j < 5 else 0)
+    b = np.where(cell == 1, np.sin(np.arange(n ** 3)), 0.0)
+    q, it, ok = cheb_pcg(n, n, n, cell, b, 1e-8, 200)
+    assert ok and it < 40
+    assert np.abs(b - O.apply_laplacian(n, n, n, cell, q))[cell == 1].max() <= 1e-8 * np.abs(b).max()
+    assert cheb_pcg(n, n, n, cell, np.zeros(n ** 3), 1e-8, 10)[1] == 0
+    assert cheb_pcg(n, n, n, cell, b, 1e-14, 1)[1:] == (1, False)
+    nx, ny, nz, cell, b = _pour_snapshot()
+    q, it, ok = cheb_pcg(nx, ny, nz, cell, b, 1e-6, 400)
+    assert ok
+    aq = O.apply_laplacian(nx, ny, nz, cell, q)
+    assert np.abs(b - aq)[cell == 1].max() <= 1e-6 * np.abs(b[cell == 1]).max()
+    _, rref = O.pcg_solve(nx, ny, nz, cell, b, 1e-6, 400)
+    assert it < rref.iterations
+
+
 def test_poisson_generator_oracle():
     """oracle/poisson3d.py: its mulberry32 equals the C++ oracle's (pinned by the recording), the
     separable clamped box filter equals a direct 27-point clamped mean, the flags give a liquid
