@@ -14,7 +14,10 @@ roofline object is computed for the kernel with the largest total time.  The CPU
 FP64 C++ oracle, single core, on a bounded sample of the same pour (same transfer).
 
 Usage: python bench.py [--gpus 1] [--steps 24] [--warmup 1] [--transfer apic|flip]
-                       [--precond jacobi|mic0] [--no-cpu-baseline]
+                       [--precond tuned|jacobi|mic0|mg|chebyshev] [--autotune]
+                       [--no-cpu-baseline]
+The default solver settings are the autotuner's result for the workload when one is committed
+under teapotfluidsimulation_amd/tuned/ (--autotune re-tunes on the GPU before the timed run).
 """
 import argparse
 import json
@@ -32,7 +35,36 @@ HBM_PEAK_GBS = 8000.0
 # the sweeps and reads r, diag and writes z (9); Chebyshev of degree d adds a d write to the first
 # step (4) and, for each further step, the stencil on z with its flag (5), r (4), d read and
 # written (8) and z written (4); MG V(2,2) over 4 levels about 89 plus the CG vector work 53.
-BYTES_PER_CELL_ITER = {"mic0": 75.0, "jacobi": 54.0, "chebyshev": 58.0 + 21.0 * (6 - 1), "mg": 142.0}
+BYTES_PER_CELL_ITER = {"mic0": 75.0, "jacobi": 54.0, "mg": 142.0}
+
+
+def bytes_per_cell_iter(settings):
+    if settings["preconditioner"] == "chebyshev":
+        return 58.0 + 21.0 * (int(settings.get("chebDegree", 6)) - 1)
+    return BYTES_PER_CELL_ITER[settings["preconditioner"]]
+
+
+def solver_settings(a):
+    """The solver and brick settings of the run: --autotune tunes them now (untimed); 'tuned' reads
+    the committed autotuner result for this workload (teapotfluidsimulation_amd/tuned/), else the
+    named preconditioner with the library defaults."""
+    from teapotfluidsimulation_amd import autotune
+    if a.autotune:
+        from teapotfluidsimulation_amd import gentlePour3D
+        cfg = gentlePour3D(a.cell, 20)
+        cfg["transfer"] = a.transfer
+        t = autotune.tune(cfg, progress=True)
+        return {k: t[k] for k in autotune.TUNED_KEYS if k in t}, "autotuned in this run"
+    if a.precond == "tuned":
+        path = os.path.join(ROOT, "teapotfluidsimulation_amd", "tuned",
+                            f"pour{a.cell:g}mm_{a.transfer}.json")
+        if os.path.exists(path):
+            with open(path) as f:
+                t = json.load(f)
+            return ({k: t[k] for k in autotune.TUNED_KEYS if k in t},
+                    f"autotuner result {os.path.relpath(path, ROOT)}")
+        return dict(autotune.DEFAULT_SOLVER), "library defaults (no tuned file)"
+    return {"preconditioner": a.precond}, "command line"
 # HBM bytes per k_pcg_jacobi launch on the default workload (4 mm, APIC, Jacobi-PCG), measured
 # with rocprofv3 FETCH_SIZE and WRITE_SIZE and calibrated on the library's copy kernel (FETCH x
 # 2.0, WRITE x 1.0): profiles/pour4mm_apic_jacobi_hbm_traffic.md.  The solve keeps its vectors in
@@ -93,7 +125,7 @@ def ensemble_bench(a):
         cfgs = []
         for i in range(a.ensemble):
             c = variedPour3D(1, i)
-            c["preconditioner"] = a.precond
+            c.update(a.settings)
             c["transfer"] = a.transfer
             cfgs.append(c)
         return Ensemble3D(cfgs)
@@ -161,7 +193,7 @@ def ensemble_bench(a):
                         f"{min(parcels):,}-{max(parcels):,} parcels, "
                         f"{'APIC' if a.transfer == 'apic' else 'PIC/FLIP 0.95'}",
             "step": "advance(0.5 s) of every incomplete member",
-            "preconditioner": a.precond,
+            "preconditioner": a.settings["preconditioner"],
         },
         "simulated_s_total": sum(sim_time),
         "substeps_total": sum(substeps),
@@ -258,7 +290,10 @@ def main():
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cell", type=float, default=4.0)
-    ap.add_argument("--precond", default="jacobi", choices=["jacobi", "mic0", "chebyshev", "mg"])
+    ap.add_argument("--precond", default="tuned",
+                    choices=["tuned", "jacobi", "mic0", "chebyshev", "mg"])
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the P2G brick and the solver on this GPU before the timed run")
     ap.add_argument("--transfer", default=None, choices=["apic", "flip"],
                     help="default: apic for the 4 mm pour, flip for --ensemble (APIC is not "
                          "stable in the fast-tilting varied pours)")
@@ -281,6 +316,7 @@ def main():
         return 2
     if a.transfer is None:
         a.transfer = "flip" if a.ensemble > 0 else "apic"
+    a.settings, a.settings_source = solver_settings(a)
     if a.ensemble > 0:
         return ensemble_bench(a)
     if a.poisson:
@@ -290,7 +326,7 @@ def main():
 
     def make():
         cfg = gentlePour3D(a.cell, 20)
-        cfg["preconditioner"] = a.precond
+        cfg.update(a.settings)
         cfg["transfer"] = a.transfer
         return Simulation3D(cfg)
 
@@ -339,8 +375,9 @@ def main():
         if "k_pcg" in kernels:
             pressure_ms = kernels["k_pcg"]["total_ms"] / max(kernels["k_pcg"]["launches"], 1)
         if dom == "k_pcg":
-            bytes_per_launch = BYTES_PER_CELL_ITER[a.precond] * rc["cellIterations"] / launches
-            unit_note = (f"{BYTES_PER_CELL_ITER[a.precond]:.0f} B per liquid cell per PCG "
+            bpc = bytes_per_cell_iter(a.settings)
+            bytes_per_launch = bpc * rc["cellIterations"] / launches
+            unit_note = (f"{bpc:.0f} B per liquid cell per PCG "
                          f"iteration x {rc['cellIterations'] / launches:.0f} cell-iterations per "
                          "launch (pressure + correction solves)")
         else:
@@ -348,7 +385,7 @@ def main():
             unit_note = "no byte model for this kernel"
         if bytes_per_launch is not None:
             achieved = bytes_per_launch / avg_s / 1e9
-            traffic = MEASURED_TRAFFIC.get((a.transfer, a.precond, a.cell))
+            traffic = MEASURED_TRAFFIC.get((a.transfer, a.settings["preconditioner"], a.cell))
             roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "traffic_source": "profiles/pour4mm_apic_jacobi_hbm_traffic.md"
@@ -382,7 +419,8 @@ def main():
                         "correction 1e-4",
             "transfer": a.transfer,
             "step": "advance(0.5 s)",
-            "preconditioner": a.precond,
+            "preconditioner": a.settings["preconditioner"],
+            "solver_settings": dict(a.settings, source=a.settings_source),
             "precision": "FP32 fields and parcels, FP64 dot products and reliable-update refinement",
             "baseline": "reference WebGPU FP32 4 mm pour, 0.87 sim-s/wall-s (RTX A2000 Laptop; "
                         "the reference's 3D pour is FLIP only, it has no APIC figure)",

@@ -141,6 +141,7 @@ typedef struct pl3d_config {
   int32_t transfer;       /* PL3D_TRANSFER_* */
   int32_t cheb_degree;    /* Chebyshev preconditioner degree, 0: PL3D_CHEB_DEGREE_DEFAULT */
   double cheb_lower;      /* its interval [cheb_lower, 2] of D^-1 A, 0: PL3D_CHEB_LOWER_DEFAULT */
+  int32_t p2g_brick;      /* P2G brick shape, an index below pl3d_p2g_bricks() (0: 8x8x4) */
 } pl3d_config;
 
 /* Diagnostics3D (sim3d/solver3d.ts:45-60) */
@@ -344,6 +345,13 @@ int pl3d_poisson_solution(pl3d_poisson *ps, int32_t precision, int32_t k, double
 int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *b,
                    double *q, double tol, int32_t max_iter, int32_t precond,
                    pl3d_solve_result *out);
+
+/* ---- tuning --------------------------------------------------------------------------------- */
+
+/* The P2G brick shapes this library is built with: returns their count; when shapes is not NULL
+ * it receives 4 int32 per shape (tx, ty, tz, threads per block), at most cap shapes.  No
+ * counterpart in the reference (its scatter has one fixed workgroup shape). */
+int pl3d_p2g_bricks(int32_t *shapes, int32_t cap);
 
 /* ---- measurement ---------------------------------------------------------------------------- */
 

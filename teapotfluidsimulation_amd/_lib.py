@@ -42,6 +42,7 @@ SIGNATURES = {
     "pl3d_copy_probe": (C.c_int, [C.c_int64, _i32, _P(_d)]),
     "pl3d_counters": (C.c_int, [_vp, _P(C.c_int64)]),
     "pl3d_varied_pour": (C.c_int, [C.c_uint32, _i32, _P(abi.Config)]),
+    "pl3d_p2g_bricks": (C.c_int, [_vp, _i32]),
     "pl3d_recorder_create": (C.c_int, [_vp, _i32, _d, _P(_vp)]),
     "pl3d_recorder_parcels": (C.c_int, [_vp, _P(_i32), _vp]),
     "pl3d_recorder_capture": (C.c_int, [_vp, _vp]),

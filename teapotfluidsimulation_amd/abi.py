@@ -90,6 +90,7 @@ class Config(C.Structure):
         ("transfer", C.c_int32),
         ("cheb_degree", C.c_int32),
         ("cheb_lower", C.c_double),
+        ("p2g_brick", C.c_int32),
     ]
 
 
@@ -222,6 +223,7 @@ def config_to_dict(c: Config) -> dict:
         "transfer": {v: k for k, v in TRANSFERS.items()}.get(c.transfer, "flip"),
         "chebDegree": c.cheb_degree,
         "chebLower": c.cheb_lower,
+        "p2gBrick": c.p2g_brick,
     }
 
 
@@ -266,4 +268,5 @@ def config_from_dict(d: dict) -> Config:
     c.transfer = TRANSFERS[d.get("transfer", "flip")]
     c.cheb_degree = int(d.get("chebDegree", 0))
     c.cheb_lower = float(d.get("chebLower", 0.0))
+    c.p2g_brick = int(d.get("p2gBrick", 0))
     return c

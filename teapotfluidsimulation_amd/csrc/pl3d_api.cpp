@@ -253,12 +253,13 @@ struct LaunchSet {
   int gridN, gridC, gridF, gridFC, gridExtrap, gridScan;
   int precond;
   pl3d_sim *timed;  // timing mode of a single simulation, else nullptr
+  int brick;        // index into p2g_bricks
 };
 
 LaunchSet single_set(pl3d_sim *s) {
   return LaunchSet{s->stream, s->dB, s->dP, s->dA, s->B.descs, 1, s->gridN, s->gridC, s->gridF,
                    s->gridFC, s->gridExtrap, s->gridScan, s->A.precond,
-                   s->timing ? s : nullptr};
+                   s->timing ? s : nullptr, s->cfg.p2g_brick};
 }
 
 template <class K>
@@ -278,7 +279,8 @@ int enqueue_substep(const LaunchSet &L) {
   if ((rc = launch(L, 1, k_count, L.gridN, STEP_THREADS))) return rc;
   if ((rc = launch(L, 2, k_classify_scan, L.gridScan, STEP_THREADS))) return rc;
   if ((rc = launch(L, 3, k_scatter, L.gridN, STEP_THREADS))) return rc;
-  if ((rc = launch(L, 4, k_p2g, L.gridFC, P2G_THREADS))) return rc;
+  if ((rc = launch(L, 4, p2g_bricks[L.brick].kernel, L.gridFC, p2g_bricks[L.brick].threads)))
+    return rc;
   if ((rc = launch(L, 5, k_liquid, L.gridC, STEP_THREADS))) return rc;
   {
     hipEvent_t a = nullptr;
@@ -443,6 +445,8 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     return fail(PL3D_EINVAL, "preconditioner %d is not available", c.preconditioner);
   if (c.cheb_degree < 0 || c.cheb_degree > 64 || !(c.cheb_lower >= 0.0 && c.cheb_lower < 2.0))
     return fail(PL3D_EINVAL, "Chebyshev degree must be in 0..64 and its lower bound in [0, 2)");
+  if (c.p2g_brick < 0 || c.p2g_brick >= PL3D_P2G_BRICKS)
+    return fail(PL3D_EINVAL, "p2g_brick must be in 0..%d", PL3D_P2G_BRICKS - 1);
   if (c.transfer != PL3D_TRANSFER_FLIP && c.transfer != PL3D_TRANSFER_APIC)
     return fail(PL3D_EINVAL, "transfer %d is not available", c.transfer);
   auto *s = new pl3d_sim();
@@ -657,8 +661,11 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   s->gridN = (n + STEP_THREADS - 1) / STEP_THREADS;
   s->gridC = (P.nc + STEP_THREADS - 1) / STEP_THREADS;
   s->gridF = (P.nfaces + STEP_THREADS - 1) / STEP_THREADS;
-  s->gridFC = ((nx + P2G_TX) / P2G_TX) * ((ny + P2G_TY) / P2G_TY) *
-              ((nz + P2G_TZ) / P2G_TZ);  // bricks covering the (n+1)-sized face index space
+  {
+    const P2gBrick &bk = p2g_bricks[c.p2g_brick];
+    s->gridFC = ((nx + bk.tx) / bk.tx) * ((ny + bk.ty) / bk.ty) *
+                ((nz + bk.tz) / bk.tz);  // bricks covering the (n+1)-sized face index space
+  }
   s->gridExtrap = (((P.nfaces + 63) & ~63) + P.nc + STEP_THREADS - 1) / STEP_THREADS;
   s->gridScan = P.scanBlocks;
   P.gridN = s->gridN;
@@ -1130,9 +1137,12 @@ int pl3d_ensemble_create(const pl3d_config *cfgs, int32_t count, pl3d_ensemble *
   g_err.clear();
   if (!cfgs || !out || count <= 0) return fail(PL3D_EINVAL, "bad ensemble arguments");
   *out = nullptr;
-  for (int32_t i = 1; i < count; i++)
+  for (int32_t i = 1; i < count; i++) {
     if (cfgs[i].preconditioner != cfgs[0].preconditioner)
       return fail(PL3D_EINVAL, "ensemble members must share one preconditioner");
+    if (cfgs[i].p2g_brick != cfgs[0].p2g_brick)
+      return fail(PL3D_EINVAL, "ensemble members must share one P2G brick shape");
+  }
   auto *e = new pl3d_ensemble();
   for (int32_t i = 0; i < count; i++) {
     pl3d_sim *m = nullptr;
@@ -1230,7 +1240,8 @@ int pl3d_ensemble_advance(pl3d_ensemble *e, double seconds, const int32_t *mask,
   }
   if (!e->graph) {
     const LaunchSet L{e->stream, e->tB, e->tP, e->tA, e->tD, int(K), e->gridN, e->gridC,
-                      e->gridF, e->gridFC, e->gridExtrap, e->gridScan, e->precond, nullptr};
+                      e->gridF, e->gridFC, e->gridExtrap, e->gridScan, e->precond, nullptr,
+                      e->members[0]->cfg.p2g_brick};
     if ((rc = capture(L, &e->graph))) return rc;
   }
   for (int guard = 0; guard < 1000000; guard++) {
@@ -1337,6 +1348,18 @@ int pl3d_recorder_destroy(pl3d_recorder *r) {
   r->destroyed = true;
   delete r;
   return PL3D_OK;
+}
+
+int pl3d_p2g_bricks(int32_t *shapes, int32_t cap) {
+  pl3d_set_error(PL3D_OK, "");
+  if (shapes)
+    for (int b = 0; b < PL3D_P2G_BRICKS && b < cap; b++) {
+      shapes[4 * b] = p2g_bricks[b].tx;
+      shapes[4 * b + 1] = p2g_bricks[b].ty;
+      shapes[4 * b + 2] = p2g_bricks[b].tz;
+      shapes[4 * b + 3] = p2g_bricks[b].threads;
+    }
+  return PL3D_P2G_BRICKS;
 }
 
 }  // extern "C"

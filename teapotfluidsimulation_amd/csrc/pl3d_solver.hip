@@ -762,8 +762,13 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   };
 
   // Chebyshev: the polynomial's recurrence constants (fixed for the solve, so the preconditioner
-  // is one fixed SPD operator and CG stays valid)
+  // is one fixed SPD operator and CG stays valid); z and, while the preconditioner runs, the CG
+  // direction s are held in registers
   const int chebDeg = D.chebDegree;
+  constexpr int CR = PC == PC_CHEB ? CPT : 1;
+  float zreg[CR], sreg[CR];
+#pragma unroll
+  for (int k = 0; k < CR; k++) zreg[k] = sreg[k] = 0.f;
   const float chebTheta = 0.5f * (2.0f + D.chebLower), chebDelta = 0.5f * (2.0f - D.chebLower);
 
   // z = M r.  MIC(0): owners write W = r, the sweeps run in place, W = z afterwards (ends with a
@@ -772,20 +777,24 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   // Jacobi: nothing to do, z is recomputed as r / diag where it is used.
   auto precondition = [&]() {
     if constexpr (PC == PC_CHEB) {
-      // step 0: z = d = D^-1 r / theta; then for each further degree, with res = r - A z from
-      // the z held in W:  d <- (rho' rho) d + (2 rho' / delta) D^-1 res,  z <- z + d,
-      // rho' = 1 / (2 sigma1 - rho), sigma1 = theta / delta, rho_0 = 1 / sigma1.  The same
-      // recurrence as the reference's GPU preconditioner (gpu3d/shaders.ts:588-620), with res
-      // recomputed from z instead of carried, so only d needs registers.
+      // step 0: z = d = D^-1 r / theta; then for each further degree, with res = r - A z:
+      //   d <- (rho' rho) d + (2 rho' / delta) D^-1 res,  z <- z + d,
+      //   rho' = 1 / (2 sigma1 - rho), sigma1 = theta / delta, rho_0 = 1 / sigma1
+      // (the recurrence of the reference's GPU preconditioner, gpu3d/shaders.ts:588-620, with res
+      // recomputed from z instead of carried).  z and d live in registers; the neighbours' z is
+      // staged in LDS, alternating between W and s_lds (the CG direction s is in registers
+      // meanwhile), so each step costs one barrier.
       const float sigma1 = chebTheta / chebDelta;
       float rho = 1.0f / sigma1;
       float dreg[CPT];
       FOR_SLOTS({
         const float d = r[k] * __builtin_amdgcn_rcpf(dgv(k) * chebTheta);
         dreg[k] = d;
+        zreg[k] = d;
         W[t] = d;
       })
       lds_barrier();
+      float *cur = W, *nxt = s_lds;
       for (int deg = 1; deg < chebDeg; deg++) {
         // opaque words: keep the neighbour decoding inside the loop (see the PCG loop)
 #pragma unroll
@@ -795,14 +804,17 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
         const float c1 = rhoN * rho, c2 = 2.0f * rhoN / chebDelta;
         FOR_SLOTS({
           const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
-          const float az = dgv(k) * W[t] - (((W[px >> 16] + W[px & 0x3FFFu]) +
-                                             (W[py >> 16] + W[py & 0x3FFFu])) +
-                                            (W[pz >> 16] + W[pz & 0x3FFFu]));
+          const float az = dgv(k) * zreg[k] - (((cur[px >> 16] + cur[px & 0x3FFFu]) +
+                                                (cur[py >> 16] + cur[py & 0x3FFFu])) +
+                                               (cur[pz >> 16] + cur[pz & 0x3FFFu]));
           dreg[k] = c1 * dreg[k] + c2 * (r[k] - az) * __builtin_amdgcn_rcpf(dgv(k));
+          zreg[k] += dreg[k];
+          nxt[t] = zreg[k];
         })
-        lds_barrier();  // every read of this z done
-        FOR_SLOTS({ W[t] += dreg[k]; })
-        lds_barrier();
+        lds_barrier();  // nxt complete; every read of cur done
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
         rho = rhoN;
       }
     } else if constexpr (PC == PC_MG) {
@@ -875,7 +887,9 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     }
   };
   auto zval = [&](int k, int t) -> float {
-    if constexpr (MIC || PC == PC_MG || PC == PC_CHEB) {
+    if constexpr (PC == PC_CHEB) {
+      return zreg[k];
+    } else if constexpr (MIC || PC == PC_MG) {
       return W[t];
     } else {
       return r[k] * __builtin_amdgcn_rcpf(dgv(k));
@@ -937,7 +951,9 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     const float fa = float(alpha);
     float rmax = 0.f;
     FOR_SLOTS({
-      xlo[k] += fa * s_lds[t];
+      const float st = s_lds[t];
+      xlo[k] += fa * st;
+      if constexpr (PC == PC_CHEB) sreg[k] = st;  // s_lds is the preconditioner's second buffer
       r[k] -= fa * W[t];
       rmax = fmaxf(rmax, fabsf(r[k]));
     })
@@ -954,7 +970,12 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     iters = it;
     // s = z + beta s now (z from the recursive residual); the refinement below may reuse W
     const float fb = float(sn / sigma);
-    FOR_SLOTS({ s_lds[t] = zval(k, t) + fb * s_lds[t]; })
+    FOR_SLOTS({
+      if constexpr (PC == PC_CHEB)
+        s_lds[t] = zval(k, t) + fb * sreg[k];
+      else
+        s_lds[t] = zval(k, t) + fb * s_lds[t];
+    })
     sigma = sn;
     PHASE_MARK(6);
     // reliable update: at the stop test, after each decade of residual reduction, and at least
@@ -1042,6 +1063,8 @@ __device__ __forceinline__ void pcg_entry(SolveLds<PC> &L, const PcgShared *__re
   const int ce = (m + NT - 1) / NT;
   if (ce <= 8)
     pcg_body<PC, 8>(L, A, D, m);
+  else if (ce <= 10)
+    pcg_body<PC, 10>(L, A, D, m);
   else if (ce <= 12)
     pcg_body<PC, 12>(L, A, D, m);
   else

@@ -5,11 +5,16 @@
 #include "pl3d_solver.h"
 
 #define STEP_THREADS 256
-// k_p2g: one block of P2G_THREADS per P2G_TX x P2G_TY x P2G_TZ brick of faces and cells
-#define P2G_TX 8
-#define P2G_TY 8
-#define P2G_TZ 4
-#define P2G_THREADS 1024
+// k_p2g: one block of NT threads per TX x TY x TZ brick of faces and cells; the shapes built are
+// listed in p2g_bricks (pl3d_step.hip), chosen per simulation by pl3d_config.p2g_brick
+#define PL3D_P2G_BRICKS 4
+struct DevBufs;
+typedef void (*StepKernel)(const DevBufs *, const DevParams *);
+struct P2gBrick {
+  int tx, ty, tz, threads;
+  StepKernel kernel;  // k_p2g<tx, ty, tz, threads>
+};
+extern const P2gBrick p2g_bricks[PL3D_P2G_BRICKS];
 #define SCAN_ITEMS 4
 #define SCAN_TILE (STEP_THREADS * SCAN_ITEMS)
 
@@ -50,7 +55,6 @@ __global__ void k_begin(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_count(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_classify_scan(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_scatter(const DevBufs *Bt, const DevParams *Pt);
-__global__ void k_p2g(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_liquid(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_faces_post(const DevBufs *Bt, const DevParams *Pt);
 __global__ void k_extrap(const DevBufs *Bt, const DevParams *Pt);

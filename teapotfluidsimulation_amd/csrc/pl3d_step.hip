@@ -332,20 +332,18 @@ __global__ void k_scatter(const DevBufs *__restrict__ Bt, const DevParams *__res
 
 // ============================================================================= k_p2g
 // particlesToGrid (solver3d.ts:410-431) per tile: a block owns the faces and cells of one
-// P2G_TX x P2G_TY x P2G_TZ brick of the index space, streams the parcels of the cells that can
+// TX x TY x TZ brick of the index space, streams the parcels of the cells that can
 // reach them (the brick plus a one-cell halo, from the cell bins) and adds each owned corner's
 // w*v and w into LDS as 64-bit fixed point (integer sums are associative: the result does not
 // depend on the order of parcels or threads).  The epilogue normalises the owned faces, stores
 // uPrev = u (:295), applies gravity on v faces next to FLUID (:433-443), the wall velocity on faces
 // next to SOLID (:446-471) and the validity mask (:545-562), and writes the cell-centre parcel
 // density of FLUID cells (densityCorrection, :635-642).
-#define P2G_TILE (P2G_TX * P2G_TY * P2G_TZ)
-#define P2G_MAXROWS ((P2G_TY + 2) * (P2G_TZ + 2))
 
 // One parcel's eight corner contributions to one component's owned faces.  APIC: the corner
 // value is vel + C . (x_face - x_p), formed as the base corner's value plus one C h step per
 // axis (src/sim/solver.ts:553-568 in 3D); (ox, oy, oz) is the component's grid offset.
-template <bool APIC>
+template <bool APIC, int TX, int TY, int TZ>
 __device__ __forceinline__ void p2g_corners(long long (*acc)[2], const int lo[3], int sx, int sy,
                                             int sz, float gx, float gy, float gz, float vel,
                                             bool withVel, float4 cr = float4{}, float px = 0.f,
@@ -360,10 +358,10 @@ __device__ __forceinline__ void p2g_corners(long long (*acc)[2], const int lo[3]
   for (int q = 0; q < 8; q++) {
     const int dx = q & 1, dy = (q >> 1) & 1, dz = q >> 2;
     const int li = a.i + dx - lo[0], lj = b.i + dy - lo[1], lk = c.i + dz - lo[2];
-    if (li < 0 || li >= P2G_TX || lj < 0 || lj >= P2G_TY || lk < 0 || lk >= P2G_TZ) continue;
+    if (li < 0 || li >= TX || lj < 0 || lj >= TY || lk < 0 || lk >= TZ) continue;
     // corner weight in the reference's order: x factor * y factor * z factor (:369-376)
     const float w = (dx ? a.f : 1.f - a.f) * (dy ? b.f : 1.f - b.f) * (dz ? c.f : 1.f - c.f);
-    const int e = li + P2G_TX * (lj + P2G_TY * lk);
+    const int e = li + TX * (lj + TY * lk);
     float v = vel;
     if (APIC) {
       v = a0;
@@ -379,9 +377,9 @@ __device__ __forceinline__ void p2g_corners(long long (*acc)[2], const int lo[3]
 }
 
 // The parcel loop of k_p2g for one transfer mode.
-template <bool APIC>
+template <bool APIC, int TX, int TY, int TZ>
 __device__ __forceinline__ void p2g_parcels(const DevBufs &B, const DevParams &P,
-                                            long long (*acc)[P2G_TILE][2], const int lo[3],
+                                            long long (*acc)[TX * TY * TZ][2], const int lo[3],
                                             const int *rowOff, const int *rowStart, int nrows,
                                             int total) {
   const int nx = P.nx, ny = P.ny, nz = P.nz;
@@ -404,37 +402,40 @@ __device__ __forceinline__ void p2g_parcels(const DevBufs &B, const DevParams &P
     if (APIC) {
       const float4 cu = B.binAff[3 * size_t(q)], cv = B.binAff[3 * size_t(q) + 1],
                    cw = B.binAff[3 * size_t(q) + 2];
-      p2g_corners<true>(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true, cu,
+      p2g_corners<true, TX, TY, TZ>(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true, cu,
                         pp.x, pp.y, pp.z, 0.f, 0.5f, 0.5f, h);
-      p2g_corners<true>(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true, cv,
+      p2g_corners<true, TX, TY, TZ>(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true, cv,
                         pp.x, pp.y, pp.z, 0.5f, 0.f, 0.5f, h);
-      p2g_corners<true>(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true, cw,
+      p2g_corners<true, TX, TY, TZ>(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true, cw,
                         pp.x, pp.y, pp.z, 0.5f, 0.5f, 0.f, h);
     } else {
-      p2g_corners<false>(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true);
-      p2g_corners<false>(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true);
-      p2g_corners<false>(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true);
+      p2g_corners<false, TX, TY, TZ>(acc[0], lo, nx + 1, ny, nz, gx, gy - 0.5f, gz - 0.5f, vv.x, true);
+      p2g_corners<false, TX, TY, TZ>(acc[1], lo, nx, ny + 1, nz, gx - 0.5f, gy, gz - 0.5f, vv.y, true);
+      p2g_corners<false, TX, TY, TZ>(acc[2], lo, nx, ny, nz + 1, gx - 0.5f, gy - 0.5f, gz, vv.z, true);
     }
-    p2g_corners<false>(acc[3], lo, nx, ny, nz, gx - 0.5f, gy - 0.5f, gz - 0.5f, 0.f, false);
+    p2g_corners<false, TX, TY, TZ>(acc[3], lo, nx, ny, nz, gx - 0.5f, gy - 0.5f, gz - 0.5f, 0.f, false);
   }
 }
 
-__global__ __launch_bounds__(P2G_THREADS) void k_p2g(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+template <int TX, int TY, int TZ, int NT>
+__global__ __launch_bounds__(NT) void k_p2g(const DevBufs *__restrict__ Bt, const DevParams *__restrict__ Pt) {
+  constexpr int TILE = TX * TY * TZ, MAXROWS = (TY + 2) * (TZ + 2);
+  static_assert(MAXROWS <= NT, "one thread per candidate row");
   MEMBER();
   if (int(blockIdx.x) >= P.gridFC || !is_active(B)) return;
   const DevCtl &C = *B.ctl;
-  __shared__ long long acc[4][P2G_TILE][2];  // u, v, w faces (w*v, w); cells (-, w)
-  __shared__ int rowOff[P2G_MAXROWS + 1];
-  __shared__ int rowStart[P2G_MAXROWS];
+  __shared__ long long acc[4][TILE][2];  // u, v, w faces (w*v, w); cells (-, w)
+  __shared__ int rowOff[MAXROWS + 1];
+  __shared__ int rowStart[MAXROWS];
   const int nx = P.nx, ny = P.ny, nz = P.nz;
-  const int tx = (nx + P2G_TX) / P2G_TX, ty = (ny + P2G_TY) / P2G_TY;  // ceil((n+1)/T)
+  const int tx = (nx + TX) / TX, ty = (ny + TY) / TY;  // ceil((n+1)/T)
   const int bt = blockIdx.x;
-  const int lo[3] = {(bt % tx) * P2G_TX, ((bt / tx) % ty) * P2G_TY, (bt / (tx * ty)) * P2G_TZ};
-  for (int e = threadIdx.x; e < 4 * P2G_TILE * 2; e += blockDim.x) (&acc[0][0][0])[e] = 0;
+  const int lo[3] = {(bt % tx) * TX, ((bt / tx) % ty) * TY, (bt / (tx * ty)) * TZ};
+  for (int e = threadIdx.x; e < 4 * TILE * 2; e += blockDim.x) (&acc[0][0][0])[e] = 0;
   // candidate rows (cy, cz) and the x-run of cells of each row
-  const int xa = max(lo[0] - 1, 0), xb = min(lo[0] + P2G_TX, nx - 1);
-  const int ya = max(lo[1] - 1, 0), yb = min(lo[1] + P2G_TY, ny - 1);
-  const int za = max(lo[2] - 1, 0), zb = min(lo[2] + P2G_TZ, nz - 1);
+  const int xa = max(lo[0] - 1, 0), xb = min(lo[0] + TX, nx - 1);
+  const int ya = max(lo[1] - 1, 0), yb = min(lo[1] + TY, ny - 1);
+  const int za = max(lo[2] - 1, 0), zb = min(lo[2] + TZ, nz - 1);
   const int nyr = max(yb - ya + 1, 0), nrows = nyr * max(zb - za + 1, 0);
   if (threadIdx.x < nrows) {
     const int cy = ya + threadIdx.x % nyr, cz = za + threadIdx.x / nyr;
@@ -450,15 +451,14 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(const DevBufs *__restrict__
   __syncthreads();
   const int total = rowOff[nrows];
   if (P.transfer == PL3D_TRANSFER_APIC_)
-    p2g_parcels<true>(B, P, acc, lo, rowOff, rowStart, nrows, total);
+    p2g_parcels<true, TX, TY, TZ>(B, P, acc, lo, rowOff, rowStart, nrows, total);
   else
-    p2g_parcels<false>(B, P, acc, lo, rowOff, rowStart, nrows, total);
+    p2g_parcels<false, TX, TY, TZ>(B, P, acc, lo, rowOff, rowStart, nrows, total);
   __syncthreads();
   // epilogue over the owned faces of the three components and the owned cells
-  for (int e = threadIdx.x; e < 4 * P2G_TILE; e += blockDim.x) {
-    const int comp = e / P2G_TILE, le = e % P2G_TILE;
-    const int i = lo[0] + le % P2G_TX, j = lo[1] + (le / P2G_TX) % P2G_TY,
-              k = lo[2] + le / (P2G_TX * P2G_TY);
+  for (int e = threadIdx.x; e < 4 * TILE; e += blockDim.x) {
+    const int comp = e / TILE, le = e % TILE;
+    const int i = lo[0] + le % TX, j = lo[1] + (le / TX) % TY, k = lo[2] + le / (TX * TY);
     if (comp == 3) {
       if (i < nx && j < ny && k < nz) {
         const int c = cidx(P, i, j, k);
@@ -489,6 +489,14 @@ __global__ __launch_bounds__(P2G_THREADS) void k_p2g(const DevBufs *__restrict__
                       ? 1 : 0;
   }
 }
+
+// The brick shapes the library is built with (pl3d_config.p2g_brick indexes this table).
+#define P2G_BRICK_KERNEL(TX, TY, TZ, NT) \
+  { TX, TY, TZ, NT, &k_p2g<TX, TY, TZ, NT> }
+const P2gBrick p2g_bricks[PL3D_P2G_BRICKS] = {
+    P2G_BRICK_KERNEL(8, 8, 4, 1024), P2G_BRICK_KERNEL(8, 8, 8, 1024),
+    P2G_BRICK_KERNEL(16, 8, 4, 1024), P2G_BRICK_KERNEL(8, 4, 4, 512)};
+#undef P2G_BRICK_KERNEL
 
 // ============================================================================= k_liquid
 // Per liquid cell: pressure rhs = -div u (divergenceRhs, solver3d.ts:473-493), the 7-point

@@ -133,8 +133,8 @@ static int name_index(const char *const *names, int n, const char *s) {
 /* ------------------------------------------------------------------ Config3D <-> pl3d_config */
 
 /* Config3D (sim3d/solver3d.ts:25-43) as a JS object -> pl3d_config.  Device-only keys
- * 'preconditioner' and 'transfer' are optional strings, 'chebDegree' and 'chebLower' optional
- * numbers (0 or absent: the library's defaults). */
+ * 'preconditioner' and 'transfer' are optional strings, 'chebDegree', 'chebLower' and
+ * 'p2gBrick' optional numbers (0 or absent: the library's defaults). */
 static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
   napi_value t, m, pv, x;
   double d;
@@ -220,6 +220,10 @@ static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
     if (napi_has_named_property(env, o, "chebLower", &has) == napi_ok && has) {
       if (!get_num(env, o, "chebLower", &d)) return 0;
       c->cheb_lower = d;
+    }
+    if (napi_has_named_property(env, o, "p2gBrick", &has) == napi_ok && has) {
+      if (!get_num(env, o, "p2gBrick", &d)) return 0;
+      c->p2g_brick = (int32_t)d;
     }
     r = get_opt_str(env, o, "transfer", buf, sizeof buf);
     if (r < 0) return 0;

@@ -11,9 +11,10 @@ import time
 from .sim3d import Simulation3D, gentlePour3D
 
 
-def run_pour(cell_mm=4.0, seconds=20.0, precond="mic0", progress=False, chunk=0.5):
+def run_pour(cell_mm=4.0, seconds=20.0, precond="mic0", progress=False, chunk=0.5, extra=None):
     cfg = gentlePour3D(cell_mm, seconds)
     cfg["preconditioner"] = precond
+    cfg.update(extra or {})
     sim = Simulation3D(cfg)
     rows = []
     done_at = -1.0

@@ -74,3 +74,20 @@ def test_error_conventions_without_gpu():
     assert b"Config3D" in lib.pl3d_last_error()
     assert lib.pl3d_phase_name(1) == b"tilt-in"
     assert lib.pl3d_phase_name(5) == b"done"
+
+
+def test_p2g_brick_table_and_tuned_settings():
+    """The brick table the library exports (host-only call) and the autotuner's config merge."""
+    from teapotfluidsimulation_amd import autotune
+    shapes = autotune.p2g_bricks()
+    assert shapes[0] == (8, 8, 4, 1024)
+    for tx, ty, tz, nt in shapes:
+        assert (ty + 2) * (tz + 2) <= nt and nt % 64 == 0
+    cfg = {"preconditioner": "mic0", "h": 0.004}
+    out = autotune.apply(cfg, {"p2gBrick": 2, "preconditioner": "chebyshev", "chebDegree": 4,
+                               "chebLower": 0.1, "trials": []})
+    assert out == {"preconditioner": "chebyshev", "h": 0.004, "p2gBrick": 2, "chebDegree": 4,
+                   "chebLower": 0.1}
+    c = abi.config_from_dict(dict(__import__("teapotfluidsimulation_amd").gentlePour3D(4, 20),
+                                  **out))
+    assert (c.p2g_brick, c.preconditioner, c.cheb_degree, c.cheb_lower) == (2, 3, 4, 0.1)

@@ -1,0 +1,61 @@
+"""P2G brick shapes and the autotuner on the GPU.
+
+- Every brick shape gives bit-identical pours: the scatter accumulates 64-bit fixed point per face
+  (integer sums do not depend on the order of parcels or on which block owns a face), so the brick
+  changes speed only.
+- The autotuner returns one of the built shapes and a valid solver setting, and the 5 mm pour run
+  with that setting meets the reference's GPU bar against its published CPU result
+  (scripts/verify-gpu.ts:57-61: volume within 1 %, shares within 2 points, nothing lost).
+"""
+import numpy as np
+import pytest
+
+from teapotfluidsimulation_amd import Simulation3D, autotune, gentlePour3D
+from teapotfluidsimulation_amd.run3d import run_pour
+
+pytestmark = pytest.mark.gpu
+
+
+def test_bricks_give_identical_pours():
+    shapes = autotune.p2g_bricks()
+    ref = None
+    for b, shape in enumerate(shapes):
+        print(f"brick {b} {shape}", flush=True)
+        cfg = gentlePour3D(5, 20)
+        cfg.update(preconditioner="chebyshev", transfer="apic", p2gBrick=b)
+        sim = Simulation3D(cfg)
+        d = sim.advance(2.5)
+        st = sim.state()
+        sim.destroy()
+        cur = (d["steps"], d["pressureIterations"], np.stack([st[k] for k in ("x", "y", "z", "vx", "vy", "vz")]))
+        if ref is None:
+            ref = cur
+        else:
+            assert cur[0] == ref[0] and cur[1] == ref[1]
+            assert np.array_equal(cur[2], ref[2]), shape
+
+
+def test_bad_brick_index_is_rejected():
+    cfg = gentlePour3D(10, 20)
+    cfg["p2gBrick"] = len(autotune.p2g_bricks())
+    with pytest.raises(Exception, match="p2g_brick"):
+        Simulation3D(cfg)
+
+
+def test_autotuner_choice_runs_within_parity(pins):
+    cfg = gentlePour3D(5, 20)
+    t = autotune.tune(cfg, probe_s=1.5, degrees=(4, 8), lowers=(0.1,), progress=True, rounds=1)
+    print({k: t[k] for k in autotune.TUNED_KEYS if k in t}, t["objective"])
+    assert 0 <= t["p2gBrick"] < len(autotune.p2g_bricks())
+    assert t["preconditioner"] in ("jacobi", "chebyshev")
+    # bricks, jacobi, two degrees, one interval (repeated settings are not probed again)
+    assert len(t["trials"]) <= len(autotune.p2g_bricks()) + 1 + 2 + 1
+    best = min(r["ms_per_sim_s"] for r in t["trials"])
+    assert t["objective"]["ms_per_sim_s"] == pytest.approx(best)
+    ref = next(r for r in pins["run3d_rows"] if r["cellMm"] == 5)
+    out = run_pour(5, 20, t["preconditioner"], extra=autotune.apply({}, t))
+    r = out["result"]
+    assert r["lostMl"] == 0 and r["pressureFailures"] == 0
+    assert abs(r["dischargedMl"] / ref["dischargedMl"] - 1) <= 0.01
+    for a, b in zip(out["dischargedFractions"], ref["dischargedFractions"]):
+        assert abs(a - b) <= 0.02
