@@ -44,6 +44,28 @@ def bytes_per_cell_iter(settings):
     return BYTES_PER_CELL_ITER[settings["preconditioner"]]
 
 
+def learned_comparison(a):
+    """The metric's second part: pressure-solve time at the 1e-6 residual on systems logged from
+    this pour, the learned preconditioner (flexible CG, trained now on the device from the other
+    logged systems) against plain MIC(0)-PCG and Chebyshev-PCG; solves through the standalone
+    solve entry points with host arrays, device time by HIP events."""
+    from teapotfluidsimulation_amd import learned
+    t0 = time.perf_counter()
+    dims, snaps = learned.log_pour(a.cell, 0.05, a.transfer)
+    tr, te = learned.split(snaps)
+    net, info = learned.train(dims, tr, steps=4000)
+    _, summary = learned.evaluate(net, dims, te, baselines=("mic0", "chebyshev"))
+    net.destroy()
+    ms = {k: v["mean_device_ms"] for k, v in summary.items()}
+    return {"systems_logged": len(snaps), "held_out": len(te), "train": info,
+            "solve_ms": ms,
+            "iterations": {k: v["mean_iterations"] for k, v in summary.items()},
+            "all_converged": {k: v["all_converged"] for k, v in summary.items()},
+            "speedup_vs_mic0": ms["mic0"] / ms["learned"],
+            "speedup_vs_chebyshev": ms["chebyshev"] / ms["learned"],
+            "wall_s": time.perf_counter() - t0}
+
+
 def solver_settings(a):
     """The solver and brick settings of the run: --autotune tunes them now (untimed); 'tuned' reads
     the committed autotuner result for this workload (teapotfluidsimulation_amd/tuned/), else the
@@ -301,6 +323,8 @@ def main():
                     help="default: apic for the 4 mm pour, flip for --ensemble (APIC is not "
                          "stable in the fast-tilting varied pours)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-preconditioner comparison (log, train, held-out solves)")
     ap.add_argument("--cpu-sample", type=float, default=20.0)
     ap.add_argument("--no-kernel-replay", action="store_true")
     ap.add_argument("--max-wall", type=float, default=150.0,
@@ -396,6 +420,10 @@ def main():
                         "avg_launch_us": avg_s * 1e6, "algorithmic_bytes_per_launch":
                         bytes_per_launch, "bytes_model": unit_note}
 
+    learned_obj = None
+    if not a.no_learned:
+        learned_obj = learned_comparison(a)
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline(int(a.cell) if a.cell == int(a.cell) else a.cell, a.cpu_sample,
@@ -438,6 +466,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "learned": learned_obj,
     }
     print(json.dumps(line))
     return 0

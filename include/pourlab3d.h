@@ -182,7 +182,7 @@ typedef struct pl3d_solve_result {
   double residual;      /* the solver's own (recursive) residual, max norm, as the reference */
   double true_residual; /* max |b - A q| recomputed in FP64 after the solve */
   int32_t liquid_cells;
-  int32_t pad;
+  int32_t fallbacks; /* pl3d_net_solve: iterations that used the Jacobi fallback (else 0) */
   double kernel_ms; /* device time of the solve kernel (HIP events around the launch) */
 } pl3d_solve_result;
 
@@ -345,6 +345,41 @@ int pl3d_poisson_solution(pl3d_poisson *ps, int32_t precision, int32_t k, double
 int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, const double *b,
                    double *q, double tol, int32_t max_iter, int32_t precond,
                    pl3d_solve_result *out);
+
+/* The pressure system of the last substep on the grid (nx*ny*nz entries each): cell flags,
+ * right-hand side b (divergenceRhs, solver3d.ts:473-493) and the solution q of the pressure solve
+ * (0 off the liquid).  Used to log training triples for the learned preconditioner. */
+int pl3d_read_pressure_system(pl3d_sim *sim, uint8_t *cell, double *b, double *q);
+
+/* ---- learned preconditioner ----------------------------------------------------------------- */
+
+/* A three-level encoder-decoder of 3x3x3 convolutions (8/16/32 channels, BF16 activations, FP32
+ * accumulation on the MFMA units) that maps a residual r on an nx*ny*nz pressure grid to an
+ * approximation of A^-1 r, trained on logged (cell, b, q) triples with Adam, and applied inside
+ * flexible CG.  No counterpart in the reference.  The parameters are the FP32 master weights of
+ * the six layers, each [16-padded cout][27 cin padded to 16] (k = tap * cin + ci,
+ * tap = dx + 3 dy + 9 dz with offsets dx, dy, dz + 1), in layer order. */
+typedef struct pl3d_net pl3d_net;
+int pl3d_net_create(int32_t nx, int32_t ny, int32_t nz, uint32_t seed, pl3d_net **out);
+int pl3d_net_destroy(pl3d_net *net);
+int pl3d_net_param_count(pl3d_net *net, int64_t *count);
+int pl3d_net_get_params(pl3d_net *net, float *out);
+int pl3d_net_set_params(pl3d_net *net, const float *in); /* also resets the Adam state */
+/* z = s * net([r / s, FLUID, SOLID]) on the grid, s = max |r| over FLUID (host arrays) */
+int pl3d_net_apply(pl3d_net *net, const uint8_t *cell, const float *r, float *z);
+/* append a training triple (host arrays, grid order): input b / max|b|, target q / max|q| */
+int pl3d_net_add_example(pl3d_net *net, const uint8_t *cell, const double *b, const double *q);
+int pl3d_net_example_count(pl3d_net *net, int32_t *count);
+/* loss sum_FLUID (z - t)^2 / sum t^2 of example k and its parameter gradient (grad may be NULL);
+ * no update */
+int pl3d_net_gradient(pl3d_net *net, int32_t k, double *loss, float *grad);
+/* `steps` Adam steps (one example each, seeded shuffled order); mean loss of the last steps */
+int pl3d_net_train(pl3d_net *net, int32_t steps, float lr, uint32_t seed, double *mean_loss);
+/* flexible CG (FCG(1), Polak-Ribiere beta) with the network as preconditioner, the stop rule of
+ * pl3d_pcg_solve, FP64 solution and true residual; kernel_ms = device time from the first to the
+ * last launch of the solve (host round trips between iterations included) */
+int pl3d_net_solve(pl3d_net *net, const uint8_t *cell, const double *b, double *q, double tol,
+                   int32_t max_iter, pl3d_solve_result *out);
 
 /* ---- tuning --------------------------------------------------------------------------------- */
 

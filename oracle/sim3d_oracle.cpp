@@ -1330,7 +1330,7 @@ void or3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t *cell, con
     out->residual = r.residual;
     out->true_residual = tr;
     out->liquid_cells = m;
-    out->pad = 0;
+    out->fallbacks = 0;
     out->kernel_ms = 0.0;
   }
 }

@@ -10,6 +10,7 @@ from .sim3d import (  # noqa: F401
     FLUID,
     IN_POT,
     LOST,
+    LearnedPreconditioner,
     SOLID,
     SPILLED,
     Pcg3D,

@@ -43,6 +43,18 @@ SIGNATURES = {
     "pl3d_counters": (C.c_int, [_vp, _P(C.c_int64)]),
     "pl3d_varied_pour": (C.c_int, [C.c_uint32, _i32, _P(abi.Config)]),
     "pl3d_p2g_bricks": (C.c_int, [_vp, _i32]),
+    "pl3d_read_pressure_system": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pl3d_net_create": (C.c_int, [_i32, _i32, _i32, C.c_uint32, _P(_vp)]),
+    "pl3d_net_destroy": (C.c_int, [_vp]),
+    "pl3d_net_param_count": (C.c_int, [_vp, _P(C.c_int64)]),
+    "pl3d_net_get_params": (C.c_int, [_vp, _vp]),
+    "pl3d_net_set_params": (C.c_int, [_vp, _vp]),
+    "pl3d_net_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pl3d_net_add_example": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "pl3d_net_example_count": (C.c_int, [_vp, _P(_i32)]),
+    "pl3d_net_gradient": (C.c_int, [_vp, _i32, _P(_d), _vp]),
+    "pl3d_net_train": (C.c_int, [_vp, _i32, C.c_float, C.c_uint32, _P(_d)]),
+    "pl3d_net_solve": (C.c_int, [_vp, _vp, _vp, _vp, _d, _i32, _vp]),
     "pl3d_recorder_create": (C.c_int, [_vp, _i32, _d, _P(_vp)]),
     "pl3d_recorder_parcels": (C.c_int, [_vp, _P(_i32), _vp]),
     "pl3d_recorder_capture": (C.c_int, [_vp, _vp]),

@@ -136,7 +136,7 @@ class SolveResult(C.Structure):
         ("residual", C.c_double),
         ("true_residual", C.c_double),
         ("liquid_cells", C.c_int32),
-        ("pad", C.c_int32),
+        ("fallbacks", C.c_int32),
         ("kernel_ms", C.c_double),
     ]
 

@@ -1019,6 +1019,30 @@ int pl3d_read_affine(pl3d_sim *s, double *out) {
   return PL3D_OK;
 }
 
+int pl3d_read_pressure_system(pl3d_sim *s, uint8_t *cell, double *b, double *q) {
+  if (int rc = check_sim(s)) return rc;
+  if (!cell || !b || !q) return fail(PL3D_EINVAL, "null argument");
+  if (int rc = sync_ctl(s)) return rc;
+  const int m = s->hctl->m;
+  const size_t nc = size_t(s->P.nc);
+  std::vector<int> lc(std::max(m, 1));
+  std::vector<double> hb(std::max(m, 1)), hq(std::max(m, 1));
+  HIP_OK(hipMemcpyAsync(cell, s->B.flag, nc, hipMemcpyDeviceToHost, s->stream));
+  if (m > 0) {
+    HIP_OK(hipMemcpyAsync(lc.data(), s->B.lcell, size_t(m) * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(hb.data(), s->B.bP, size_t(m) * 8, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(hq.data(), s->B.qP, size_t(m) * 8, hipMemcpyDeviceToHost, s->stream));
+  }
+  HIP_OK(hipStreamSynchronize(s->stream));
+  std::fill(b, b + nc, 0.0);
+  std::fill(q, q + nc, 0.0);
+  for (int t = 0; t < m; t++) {
+    b[lc[t]] = hb[t];
+    q[lc[t]] = hq[t];
+  }
+  return PL3D_OK;
+}
+
 int pl3d_set_target(pl3d_sim *s, double angle_deg) {
   if (int rc = check_steppable(s)) return rc;
   if (int rc = sync_ctl(s)) return rc;

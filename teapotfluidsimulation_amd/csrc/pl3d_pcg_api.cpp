@@ -195,7 +195,7 @@ extern "C" int pl3d_pcg_solve(int32_t nx, int32_t ny, int32_t nz, const uint8_t 
     out->residual = m == 0 ? 0.0 : hres.residual;
     out->true_residual = m == 0 ? 0.0 : hres.trueResidual;
     out->liquid_cells = m;
-    out->pad = 0;
+    out->fallbacks = 0;
   }
   return done(PL3D_OK);
 }

@@ -1,0 +1,175 @@
+"""Training and evaluation of the learned pressure preconditioner on logged pour snapshots.
+
+A pour is run with the plain device solver and, after every `every_s` simulated seconds, the
+last substep's pressure system (cell flags, rhs b, solution q) is logged; those triples, plus
+synthetic (A e, e) pairs on the same masks, train the network (LearnedPreconditioner,
+pl3d_net_*), and held-out snapshots compare the flexible CG with the network against the plain
+PCG solves (pl3d_pcg_solve) at the same stop rule.  No counterpart
+in the reference.
+
+Usage: python -m teapotfluidsimulation_amd.learned [cellMm] [--steps N] [--out FILE]
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from .sim3d import LearnedPreconditioner, Pcg3D, Simulation3D, gentlePour3D
+
+
+def log_pour(cell_mm=4.0, every_s=0.05, transfer="apic", preconditioner="chebyshev", seed=1,
+             progress=False):
+    """[(cell, b, q)] of the pour's pressure solves, one per every_s simulated seconds, while
+    the pot holds liquid (the run stops 1.5 s after the controller is done, as run3d.ts)."""
+    cfg = gentlePour3D(cell_mm, 20)
+    cfg.update(transfer=transfer, preconditioner=preconditioner, seed=seed)
+    sim = Simulation3D(cfg)
+    out = []
+    done_at = -1.0
+    while not sim.finished:
+        d = sim.advance(every_s)
+        cell, b, q = sim.readPressureSystem()
+        if np.any(cell == 1) and np.abs(b[cell == 1]).max() > 0:
+            out.append((cell, b, q))
+        if d["phase"] == "done" and done_at < 0:
+            done_at = d["time"]
+        if done_at >= 0 and d["time"] > done_at + 1.5:
+            break
+        if progress and len(out) % 40 == 0:
+            sys.stderr.write(f"logged {len(out)} systems, t={d['time']:.2f}\n")
+    dims = (sim.nx, sim.ny, sim.nz)
+    sim.destroy()
+    return dims, out
+
+
+def apply_a(cell, x):
+    """The pressure operator on the grid (FLUID rows; AIR = 0, SOLID = Neumann), (nz, ny, nx)."""
+    fl = cell == 1
+    xp = np.pad(np.where(fl, x, 0.0), 1)
+    cp = np.pad(cell, 1, constant_values=2)
+    out = np.zeros_like(x)
+    dg = np.zeros_like(x)
+    nz, ny, nx = x.shape
+    for ax in range(3):
+        for sgn in (-1, 1):
+            sl = [slice(1, 1 + nz), slice(1, 1 + ny), slice(1, 1 + nx)]
+            sl[ax] = slice(1 + sgn, 1 + sgn + x.shape[ax])
+            nb_c = cp[tuple(sl)]
+            dg += nb_c != 2
+            out -= np.where(nb_c == 1, xp[tuple(sl)], 0.0)
+    return np.where(fl, dg * x + out, 0.0)
+
+
+def synthetic(cell, rng):
+    """(cell, A e, e) for a random error field e on the FLUID cells: box-filtered noise with a
+    random number of passes, so the pairs cover rough to smooth residuals (CG's later residuals
+    are not like the first right-hand side)."""
+    e = rng.standard_normal(cell.shape)
+    for _ in range(int(rng.choice([0, 1, 2, 4, 8, 16]))):
+        e = sum(np.roll(e, s, a) for a in range(3) for s in (-1, 0, 1)) / 9.0
+    e = np.where(cell == 1, e, 0.0)
+    return cell, apply_a(cell, e), e
+
+
+def split(snaps, holdout_every=4):
+    """Every holdout_every-th snapshot is held out for evaluation."""
+    train = [s for i, s in enumerate(snaps) if i % holdout_every != holdout_every - 1]
+    test = [s for i, s in enumerate(snaps) if i % holdout_every == holdout_every - 1]
+    return train, test
+
+
+def train(dims, snaps, steps=3000, lr=1e-3, seed=1, progress=False, synthetic_per_snap=1):
+    """Train on the logged triples plus synthetic_per_snap (A e, e) pairs per logged mask."""
+    nx, ny, nz = dims
+    net = LearnedPreconditioner(nx, ny, nz, seed=seed)
+    rng = np.random.default_rng(seed)
+    for cell, b, q in snaps:
+        net.add_example(cell, b, q)
+        for _ in range(synthetic_per_snap):
+            net.add_example(*synthetic(cell, rng))
+    l0 = float(np.mean([net.gradient(k)[0] for k in range(0, net.examples, max(1, net.examples // 8))]))
+    t0 = time.perf_counter()
+    done, losses = 0, []
+    while done < steps:
+        n = min(500, steps - done)
+        # step decay: full rate for the first half, then 1/3, then 1/10
+        rate = lr if done < steps // 2 else (lr / 3 if done < 3 * steps // 4 else lr / 10)
+        losses.append(net.train(n, lr=rate, seed=seed + done))
+        done += n
+        if progress:
+            sys.stderr.write(f"step {done}: loss {losses[-1]:.4f}\n")
+    wall = time.perf_counter() - t0
+    return net, {"steps": steps, "lr": lr, "examples": net.examples, "initial_loss": l0,
+                 "final_loss": losses[-1], "train_wall_s": wall}
+
+
+def evaluate(net, dims, snaps, tol=1e-6, max_iter=400, baselines=("mic0", "chebyshev"),
+             warm=True, without_net=True):
+    """Per-snapshot iterations and solve time: the flexible CG with the network against plain
+    PCG with each baseline preconditioner (pl3d_pcg_solve; Chebyshev with the library default
+    degree and interval) and, with without_net, against the same flexible CG whose network has
+    all-zero weights (the smoothing steps alone): what the trained network adds."""
+    nx, ny, nz = dims
+    rows = []
+    zero = None
+    if without_net:
+        zero = LearnedPreconditioner(nx, ny, nz, seed=1)
+        zero.params = np.zeros(zero.param_count, np.float32)
+    if warm and snaps:  # first launches load the kernels
+        c, b, _ = snaps[0]
+        net.solve(c, b, np.zeros(b.size), tol, max_iter)
+        for pc in baselines:
+            Pcg3D(nx, ny, nz, pc).solve(c.ravel(), b.ravel(), np.zeros(b.size), tol, max_iter)
+    for cell, b, _ in snaps:
+        row = {"liquidCells": int(np.sum(cell == 1))}
+        q = np.zeros(b.size)
+        t0 = time.perf_counter()
+        r = net.solve(cell, b, q, tol, max_iter)
+        row["learned"] = {"iterations": r["iterations"], "converged": r["converged"],
+                          "trueResidual": r["trueResidual"], "fallbacks": r["fallbacks"],
+                          "device_ms": r["kernelMs"], "wall_ms": 1e3 * (time.perf_counter() - t0)}
+        if zero is not None:
+            r0 = zero.solve(cell, b, np.zeros(b.size), tol, max_iter)
+            row["without_net"] = {"iterations": r0["iterations"], "converged": r0["converged"],
+                                  "trueResidual": r0["trueResidual"], "device_ms": r0["kernelMs"]}
+        for pc in baselines:
+            q2 = np.zeros(b.size)
+            r2 = Pcg3D(nx, ny, nz, pc).solve(cell.ravel(), b.ravel(), q2, tol, max_iter)
+            row[pc] = {"iterations": r2["iterations"], "converged": r2["converged"],
+                       "trueResidual": r2["trueResidual"], "device_ms": r2["kernelMs"]}
+        rows.append(row)
+    if zero is not None:
+        zero.destroy()
+    summary = {}
+    keys = ("learned",) + (("without_net",) if zero is not None else ()) + tuple(baselines)
+    for key in keys:
+        its = [r[key]["iterations"] for r in rows]
+        ms = [r[key]["device_ms"] for r in rows]
+        summary[key] = {"mean_iterations": float(np.mean(its)), "mean_device_ms": float(np.mean(ms)),
+                        "all_converged": all(r[key]["converged"] for r in rows)}
+    summary["learned"]["mean_fallbacks"] = float(np.mean([r["learned"]["fallbacks"] for r in rows]))
+    return rows, summary
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cell", nargs="?", type=float, default=4.0)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--every", type=float, default=0.05)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    dims, snaps = log_pour(a.cell, a.every, progress=True)
+    tr, te = split(snaps)
+    net, info = train(dims, tr, a.steps, progress=True)
+    rows, summary = evaluate(net, dims, te)
+    res = {"grid": dims, "logged": len(snaps), "train": info, "heldout": len(te), "summary": summary}
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(dict(res, rows=rows), f, indent=1)
+    print(json.dumps(res, indent=1))
+
+
+if __name__ == "__main__":
+    main()

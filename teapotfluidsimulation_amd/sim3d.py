@@ -210,6 +210,16 @@ class Simulation3D:
         check(load().pl3d_read_affine(self._handle(), _ptr(out)))
         return out
 
+    def readPressureSystem(self):
+        """(cell, b, q) of the last substep's pressure solve on the grid, each (nz, ny, nx)."""
+        nc = self.nx * self.ny * self.nz
+        cell = np.zeros(nc, np.uint8)
+        b = np.zeros(nc, np.float64)
+        q = np.zeros(nc, np.float64)
+        check(load().pl3d_read_pressure_system(self._handle(), _ptr(cell), _ptr(b), _ptr(q)))
+        shp = (self.nz, self.ny, self.nx)
+        return cell.reshape(shp), b.reshape(shp), q.reshape(shp)
+
     def setTarget(self, angleDeg: float):
         """Manual tilt target (PourController.setTarget, motion.ts:40-44)."""
         check(load().pl3d_set_target(self._handle(), float(angleDeg)))
@@ -364,6 +374,98 @@ class Ensemble3D:
             for m in self.members:
                 m._h = None
             check(load().pl3d_ensemble_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class LearnedPreconditioner:
+    """The learned pressure preconditioner (pl3d_net_*): a 3D conv encoder-decoder over an
+    nx x ny x nz pressure grid, trained on the device on logged (cell, b, q) triples and applied
+    inside flexible CG.  Grid arrays are (nz, ny, nx) or flat in that order."""
+
+    def __init__(self, nx: int, ny: int, nz: int, seed: int = 1):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        h = C.c_void_p()
+        check(load().pl3d_net_create(nx, ny, nz, seed, C.byref(h)))
+        self._h = h
+        cnt = C.c_int64()
+        check(load().pl3d_net_param_count(self._h, C.byref(cnt)))
+        self.param_count = cnt.value
+
+    def _handle(self):
+        if not getattr(self, "_h", None):
+            raise RuntimeError("network was destroyed")
+        return self._h
+
+    def _flat(self, a, dtype):
+        a = np.ascontiguousarray(np.asarray(a, dtype).ravel())
+        if a.size != self.nx * self.ny * self.nz:
+            raise ValueError("array does not match the network's grid")
+        return a
+
+    @property
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.param_count, np.float32)
+        check(load().pl3d_net_get_params(self._handle(), _ptr(out)))
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(np.asarray(p, np.float32))
+        if p.size != self.param_count:
+            raise ValueError("wrong parameter count")
+        check(load().pl3d_net_set_params(self._handle(), _ptr(p)))
+
+    def apply(self, cell, r) -> np.ndarray:
+        cell, r = self._flat(cell, np.uint8), self._flat(r, np.float32)
+        z = np.zeros_like(r)
+        check(load().pl3d_net_apply(self._handle(), _ptr(cell), _ptr(r), _ptr(z)))
+        return z
+
+    def add_example(self, cell, b, q):
+        check(load().pl3d_net_add_example(self._handle(), _ptr(self._flat(cell, np.uint8)),
+                                          _ptr(self._flat(b, np.float64)),
+                                          _ptr(self._flat(q, np.float64))))
+
+    @property
+    def examples(self) -> int:
+        v = C.c_int32()
+        check(load().pl3d_net_example_count(self._handle(), C.byref(v)))
+        return v.value
+
+    def gradient(self, k: int):
+        loss = C.c_double()
+        g = np.zeros(self.param_count, np.float32)
+        check(load().pl3d_net_gradient(self._handle(), k, C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train(self, steps: int, lr: float = 1e-3, seed: int = 1) -> float:
+        loss = C.c_double()
+        check(load().pl3d_net_train(self._handle(), steps, float(lr), seed, C.byref(loss)))
+        return loss.value
+
+    def solve(self, cell, b, q, tol: float, maxIter: int) -> dict:
+        """Flexible CG with the network; fills q (flat float64 array, grid order)."""
+        cell = self._flat(cell, np.uint8)
+        b = self._flat(b, np.float64)
+        if q.dtype != np.float64 or not q.flags.c_contiguous or q.size != b.size:
+            raise ValueError("q must be a contiguous float64 array of the grid's size")
+        r = abi.SolveResult()
+        check(load().pl3d_net_solve(self._handle(), _ptr(cell), _ptr(b), _ptr(q), float(tol),
+                                    int(maxIter), C.byref(r)))
+        return {"iterations": r.iterations, "converged": bool(r.converged),
+                "residual": r.residual, "trueResidual": r.true_residual,
+                "liquidCells": r.liquid_cells, "kernelMs": r.kernel_ms,
+                "fallbacks": r.fallbacks}
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            check(load().pl3d_net_destroy(self._h))
             self._h = None
 
     def __del__(self):
