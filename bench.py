@@ -87,13 +87,13 @@ def solver_settings(a):
                     f"autotuner result {os.path.relpath(path, ROOT)}")
         return dict(autotune.DEFAULT_SOLVER), "library defaults (no tuned file)"
     return {"preconditioner": a.precond}, "command line"
-# HBM bytes per PCG launch on the 4 mm APIC pour (Jacobi; the tuned Chebyshev of the default
-# run), measured with rocprofv3 FETCH_SIZE and WRITE_SIZE and calibrated on the library's copy
-# kernel (FETCH x 2.0, WRITE x 1.0), with the profile each comes from.  The solve keeps its
+# HBM bytes per PCG launch on the 4 mm APIC pour (Jacobi; Chebyshev of degree 12), measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE and calibrated on the library's copy kernel (FETCH x 2.0,
+# WRITE x 1.0), with the profile each comes from; other settings report traffic null.  The solve keeps its
 # vectors in LDS and registers, so this is far below the algorithmic bytes.
 MEASURED_TRAFFIC = {
-    ("apic", "jacobi", 4.0): (1.433e6, "profiles/pour4mm_apic_jacobi_hbm_traffic.md"),
-    ("apic", "chebyshev", 4.0): (1.116e6, "profiles/pour4mm_apic_chebyshev_hbm_traffic.md"),
+    ("apic", "jacobi", 4.0, None): (1.433e6, "profiles/pour4mm_apic_jacobi_hbm_traffic.md"),
+    ("apic", "chebyshev", 4.0, 12): (1.116e6, "profiles/pour4mm_apic_chebyshev_hbm_traffic.md"),
 }
 # Published reference figure for this metric: the reference's WebGPU FP32 4 mm pour, 0.87x real
 # time on an RTX A2000 Laptop (benchmarks/gpu3d.json:97-137).
@@ -412,8 +412,10 @@ def main():
             unit_note = "no byte model for this kernel"
         if bytes_per_launch is not None:
             achieved = bytes_per_launch / avg_s / 1e9
+            deg = (int(a.settings.get("chebDegree", 6))
+                   if a.settings["preconditioner"] == "chebyshev" else None)
             traffic, traffic_src = MEASURED_TRAFFIC.get(
-                (a.transfer, a.settings["preconditioner"], a.cell), (None, None))
+                (a.transfer, a.settings["preconditioner"], a.cell, deg), (None, None))
             roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                         "traffic_source": traffic_src,

@@ -142,6 +142,8 @@ typedef struct pl3d_config {
   int32_t cheb_degree;    /* Chebyshev preconditioner degree, 0: PL3D_CHEB_DEGREE_DEFAULT */
   double cheb_lower;      /* its interval [cheb_lower, 2] of D^-1 A, 0: PL3D_CHEB_LOWER_DEFAULT */
   int32_t p2g_brick;      /* P2G brick shape, an index below pl3d_p2g_bricks() (0: 8x8x4) */
+  int32_t warm_start;     /* 1: each solve starts from the previous substep's solution (the
+                             reference starts from 0, solver3d.ts:867); same stop rule */
 } pl3d_config;
 
 /* Diagnostics3D (sim3d/solver3d.ts:45-60) */

@@ -91,6 +91,7 @@ class Config(C.Structure):
         ("cheb_degree", C.c_int32),
         ("cheb_lower", C.c_double),
         ("p2g_brick", C.c_int32),
+        ("warm_start", C.c_int32),
     ]
 
 
@@ -224,6 +225,7 @@ def config_to_dict(c: Config) -> dict:
         "chebDegree": c.cheb_degree,
         "chebLower": c.cheb_lower,
         "p2gBrick": c.p2g_brick,
+        "warmStart": c.warm_start,
     }
 
 
@@ -269,4 +271,5 @@ def config_from_dict(d: dict) -> Config:
     c.cheb_degree = int(d.get("chebDegree", 0))
     c.cheb_lower = float(d.get("chebLower", 0.0))
     c.p2g_brick = int(d.get("p2gBrick", 0))
+    c.warm_start = int(bool(d.get("warmStart", 0)))
     return c

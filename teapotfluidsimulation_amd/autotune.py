@@ -10,7 +10,9 @@ default solver), then the preconditioner family, then the Chebyshev degree at th
 interval, then the interval at the best degree.  Every candidate solves to the same stop rule, so
 the choice changes speed only; results stay within the pour's statistical parity bar.  The
 degree and interval sweeps alternate until neither moves (at most `rounds` passes), since the
-best degree depends on the interval.
+best degree depends on the interval.  Last, warm-started solves (each solve starts from the
+previous substep's solution) are tried, and kept, with one more degree and interval sweep, when
+faster.
 
 Usage: python -m teapotfluidsimulation_amd.autotune [cellMm] [--transfer apic|flip] [--out FILE]
 """
@@ -26,7 +28,7 @@ from .sim3d import Simulation3D, gentlePour3D
 DEGREES = (2, 3, 4, 5, 6, 8, 10, 12)
 LOWERS = (0.002, 0.005, 0.01, 0.02, 0.05, 0.1, 0.2)
 DEFAULT_SOLVER = {"preconditioner": "chebyshev", "chebDegree": 6, "chebLower": 0.05}
-TUNED_KEYS = ("p2gBrick", "preconditioner", "chebDegree", "chebLower")
+TUNED_KEYS = ("p2gBrick", "preconditioner", "chebDegree", "chebLower", "warmStart")
 
 
 def p2g_bricks():
@@ -65,7 +67,7 @@ def probe(cfg: dict, probe_s: float, chunk: float = 0.5) -> dict:
 
 
 def tune(cfg: dict, probe_s: float = 3.0, degrees=DEGREES, lowers=LOWERS, bricks=None,
-         families=("jacobi", "chebyshev"), progress=False, rounds=2) -> dict:
+         families=("jacobi", "chebyshev"), progress=False, rounds=2, warm=True) -> dict:
     """Returns {"p2gBrick", "preconditioner", "chebDegree", "chebLower", "objective", "trials"}."""
     trials = []
     seen = {}
@@ -121,6 +123,23 @@ def tune(cfg: dict, probe_s: float = 3.0, degrees=DEGREES, lowers=LOWERS, bricks
                 break
         if cheb_v < best_v:
             chosen, best_v = cheb_best, cheb_v
+    if warm:  # warm-started solves, then the Chebyshev sweeps again from there
+        c = dict(chosen, warmStart=1)
+        v = run("warm start", apply(cfg, c))
+        if v < best_v:
+            chosen, best_v = c, v
+            if chosen.get("preconditioner") == "chebyshev":
+                for d in degrees:
+                    c = dict(chosen, chebDegree=d)
+                    v = run(f"warm start, chebyshev degree {d} lower {c['chebLower']}", apply(cfg, c))
+                    if v < best_v:
+                        chosen, best_v = c, v
+                for lo in lowers:
+                    c = dict(chosen, chebLower=lo)
+                    v = run(f"warm start, chebyshev degree {c['chebDegree']} lower {lo}",
+                            apply(cfg, c))
+                    if v < best_v:
+                        chosen, best_v = c, v
     out = {k: chosen[k] for k in TUNED_KEYS if k in chosen}
     out["brickShape"] = list(shapes[out["p2gBrick"]])
     out["objective"] = {"ms_per_sim_s": round(best_v, 3), "probe_s": probe_s}

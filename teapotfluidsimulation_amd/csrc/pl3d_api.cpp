@@ -628,7 +628,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       {(void **)&B.lcell, NC * 4}, {(void **)&B.nbr, NC * 12},
       {(void **)&B.diag, NC}, {(void **)&B.mic, NC}, {(void **)&B.plane, NC * 2},
       {(void **)&B.bP, NC * 8}, {(void **)&B.bD, NC * 8}, {(void **)&B.qP, NC * 8},
-      {(void **)&B.qD, NC * 8},
+      {(void **)&B.qD, NC * 8}, {(void **)&B.qWarm, 2 * NC * 8},
       {(void **)&B.porder, 2 * NC * 4}, {(void **)&B.pc2, 2 * NC * 4},
       {(void **)&B.nbrPlane, 2 * 3 * NC * 4}, {(void **)&B.descs, 2 * sizeof(PcgDesc)},
       {(void **)&s->dB, sizeof(DevBufs)}, {(void **)&s->dP, sizeof(DevParams)},
@@ -729,6 +729,10 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     d.chebDegree = c.cheb_degree > 0 ? c.cheb_degree : PL3D_CHEB_DEGREE_DEFAULT;
     d.chebLower = float(c.cheb_lower > 0.0 ? c.cheb_lower : PL3D_CHEB_LOWER_DEFAULT);
   }
+  if (c.warm_start) {
+    descs[0].warm = B.qWarm;
+    descs[1].warm = B.qWarm + NC;
+  }
   if (s->mg) {
     descs[0].mg = s->mg;
     descs[1].mg = s->mg + pl3d_mg_layout(nx, ny, nz).bytes;
@@ -755,6 +759,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
       (e = hipMemsetAsync(B.uc, 0, NF * 4, s->stream)) ||
       (e = hipMemsetAsync(B.qP, 0, NC * 8, s->stream)) ||
       (e = hipMemsetAsync(B.qD, 0, NC * 8, s->stream)) ||
+      (e = hipMemsetAsync(B.qWarm, 0, 2 * NC * 8, s->stream)) ||
       (e = up(B.descs, descs, sizeof descs)) || (e = up(B.ctl, &H, sizeof(DevCtl))) ||
       (e = up(s->dB, &B, sizeof(DevBufs))) || (e = up(s->dP, &P, sizeof(DevParams))) ||
       (e = up(s->dA, &s->A, sizeof(PcgShared))) ||

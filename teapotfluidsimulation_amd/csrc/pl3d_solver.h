@@ -49,6 +49,8 @@ struct PcgDesc {
   unsigned char *mg;  // multigrid scratch of pl3d_mg_scratch_bytes(nx, ny, nz) (MG only)
   int chebDegree;     // Chebyshev only: polynomial degree (>= 1) and lower bound of the interval
   float chebLower;    // [lower, 2] of D^-1 A it targets
+  double *warm;       // nullptr: start from q = 0 (the reference); else the grid-indexed initial
+                      // guess, overwritten with this solve's solution (warm start)
 };
 
 // ---- multigrid (V(2,2) red-black Gauss-Seidel, up to PL3D_MG_LEVELS levels) -------------------

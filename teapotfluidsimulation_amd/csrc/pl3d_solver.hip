@@ -665,6 +665,27 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     return;
   }
   const double target = fmax(double(D.tol) * bnorm, 1e-14);
+  // warm start: x_hi from the previous solution on the grid, r = b - A x_hi in FP64
+  double rwarm = bnorm;
+  if (D.warm) {
+    FOR_SLOTS({
+      if (own) xhi[t] = D.warm[A.lcell[tc]];
+      W[t] = 0.f;
+    })
+    const double mx = fold_and_residual(W, xhi, b, nbx, nby, nbz, m, ce, tb);
+    FOR_SLOTS({ r[k] = own ? W[t] : 0.f; })
+    rwarm = block_max_d(R, rb, mx);
+    if (rwarm <= target) {  // the guess already meets the stop rule
+      if (tid == 0) {
+        *D.iterations = 0;
+        *D.converged = 1;
+        if (D.residual) *D.residual = rwarm;
+        if (D.trueResidual) *D.trueResidual = rwarm;
+        if (D.bnorm) *D.bnorm = bnorm;
+      }
+      return;
+    }
+  }
 
   // ---- MIC(0): hyperplane ordering and factorisation
   int nplanes = 0;
@@ -922,8 +943,8 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     sig_part += double(zt) * double(r[k]);
   })
   double sigma = block_sum(R, rb, sig_part);
-  double res = bnorm;
-  double resAtUpdate = bnorm;
+  double res = rwarm;
+  double resAtUpdate = rwarm;
   int iters = 0;
   bool converged = false;
   int lastUpdate = 0;
@@ -1014,6 +1035,10 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   } else {
     tres = res;
   }
+  if (D.warm)  // each thread's own x_hi entries (written by itself in the last refinement)
+    FOR_SLOTS({
+      if (own) D.warm[A.lcell[tc]] = xhi[t];
+    })
   if (tid == 0) {
     if (A.cellIterations) {
       atomicAdd(A.cellIterations, (unsigned long long)m * (unsigned long long)iters);

@@ -44,6 +44,7 @@ struct DevBufs {
   uint8_t *diag, *mic;
   uint16_t *plane;
   double *bP, *bD, *qP, *qD;
+  double *qWarm;  // warm start: the two solves' last solutions on the grid (2 x C, FP64)
   // solver scratch (two solves)
   int *porder;     // 2 x C
   float *pc2;      // 2 x C

@@ -225,6 +225,16 @@ static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
       if (!get_num(env, o, "p2gBrick", &d)) return 0;
       c->p2g_brick = (int32_t)d;
     }
+    if (napi_has_named_property(env, o, "warmStart", &has) == napi_ok && has) {
+      napi_value wv;
+      bool bv = false;
+      if (napi_get_named_property(env, o, "warmStart", &wv) != napi_ok) return 0;
+      if (napi_get_value_bool(env, wv, &bv) != napi_ok) {
+        if (!get_num(env, o, "warmStart", &d)) return 0;
+        bv = d != 0;
+      }
+      c->warm_start = bv ? 1 : 0;
+    }
     r = get_opt_str(env, o, "transfer", buf, sizeof buf);
     if (r < 0) return 0;
     c->transfer = r == 0 ? PL3D_TRANSFER_FLIP : name_index(TRANSFERS, 2, buf);

@@ -95,3 +95,48 @@ console.log(JSON.stringify(out));
         assert js[pc]["iterations"] == res["iterations"]
         assert js[pc]["trueResidual"] == res["trueResidual"]
         assert np.array_equal(q_js, q_py)
+
+
+def test_live_view_drives_gpu_simulation3d(tmp_path):
+    """node/src/gpu3d/solver.js, the asynchronous GpuSimulation3D the live view (live.ts:29-53,
+    :291-308) drives: a manual-tilt session ('manual-5', liveConfig) stepped by awaited step()
+    calls with a tilt target set on the controller gives the same render positions, bit for bit,
+    as the Python host stepping the same substeps."""
+    script = f"""
+const fs = require('fs');
+const {{ GpuSimulation3D }} = require('./src/gpu3d/solver');
+const {{ gentlePour3D }} = require('./src/sim3d/presets3d');
+(async () => {{
+  const cfg = gentlePour3D(5, 3600);
+  cfg.motion = Object.assign({{}}, cfg.motion, {{ kind: 'manual' }});
+  const sim = new GpuSimulation3D(null, cfg);
+  sim.controller.setTarget(35, sim.time);
+  for (let k = 0; k < 60; k++) await sim.step();
+  await sim.sync();
+  const xyz = await sim.readRenderPositions();
+  fs.writeFileSync('{tmp_path}/render.bin', Buffer.from(xyz.buffer, xyz.byteOffset, xyz.byteLength));
+  const out = {{ n: sim.n, time: sim.time, steps: sim.steps, angle: sim.controller.angleDeg,
+                 phase: sim.controller.phase, origin: sim.cpu.origin.length,
+                 byOrigin: sim.stats.dischargedByOrigin, h: sim.cpu.h }};
+  sim.destroy();
+  console.log(JSON.stringify(out));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});
+"""
+    print("node GpuSimulation3D live session", flush=True)
+    r = subprocess.run([NODE, "-e", script], cwd=NODE_DIR, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    js = json.loads(r.stdout)
+    cfg = gentlePour3D(5, 3600)
+    cfg["motion"]["kind"] = "manual"
+    sim = Simulation3D(cfg)
+    sim.setTarget(35)
+    for _ in range(60):
+        sim.step(min(sim.stableDt(), cfg["duration"] - sim.time))
+    py = sim.readRenderPositions()
+    d = sim.diagnostics()
+    assert js["steps"] == 60 == d["steps"] and js["time"] == d["time"]
+    assert js["angle"] == d["angleDeg"] and js["phase"] == d["phase"] == "manual"
+    assert js["angle"] > 0 and js["origin"] == sim.n == js["n"]
+    assert np.array_equal(np.fromfile(tmp_path / "render.bin", np.float32), py)
+    sim.destroy()

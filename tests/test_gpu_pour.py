@@ -15,12 +15,13 @@ def _row(pins, cell):
     return next(r for r in pins["run3d_rows"] if r["cellMm"] == cell)
 
 
-@pytest.mark.parametrize("cell,vol_tol,share_tol,precond", [
-    (5, 0.01, 0.02, "mic0"), (4, 0.003, 0.013, "mic0"), (4, 0.003, 0.013, "jacobi"),
-    (4, 0.003, 0.013, "mg"), (4, 0.003, 0.013, "chebyshev")])
-def test_full_pour_matches_cpu_reference(pins, cell, vol_tol, share_tol, precond):
+@pytest.mark.parametrize("cell,vol_tol,share_tol,precond,extra", [
+    (5, 0.01, 0.02, "mic0", {}), (4, 0.003, 0.013, "mic0", {}), (4, 0.003, 0.013, "jacobi", {}),
+    (4, 0.003, 0.013, "mg", {}), (4, 0.003, 0.013, "chebyshev", {}),
+    (4, 0.003, 0.013, "chebyshev", {"warmStart": 1, "chebDegree": 12, "chebLower": 0.01})])
+def test_full_pour_matches_cpu_reference(pins, cell, vol_tol, share_tol, precond, extra):
     ref = _row(pins, cell)
-    out = run_pour(cell, 20, precond)
+    out = run_pour(cell, 20, precond, extra=extra)
     r = out["result"]
     print(f"{cell} mm {precond}: {r['dischargedMl']:.2f} mL vs {ref['dischargedMl']:.2f}; "
           f"shares {out['dischargedFractions']} vs {ref['dischargedFractions']}; "
