@@ -611,43 +611,68 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   const int ce = (m + NT - 1) / NT;
   const int tb = ((tid >> 6) * ce) * 64 + (tid & 63);
 
-  // registers per slot: r, x_lo, and the six neighbour slots as two 14-bit LDS indices per word
-  // (bits 0-13 and 16-29; missing neighbours -> ZSLOT).  The diagonal (0..6) rides in the spare
-  // bits 14-15 of word 0 and bit 14 of word 1, which saves a register per slot.  The words are
-  // made opaque at the top of each iteration so the compiler cannot hoist their decoding out of
-  // the loop (that would keep six addresses per slot live).
+  // registers per slot: r, x_lo, and the six neighbours as two 16-bit LDS byte offsets per word
+  // (missing neighbours -> the zero slot), so a neighbour load is one mask or shift and a
+  // ds_read with the vector's base as immediate offset.  The diagonals (0..6) of a thread's
+  // slots are packed 3 bits each into dgp, 10 per register.  Words and diagonals are made opaque
+  // at the top of each iteration so the compiler cannot hoist their decoding out of the loop
+  // (that would keep six addresses per slot live).
   float r[CPT], xlo[CPT];
   uint32_t nbc[CPT][3];
+  constexpr int DGW = (CPT + 9) / 10;
+  uint32_t dgp[DGW];
   double bmax = 0.0;
   auto local_word = [](uint32_t w) -> uint32_t {
     const uint32_t lo = w & 0xFFFFu, hi = w >> 16;
-    return (lo < NB_AIR ? lo : uint32_t(ZSLOT)) | ((hi < NB_AIR ? hi : uint32_t(ZSLOT)) << 16);
+    return ((lo < NB_AIR ? lo : uint32_t(ZSLOT)) << 2) |
+           ((hi < NB_AIR ? hi : uint32_t(ZSLOT)) << 18);
   };
-  constexpr uint32_t Z2 = uint32_t(ZSLOT) | (uint32_t(ZSLOT) << 16);
-  // padding slots: diagonal 6 (= 2 | 1 << 2), no coupling, r = 0
-  constexpr uint32_t D6W0 = 2u << 14, D6W1 = 1u << 14;
+  constexpr uint32_t Z2 = (uint32_t(ZSLOT) << 2) | (uint32_t(ZSLOT) << 18);
+  // padding slots: diagonal 6, no coupling, r = 0
+#pragma unroll
+  for (int w = 0; w < DGW; w++) dgp[w] = 0u;
 #pragma unroll
   for (int k = 0; k < CPT; k++) {
     r[k] = xlo[k] = 0.f;
-    nbc[k][0] = Z2 | D6W0;
-    nbc[k][1] = Z2 | D6W1;
-    nbc[k][2] = Z2;
+    nbc[k][0] = nbc[k][1] = nbc[k][2] = Z2;
+    dgp[k / 10] |= 6u << (3 * (k % 10));
   }
   FOR_SLOTS({
     const uint32_t wx = nbx[tc], wy = nby[tc], wz = nbz[tc];
     const double bl = b[tc];
     const uint32_t d = own ? uint32_t(diag_of(wx, wy, wz)) : 6u;
-    nbc[k][0] = (own ? local_word(wx) : Z2) | ((d & 3u) << 14);
-    nbc[k][1] = (own ? local_word(wy) : Z2) | ((d >> 2) << 14);
+    nbc[k][0] = own ? local_word(wx) : Z2;
+    nbc[k][1] = own ? local_word(wy) : Z2;
     nbc[k][2] = own ? local_word(wz) : Z2;
+    dgp[k / 10] = (dgp[k / 10] & ~(7u << (3 * (k % 10)))) | (d << (3 * (k % 10)));
     const double bt = own ? bl : 0.0;
     r[k] = float(bt);
     bmax = fmax(bmax, fabs(bt));
     if (own) xhi[t] = 0.0;
   })
-  // the diagonal of slot k, and the two LDS indices of a packed word
-  auto dgv = [&](int k) -> float {
-    return float(((nbc[k][0] >> 14) & 3u) | (((nbc[k][1] >> 14) & 1u) << 2));
+  // the diagonal of slot k
+  auto dgv = [&](int k) -> float { return float((dgp[k / 10] >> (3 * (k % 10))) & 7u); };
+  // the LDS float at a byte offset from the start of an LDS vector
+  auto lds_at = [](const float *v, uint32_t off) -> float {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(v) + off);
+  };
+  // the six neighbour values of slot k from an LDS vector (missing ones read the zero slot)
+  auto gather6 = [&](const float *__restrict__ v, int k, float *g) {
+    const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
+    g[0] = lds_at(v, px >> 16);
+    g[1] = lds_at(v, px & 0xFFFFu);
+    g[2] = lds_at(v, py >> 16);
+    g[3] = lds_at(v, py & 0xFFFFu);
+    g[4] = lds_at(v, pz >> 16);
+    g[5] = lds_at(v, pz & 0xFFFFu);
+  };
+  // keep the words and diagonals opaque (see above)
+  auto opaque = [&]() {
+#pragma unroll
+    for (int k = 0; k < CPT; k++)
+      asm volatile("" : "+v"(nbc[k][0]), "+v"(nbc[k][1]), "+v"(nbc[k][2]));
+#pragma unroll
+    for (int w = 0; w < DGW; w++) asm volatile("" : "+v"(dgp[w]));
   };
   if (tid == 0) {
     s_lds[ZSLOT] = 0.f;
@@ -774,8 +799,9 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
     FOR_SLOTS({
       if (((colmask >> k) & 1u) == c) {
         const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
-        const float sum = ((W[px >> 16] + W[px & 0x3FFFu]) + (W[py >> 16] + W[py & 0x3FFFu])) +
-                          (W[pz >> 16] + W[pz & 0x3FFFu]);
+        const float sum = ((lds_at(W, px >> 16) + lds_at(W, px & 0xFFFFu)) +
+                           (lds_at(W, py >> 16) + lds_at(W, py & 0xFFFFu))) +
+                          (lds_at(W, pz >> 16) + lds_at(W, pz & 0xFFFFu));
         W[t] = (r[k] + sum) * __builtin_amdgcn_rcpf(dgv(k));
       }
     })
@@ -815,28 +841,36 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
         W[t] = d;
       })
       lds_barrier();
-      float *cur = W, *nxt = s_lds;
-      for (int deg = 1; deg < chebDeg; deg++) {
-        // opaque words: keep the neighbour decoding inside the loop (see the PCG loop)
-#pragma unroll
-        for (int k = 0; k < CPT; k++)
-          asm volatile("" : "+v"(nbc[k][0]), "+v"(nbc[k][1]), "+v"(nbc[k][2]));
+      // one step from the z in src to the z in dst (src and dst are the two LDS vectors, fixed
+      // per call site, so their bases fold into the loads' offsets)
+      auto step = [&](const float *__restrict__ cur, float *__restrict__ nxt) {
+        opaque();
         const float rhoN = 1.0f / (2.0f * sigma1 - rho);
         const float c1 = rhoN * rho, c2 = 2.0f * rhoN / chebDelta;
-        FOR_SLOTS({
-          const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
-          const float az = dgv(k) * zreg[k] - (((cur[px >> 16] + cur[px & 0x3FFFu]) +
-                                                (cur[py >> 16] + cur[py & 0x3FFFu])) +
-                                               (cur[pz >> 16] + cur[pz & 0x3FFFu]));
-          dreg[k] = c1 * dreg[k] + c2 * (r[k] - az) * __builtin_amdgcn_rcpf(dgv(k));
-          zreg[k] += dreg[k];
-          nxt[t] = zreg[k];
-        })
+        // the six neighbour values of slot k + 1 are loaded before slot k's arithmetic, so the
+        // LDS latency of one slot overlaps the work of the previous one
+        float g0[6], g1[6];
+        gather6(cur, 0, g0);
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+          if (k < ce) {
+            float *gc = (k & 1) ? g1 : g0;
+            float *gn = (k & 1) ? g0 : g1;
+            if (k + 1 < CPT && k + 1 < ce) gather6(cur, k + 1, gn);
+            const int t = tb + k * 64;
+            const float az =
+                dgv(k) * zreg[k] - (((gc[0] + gc[1]) + (gc[2] + gc[3])) + (gc[4] + gc[5]));
+            dreg[k] = c1 * dreg[k] + c2 * (r[k] - az) * __builtin_amdgcn_rcpf(dgv(k));
+            zreg[k] += dreg[k];
+            nxt[t] = zreg[k];
+          }
+        }
         lds_barrier();  // nxt complete; every read of cur done
-        float *tmp = cur;
-        cur = nxt;
-        nxt = tmp;
         rho = rhoN;
+      };
+      for (int deg = 1; deg < chebDeg; deg += 2) {
+        step(W, s_lds);
+        if (deg + 1 < chebDeg) step(s_lds, W);
       }
     } else if constexpr (PC == PC_MG) {
       if (!L.x.ok) {  // hierarchy over capacity: Jacobi for this solve
@@ -853,8 +887,9 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
       if (L.x.nl > 1) {
         FOR_SLOTS({
           const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
-          const float sum = ((W[px >> 16] + W[px & 0x3FFFu]) + (W[py >> 16] + W[py & 0x3FFFu])) +
-                            (W[pz >> 16] + W[pz & 0x3FFFu]);
+          const float sum = ((lds_at(W, px >> 16) + lds_at(W, px & 0xFFFFu)) +
+                             (lds_at(W, py >> 16) + lds_at(W, py & 0xFFFFu))) +
+                            (lds_at(W, pz >> 16) + lds_at(W, pz & 0xFFFFu));
           if (own) res0[t] = r[k] - (dgv(k) * W[t] - sum);
         })
         __syncthreads();  // global res0 and LDS
@@ -950,20 +985,36 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   int lastUpdate = 0;
   PHASE_T0();
   for (int it = 1; it <= D.maxIter; it++) {
-#pragma unroll
-    for (int k = 0; k < CPT; k++) asm volatile("" : "+v"(nbc[k][0]), "+v"(nbc[k][1]), "+v"(nbc[k][2]));
+    opaque();
     __syncthreads();  // s complete
     PHASE_MARK(0);
     double sas_part = 0.0;
-    FOR_SLOTS({  // A s = diag s_t - sum of the liquid neighbours (missing ones read the zero slot)
-      const uint32_t px = nbc[k][0], py = nbc[k][1], pz = nbc[k][2];
-      const float st = s_lds[t];
-      const float acc = dgv(k) * st - (((s_lds[px >> 16] + s_lds[px & 0x3FFFu]) +
-                                       (s_lds[py >> 16] + s_lds[py & 0x3FFFu])) +
-                                      (s_lds[pz >> 16] + s_lds[pz & 0x3FFFu]));
-      W[t] = acc;  // A s, read back by the owner only
-      sas_part += double(st) * double(acc);
-    })
+    {  // A s = diag s_t - sum of the liquid neighbours (missing ones read the zero slot); slot
+       // k + 1's loads are issued before slot k's arithmetic
+      float g0[6], g1[6];
+      float sa = s_lds[tb], sb = 0.f;
+      gather6(s_lds, 0, g0);
+#pragma unroll
+      for (int k = 0; k < CPT; k++) {
+        if (k < ce) {
+          float *gc = (k & 1) ? g1 : g0;
+          float *gn = (k & 1) ? g0 : g1;
+          const float st = (k & 1) ? sb : sa;
+          if (k + 1 < CPT && k + 1 < ce) {
+            gather6(s_lds, k + 1, gn);
+            if (k & 1)
+              sa = s_lds[tb + (k + 1) * 64];
+            else
+              sb = s_lds[tb + (k + 1) * 64];
+          }
+          const int t = tb + k * 64;
+          const float acc =
+              dgv(k) * st - (((gc[0] + gc[1]) + (gc[2] + gc[3])) + (gc[4] + gc[5]));
+          W[t] = acc;  // A s, read back by the owner only
+          sas_part += double(st) * double(acc);
+        }
+      }
+    }
     PHASE_MARK(1);
     const double sas = block_sum(R, rb, sas_part);
     PHASE_MARK(2);
