@@ -594,7 +594,7 @@ __device__ __attribute__((noinline)) void mg_coarse(MgLds &G, unsigned char *bas
     }                                                    \
   }
 
-template <int PC, int CPT>
+template <int PC, int CPT, bool EXACT = false>
 __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, const PcgDesc &D,
                                          const int m) {
   constexpr bool MIC = PC == PC_MIC;
@@ -608,7 +608,8 @@ __device__ __forceinline__ void pcg_body(SolveLds<PC> &L, const PcgShared &A, co
   const uint32_t *__restrict__ nbx = A.nbr;
   const uint32_t *__restrict__ nby = A.nbr + A.stride;
   const uint32_t *__restrict__ nbz = A.nbr + 2 * A.stride;
-  const int ce = (m + NT - 1) / NT;
+  // slots per thread; EXACT bodies are compiled for ce == CPT, so the slot guards fold away
+  const int ce = EXACT ? CPT : (m + NT - 1) / NT;
   const int tb = ((tid >> 6) * ce) * 64 + (tid & 63);
 
   // registers per slot: r, x_lo, and the six neighbours as two 16-bit LDS byte offsets per word
@@ -1137,6 +1138,18 @@ __device__ __forceinline__ void pcg_entry(SolveLds<PC> &L, const PcgShared *__re
     return;
   }
   const int ce = (m + NT - 1) / NT;
+  if constexpr (PC == PC_CHEB) {  // the fast path: bodies for the exact slot count of 6..12
+    switch (ce) {
+      case 6: pcg_body<PC, 6, true>(L, A, D, m); return;
+      case 7: pcg_body<PC, 7, true>(L, A, D, m); return;
+      case 8: pcg_body<PC, 8, true>(L, A, D, m); return;
+      case 9: pcg_body<PC, 9, true>(L, A, D, m); return;
+      case 10: pcg_body<PC, 10, true>(L, A, D, m); return;
+      case 11: pcg_body<PC, 11, true>(L, A, D, m); return;
+      case 12: pcg_body<PC, 12, true>(L, A, D, m); return;
+      default: break;
+    }
+  }
   if (ce <= 8)
     pcg_body<PC, 8>(L, A, D, m);
   else if (ce <= 10)
