@@ -249,6 +249,9 @@ def poisson_bench(a):
     t0 = time.perf_counter()
     ps = PoissonSet(n, P, 1000)
     gen_s = time.perf_counter() - t0
+    pc = a.poisson_precond
+    if pc == "chebyshev":
+        ps.setPreconditioner("chebyshev", a.poisson_degree, a.poisson_lower)
     ps.solve("refined", 1e-6, 5)  # warm-up (allocation, first launches)
     runs = {}
     for prec in ("fp64", "refined", "fp32"):
@@ -263,9 +266,14 @@ def poisson_bench(a):
     iters = max(x["iterations"] for x in r)
     kms = r[0]["kernelMs"]
     cells = n ** 3 * P
-    # bytes of the lockstep iterations actually run (a problem stops streaming once converged)
-    it_bytes = sum((PS_VECTOR_BYTES * mi) * x["iterations"] for mi, x in zip(m, r)) + \
-        PS_FLAG_BYTES * cells * iters
+    # bytes of the lockstep iterations actually run (a problem stops streaming once converged);
+    # each Chebyshev step after the first streams z (neighbours), r, d and writes d, z (20 B per
+    # FLUID cell) plus the cells' info words, the first step 12 B, and z.r reads z and r (8 B)
+    deg = a.poisson_degree if pc == "chebyshev" else 0
+    per_cell = PS_VECTOR_BYTES + (12.0 + 20.0 * (deg - 1) + 8.0 if deg else 0.0)
+    per_flag = PS_FLAG_BYTES + (2.0 * (deg + 1) if deg else 0.0)
+    it_bytes = sum((per_cell * mi) * x["iterations"] for mi, x in zip(m, r)) + \
+        per_flag * cells * iters
     achieved = it_bytes / (kms / 1000) / 1e9
     summary = {}
     for prec, (rr, w) in runs.items():
@@ -280,7 +288,7 @@ def poisson_bench(a):
         }
     line = {
         "metric": f"Poisson set: {P} problems on {n}^3 solved to 1e-6 per second (FP32 + FP64 "
-                  "refinement, Jacobi-PCG)",
+                  f"refinement, {'Chebyshev' if pc == 'chebyshev' else 'Jacobi'}-PCG)",
         "value": P / wall,
         "unit": "problems/s",
         "n_gpus": 1,
@@ -295,14 +303,16 @@ def poisson_bench(a):
                 "generated on the device",
         "config": {"workload": f"{P} x {n}^3 pressure-Poisson problems, liquid cells "
                                f"{min(m):,}-{max(m):,} (mean {sum(m) / len(m):,.0f})",
-                   "generation_s": gen_s},
+                   "generation_s": gen_s,
+                   "preconditioner": pc if pc != "chebyshev" else
+                   f"chebyshev degree {a.poisson_degree} on [{a.poisson_lower}, 2]"},
         "precisions": summary,
         "roofline": {"kernel": "batched PCG iteration (k_ps_spmv, k_ps_update, k_ps_dir + "
                                "per-problem reductions)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_model": f"{PS_VECTOR_BYTES:.0f} B per FLUID cell per iteration + "
-                                    f"{PS_FLAG_BYTES:.0f} B of flags per cell per iteration"},
+                     "bytes_model": f"{per_cell:.0f} B per FLUID cell per iteration + "
+                                    f"{per_flag:.0f} B of flags per cell per iteration"},
         "cpu_baseline": None,
     }
     print(json.dumps(line))
@@ -334,6 +344,9 @@ def main():
     ap.add_argument("--poisson-n", type=int, default=128)
     ap.add_argument("--poisson-problems", type=int, default=256)
     ap.add_argument("--poisson-max-iter", type=int, default=4000)
+    ap.add_argument("--poisson-precond", default="chebyshev", choices=["jacobi", "chebyshev"])
+    ap.add_argument("--poisson-degree", type=int, default=8)
+    ap.add_argument("--poisson-lower", type=float, default=0.01)
     ap.add_argument("--ensemble", type=int, default=0,
                     help="K > 0: run K varied pours as one ensemble instead of the 4 mm pour")
     a = ap.parse_args()

@@ -332,6 +332,10 @@ int pl3d_poisson_problem(pl3d_poisson *ps, int32_t k, uint8_t *flags, float *rhs
                          int64_t *liquid_cells);
 /* Solve every problem to max|r| <= max(tol max|b|, 1e-14) (solver3d.ts:910) at one precision;
  * out receives `problems` results.  The solution is kept per precision. */
+/* the preconditioner of later solves: PL3D_PRECOND_JACOBI (the default) or
+ * PL3D_PRECOND_CHEBYSHEV with a degree and interval lower bound (0: the library defaults) */
+int pl3d_poisson_set_preconditioner(pl3d_poisson *ps, int32_t precond, int32_t degree,
+                                    double lower);
 int pl3d_poisson_solve(pl3d_poisson *ps, int32_t precision, double tol, int32_t max_iter,
                        pl3d_poisson_result *out);
 /* the kept solution of problem k at a precision, n^3 doubles (0 off the liquid) */

@@ -59,6 +59,7 @@ SIGNATURES = {
     "pl3d_recorder_parcels": (C.c_int, [_vp, _P(_i32), _vp]),
     "pl3d_recorder_capture": (C.c_int, [_vp, _vp]),
     "pl3d_recorder_destroy": (C.c_int, [_vp]),
+    "pl3d_poisson_set_preconditioner": (C.c_int, [_vp, _i32, _i32, _d]),
     "pl3d_poisson_create": (C.c_int, [_i32, _i32, C.c_uint32, _P(_vp)]),
     "pl3d_poisson_destroy": (C.c_int, [_vp]),
     "pl3d_poisson_problem": (C.c_int, [_vp, _i32, _vp, _vp, _P(C.c_int64)]),

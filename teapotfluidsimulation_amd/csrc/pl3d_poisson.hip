@@ -541,6 +541,127 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_sigma(const double *partial, 
   }
 }
 
+// ---- Chebyshev preconditioner (the polynomial of the single-pour solver, on all problems):
+// step 0: d = r / (diag theta), z = d; step j: d = c1 d + c2 (r - A z) / diag, z' = z + d with
+// z read from zin (neighbours) and written to zout (the two z buffers alternate).
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_cheb0(const uint16_t *__restrict__ infov,
+                                                         const T *__restrict__ rv,
+                                                         T *__restrict__ zv, T *__restrict__ dv,
+                                                         const Scal *__restrict__ sc, int n,
+                                                         float theta) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const int64_t safe = 1 + n + int64_t(n) * n;
+  const uint16_t *info = infov + base;
+  const T *r = rv + base;
+  T *z = zv + base, *d = dv + base;
+  PS_LOAD_INFO();
+  if (!waveLiquid) return;
+  T rs[PS_CPT];
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) rs[q] = r[cc[q]];
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++)
+    if (inf[q] & 0x8000u) {
+      const T v = rs[q] / (T(PS_DIAG(inf[q])) * T(theta));
+      z[cc[q]] = v;
+      d[cc[q]] = v;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_cheb(const uint16_t *__restrict__ infov,
+                                                        const T *__restrict__ zin,
+                                                        const T *__restrict__ rv,
+                                                        T *__restrict__ dv, T *__restrict__ zout,
+                                                        const Scal *__restrict__ sc, int n,
+                                                        float c1, float c2) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const int64_t safe = 1 + n + int64_t(n) * n;
+  const uint16_t *info = infov + base;
+  const T *z = zin + base, *r = rv + base;
+  T *d = dv + base, *zo = zout + base;
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  PS_LOAD_INFO();
+  if (!waveLiquid) return;
+  T z0[PS_CPT], nb[PS_CPT][6], rs[PS_CPT], ds[PS_CPT];
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) {
+    z0[q] = z[cc[q]];
+    rs[q] = r[cc[q]];
+    ds[q] = d[cc[q]];
+#pragma unroll
+    for (int o = 0; o < 6; o++) nb[q][o] = z[cc[q] + off[o]];
+  }
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) {
+    T sum = T(0);
+#pragma unroll
+    for (int o = 0; o < 6; o++) sum += PS_NB(inf[q], o) ? nb[q][o] : T(0);
+    const T dg = T(PS_DIAG(inf[q]));
+    const T dn = T(c1) * ds[q] + T(c2) * (rs[q] - (dg * z0[q] - sum)) / dg;
+    if (inf[q] & 0x8000u) {
+      d[cc[q]] = dn;
+      zo[cc[q]] = z0[q] + dn;
+    }
+  }
+}
+
+// partial z.r over the FLUID cells (z from the preconditioner); only restart-flagged problems
+// with onlyRestart
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_zdot(const uint16_t *__restrict__ infov,
+                                                        const T *__restrict__ zv,
+                                                        const T *__restrict__ rv,
+                                                        double *__restrict__ partial,
+                                                        const Scal *__restrict__ sc, int n,
+                                                        int bpp, int onlyRestart) {
+  __shared__ double sh[PS_THREADS / 64];
+  const int p = blockIdx.y;
+  if (!sc[p].active || (onlyRestart && !sc[p].restart)) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const uint16_t *info = infov + base;
+  const T *z = zv + base, *r = rv + base;
+  double acc = 0.0;
+  PS_CELLS({ acc += double(z[c]) * double(r[c]); })
+  const double t = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) partial[int64_t(p) * bpp + blockIdx.x] = t;
+}
+
+// s = z + beta s (mode 0, beta in the alpha slot), s = z for restart-flagged problems (mode 1)
+// or for every active problem (mode 2), z from the preconditioner
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_ps_dirz(const uint16_t *__restrict__ infov,
+                                                        const T *__restrict__ zv,
+                                                        T *__restrict__ sv,
+                                                        const Scal *__restrict__ sc, int n,
+                                                        int mode) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  if (mode == 1 && !sc[p].restart) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const int64_t safe = 1 + n + int64_t(n) * n;
+  const uint16_t *info = infov + base;
+  const T *z = zv + base;
+  T *s = sv + base;
+  const T beta = mode == 0 ? T(sc[p].alpha) : T(0);
+  PS_LOAD_INFO();
+  if (!waveLiquid) return;
+  T zs[PS_CPT], ss[PS_CPT];
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) {
+    zs[q] = z[cc[q]];
+    ss[q] = mode == 0 ? s[cc[q]] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++)
+    if (inf[q] & 0x8000u) s[cc[q]] = zs[q] + beta * ss[q];
+}
+
 // start: r = b, x = 0 (and x_hi = 0), per-block max|b|
 template <class T>
 __global__ __launch_bounds__(PS_THREADS) void k_ps_init(const uint8_t *flags, const float *bv,
@@ -656,13 +777,17 @@ struct pl3d_poisson {
   double *partial = nullptr, *partial2 = nullptr;
   Scal *sc = nullptr;
   double *sol[3] = {nullptr, nullptr, nullptr};
+  // Chebyshev preconditioner (PL3D_PRECOND_CHEBYSHEV) or Jacobi (PL3D_PRECOND_JACOBI)
+  int precond = PL3D_PRECOND_JACOBI, chebDegree = 6;
+  double chebLower = 0.05;
+  void *za = nullptr, *zb = nullptr, *d = nullptr;  // its work vectors (element size workElem)
   bool destroyed = false;
 };
 
 namespace {
 
 void free_work(pl3d_poisson *ps) {
-  for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As}) {
+  for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As, &ps->za, &ps->zb, &ps->d}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
@@ -678,6 +803,8 @@ int alloc_work(pl3d_poisson *ps, size_t elem, bool hi) {
     for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As}) PS_HIP(hipMalloc(p, cells * elem));
     ps->workElem = elem;
   }
+  if (ps->precond == PL3D_PRECOND_CHEBYSHEV && !ps->za)
+    for (void **p : {&ps->za, &ps->zb, &ps->d}) PS_HIP(hipMalloc(p, cells * elem));
   if (hi && !ps->xhi) PS_HIP(hipMalloc((void **)&ps->xhi, cells * 8));
   return PL3D_OK;
 }
@@ -695,6 +822,25 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
   T *x = static_cast<T *>(ps->x), *r = static_cast<T *>(ps->r), *s = static_cast<T *>(ps->s),
     *As = static_cast<T *>(ps->As);
   const dim3 grid(bpp, P), blk(PS_THREADS);
+  const bool cheb = ps->precond == PL3D_PRECOND_CHEBYSHEV;
+  T *za = static_cast<T *>(ps->za), *zb = static_cast<T *>(ps->zb), *dd = static_cast<T *>(ps->d);
+  // z = M r for the active problems; returns the buffer holding z
+  auto chebyshev = [&]() -> T * {
+    const double theta = 0.5 * (2.0 + ps->chebLower), delta = 0.5 * (2.0 - ps->chebLower);
+    const double sigma1 = theta / delta;
+    double rho = 1.0 / sigma1;
+    hipLaunchKernelGGL(k_ps_cheb0<T>, grid, blk, 0, st, ps->info, r, za, dd, ps->sc, n,
+                       float(theta));
+    T *cur = za, *nxt = zb;
+    for (int k = 1; k < ps->chebDegree; k++) {
+      const double rhoN = 1.0 / (2.0 * sigma1 - rho);
+      hipLaunchKernelGGL(k_ps_cheb<T>, grid, blk, 0, st, ps->info, cur, r, dd, nxt, ps->sc, n,
+                         float(rhoN * rho), float(2.0 * rhoN / delta));
+      std::swap(cur, nxt);
+      rho = rhoN;
+    }
+    return cur;
+  };
   hipEvent_t e0, e1;
   PS_HIP(hipEventCreate(&e0));
   PS_HIP(hipEventCreate(&e1));
@@ -702,8 +848,15 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
   hipLaunchKernelGGL(k_ps_init<T>, grid, blk, 0, st, ps->flags, ps->b, x, r,
                      refined ? ps->xhi : nullptr, ps->partial, n, bpp);
   hipLaunchKernelGGL(k_ps_start, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, tol);
-  hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 2);  // s = z
-  hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n, bpp, 0);
+  if (cheb) {
+    T *z = chebyshev();
+    hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, z, s, ps->sc, n, 2);  // s = z
+    hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, z, r, ps->partial, ps->sc, n,
+                       bpp, 0);
+  } else {
+    hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 2);  // s = z
+    hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n, bpp, 0);
+  }
   hipLaunchKernelGGL(k_ps_sigma, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, 0);
   PS_HIP(hipGetLastError());
   std::vector<Scal> h(P);
@@ -714,9 +867,18 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
     hipLaunchKernelGGL(k_ps_alpha, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp);
     hipLaunchKernelGGL(k_ps_update<T>, grid, blk, 0, st, ps->info, x, r, s, As, ps->partial,
                        ps->partial2, ps->sc, n, bpp);
+    T *z = nullptr;
+    if (cheb) {  // z = M r replaces the update's Jacobi z.r partials
+      z = chebyshev();
+      hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, z, r, ps->partial, ps->sc, n,
+                         bpp, 0);
+    }
     hipLaunchKernelGGL(k_ps_beta, dim3(P), blk, 0, st, ps->partial, ps->partial2, ps->sc, bpp,
                        refined ? 1 : 0);
-    hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 0);
+    if (cheb)
+      hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, z, s, ps->sc, n, 0);
+    else
+      hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 0);
     PS_HIP(hipGetLastError());
     sinceRefresh++;
     const bool poll = it % 5 == 0 || it == maxIter;
@@ -742,9 +904,16 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
         hipLaunchKernelGGL(k_ps_refresh_verdict, dim3(P), blk, 0, st, ps->partial, ps->partial2,
                            ps->sc, bpp);
         // restart where flagged: s = z, sigma = z.r
-        hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 1);
-        hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n,
-                           bpp, 1);
+        if (cheb) {
+          T *zr = chebyshev();
+          hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, zr, s, ps->sc, n, 1);
+          hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, zr, r, ps->partial,
+                             ps->sc, n, bpp, 1);
+        } else {
+          hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 1);
+          hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n,
+                             bpp, 1);
+        }
         hipLaunchKernelGGL(k_ps_sigma, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, 1);
         PS_HIP(hipGetLastError());
       }
@@ -867,6 +1036,19 @@ int pl3d_poisson_problem(pl3d_poisson *ps, int32_t k, uint8_t *flags, float *rhs
     PS_HIP(hipMemcpyAsync(rhs, ps->b + off, ps->N * 4, hipMemcpyDeviceToHost, ps->stream));
   PS_HIP(hipStreamSynchronize(ps->stream));
   if (liquid_cells) *liquid_cells = ps->liquid[k];
+  return PL3D_OK;
+}
+
+int pl3d_poisson_set_preconditioner(pl3d_poisson *ps, int32_t precond, int32_t degree,
+                                    double lower) {
+  if (int rc = check_ps(ps)) return rc;
+  if (precond != PL3D_PRECOND_JACOBI && precond != PL3D_PRECOND_CHEBYSHEV)
+    return perr(PL3D_EINVAL, "the Poisson set solves with Jacobi or Chebyshev");
+  if (degree < 0 || degree > 64 || !(lower >= 0.0 && lower < 2.0))
+    return perr(PL3D_EINVAL, "Chebyshev degree must be in 0..64 and its lower bound in [0, 2)");
+  ps->precond = precond;
+  ps->chebDegree = degree > 0 ? degree : PL3D_CHEB_DEGREE_DEFAULT;
+  ps->chebLower = lower > 0.0 ? lower : PL3D_CHEB_LOWER_DEFAULT;
   return PL3D_OK;
 }
 

@@ -501,6 +501,12 @@ class PoissonSet:
         check(load().pl3d_poisson_problem(self._handle(), int(k), _ptr(fl), _ptr(b), C.byref(m)))
         return fl, b, m.value
 
+    def setPreconditioner(self, name: str = "jacobi", degree: int = 0, lower: float = 0.0):
+        """"jacobi" (default) or "chebyshev" (degree, interval [lower, 2]; 0: library defaults)
+        for the following solves."""
+        check(load().pl3d_poisson_set_preconditioner(self._handle(), abi.PRECONDITIONERS[name],
+                                                    int(degree), float(lower)))
+
     def solve(self, precision: str = "refined", tol: float = 1e-6, maxIter: int = 5000) -> list:
         out = (abi.PoissonResult * self.problems)()
         check(load().pl3d_poisson_solve(self._handle(), abi.POISSON_PRECISIONS[precision],
