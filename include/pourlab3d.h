@@ -67,6 +67,10 @@ extern "C" {
 #define PL3D_MOTION_POUR 1
 #define PL3D_MOTION_TILT 2
 #define PL3D_MOTION_MANUAL 3
+/* build-only (not a reference MotionConfig.kind): the tilt-in is a constant-rate ramp at
+ * hold_tilt_rate deg/s up to max_angle_deg (e.g. 90), held there until the volume target or
+ * max_hold, then the usual return */
+#define PL3D_MOTION_RAMP 4
 
 /* MotionPhase (sim/motion.ts:10); pl3d_phase_name() gives the reference's strings */
 #define PL3D_PHASE_WAIT 0
@@ -290,6 +294,8 @@ typedef struct pl3d_ensemble pl3d_ensemble;
  * pot would hold more than ~11,000 liquid cells), the rest as gentlePour3D(., 20).  Build-only:
  * the reference has one preset (presets3d.ts:9-26). */
 int pl3d_varied_pour(uint32_t seed, int32_t index, pl3d_config *out);
+/* the gentle pour with a ramp motion: tilt at rate_deg_s up to max_angle_deg (build-only) */
+int pl3d_ramp_pour(double cell_mm, double rate_deg_s, double max_angle_deg, pl3d_config *out);
 int pl3d_ensemble_create(const pl3d_config *cfgs, int32_t count, pl3d_ensemble **out);
 int pl3d_ensemble_destroy(pl3d_ensemble *ens);
 int pl3d_ensemble_size(pl3d_ensemble *ens, int32_t *count);

@@ -191,7 +191,14 @@ struct Controller {
       } else if ((phase == PL3D_PHASE_TILT_IN || phase == PL3D_PHASE_HOLD) &&
                  should_return(t, discharged)) {
         enter(PL3D_PHASE_RETURN, t);
-      } else if (phase == PL3D_PHASE_TILT_IN && el >= c.tilt_duration) {
+      } else if (phase == PL3D_PHASE_TILT_IN && c.kind == PL3D_MOTION_RAMP &&
+                 el >= (maxA - startAngle) / (c.hold_tilt_rate * DEG)) {
+        // build-only ramp: the constant-rate tilt-in ends at the maximum angle
+        const double tr = (maxA - startAngle) / (c.hold_tilt_rate * DEG);
+        angle = maxA;
+        enter(PL3D_PHASE_HOLD, phaseStart + tr);
+      } else if (phase == PL3D_PHASE_TILT_IN && c.kind != PL3D_MOTION_RAMP &&
+                 el >= c.tilt_duration) {
         angle = hold;
         enter(PL3D_PHASE_HOLD, phaseStart + c.tilt_duration);
       } else if (phase == PL3D_PHASE_RETURN && el >= c.return_duration) {
@@ -208,6 +215,10 @@ struct Controller {
     double el = t - phaseStart;
     switch (phase) {
       case PL3D_PHASE_TILT_IN: {
+        if (c.kind == PL3D_MOTION_RAMP) {
+          angle = std::min(maxA, startAngle + c.hold_tilt_rate * DEG * el);
+          break;
+        }
         double s = std::min(1.0, el / c.tilt_duration);
         angle = startAngle + (hold - startAngle) * 0.5 * (1 - std::cos(PI * s));
         break;

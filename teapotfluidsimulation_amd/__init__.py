@@ -18,5 +18,6 @@ from .sim3d import (  # noqa: F401
     Recorder3D,
     Simulation3D,
     gentlePour3D,
+    rampPour3D,
     variedPour3D,
 )

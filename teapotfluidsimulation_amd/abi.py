@@ -14,7 +14,7 @@ ESTATE = -4
 AIR, FLUID, SOLID = 0, 1, 2
 IN_POT, DISCHARGED, SPILLED, LOST = 0, 1, 2, 3
 
-MOTION_KINDS = {"still": 0, "pour": 1, "tilt": 2, "manual": 3}
+MOTION_KINDS = {"still": 0, "pour": 1, "tilt": 2, "manual": 3, "ramp": 4}
 PHASES = ["wait", "tilt-in", "hold", "return", "rest", "done", "manual"]
 PRECONDITIONERS = {"mic0": 0, "mg": 1, "jacobi": 2, "chebyshev": 3}
 TRANSFERS = {"flip": 0, "apic": 1}

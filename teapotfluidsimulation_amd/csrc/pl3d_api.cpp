@@ -404,6 +404,16 @@ int pl3d_gentle_pour(double cell_mm, double seconds, pl3d_config *out) {
   return PL3D_OK;
 }
 
+int pl3d_ramp_pour(double cell_mm, double rate_deg_s, double max_angle_deg, pl3d_config *out) {
+  if (!out || !(cell_mm > 0) || !(rate_deg_s > 0) || !(max_angle_deg > 0 && max_angle_deg <= 90))
+    return fail(PL3D_EINVAL, "bad ramp pour arguments");
+  fill_gentle(cell_mm, 20, out);
+  out->motion.kind = PL3D_MOTION_RAMP;
+  out->motion.hold_tilt_rate = rate_deg_s;
+  out->motion.max_angle_deg = max_angle_deg;
+  return PL3D_OK;
+}
+
 int pl3d_varied_pour(uint32_t seed, int32_t index, pl3d_config *out) {
   if (!out || index < 0) return fail(PL3D_EINVAL, "bad varied pour arguments");
   // one mulberry32 stream from `seed`; member `index` takes the 8 draws after those of the
@@ -445,6 +455,10 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     return fail(PL3D_EINVAL, "preconditioner %d is not available", c.preconditioner);
   if (c.cheb_degree < 0 || c.cheb_degree > 64 || !(c.cheb_lower >= 0.0 && c.cheb_lower < 2.0))
     return fail(PL3D_EINVAL, "Chebyshev degree must be in 0..64 and its lower bound in [0, 2)");
+  if (c.motion.kind < PL3D_MOTION_STILL || c.motion.kind > PL3D_MOTION_RAMP ||
+      (c.motion.kind == PL3D_MOTION_RAMP && !(c.motion.hold_tilt_rate > 0)))
+    return fail(PL3D_EINVAL, "motion kind %d is not available (a ramp needs a positive rate)",
+                c.motion.kind);
   if (c.p2g_brick < 0 || c.p2g_brick >= PL3D_P2G_BRICKS)
     return fail(PL3D_EINVAL, "p2g_brick must be in 0..%d", PL3D_P2G_BRICKS - 1);
   if (c.transfer != PL3D_TRANSFER_FLIP && c.transfer != PL3D_TRANSFER_APIC)

@@ -66,7 +66,13 @@ __device__ double controller_advance(DevCtl &C, const DevParams &P, double t, do
       enter(C, 1, P.startDelay);
     } else if ((C.phase == 1 || C.phase == 2) && should_return(C, P, t, discharged)) {
       enter(C, 3, t);
-    } else if (C.phase == 1 && el >= P.tiltDuration) {
+    } else if (C.phase == 1 && P.motionKind == 4 &&
+               el >= (maxA - C.startAngle) / (P.holdTiltRate * DEG)) {
+      // build-only ramp: the constant-rate tilt-in ends at the maximum angle
+      const double tr = (maxA - C.startAngle) / (P.holdTiltRate * DEG);
+      C.angle = maxA;
+      enter(C, 2, C.phaseStart + tr);
+    } else if (C.phase == 1 && P.motionKind != 4 && el >= P.tiltDuration) {
       C.angle = hold;
       enter(C, 2, C.phaseStart + P.tiltDuration);
     } else if (C.phase == 3 && el >= P.returnDuration) {
@@ -84,6 +90,10 @@ __device__ double controller_advance(DevCtl &C, const DevParams &P, double t, do
   const double el = t - C.phaseStart;
   switch (C.phase) {
     case 1: {
+      if (P.motionKind == 4) {  // ramp
+        C.angle = fmin(maxA, C.startAngle + P.holdTiltRate * DEG * el);
+        break;
+      }
       const double s = fmin(1.0, el / P.tiltDuration);
       C.angle = C.startAngle + (hold - C.startAngle) * 0.5 * (1 - cos(PI_D * s));
       break;

@@ -120,7 +120,7 @@ static int get_opt_str(napi_env env, napi_value obj, const char *key, char *buf,
   return 1;
 }
 
-static const char *MOTION_KINDS[] = {"still", "pour", "tilt", "manual"};
+static const char *MOTION_KINDS[] = {"still", "pour", "tilt", "manual", "ramp"};
 static const char *PRECONDS[] = {"mic0", "mg", "jacobi", "chebyshev"};
 static const char *TRANSFERS[] = {"flip", "apic"};
 
@@ -179,9 +179,9 @@ static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
   {
     int r = get_opt_str(env, m, "kind", buf, sizeof buf);
     if (r < 0) return 0;
-    int k = r == 0 ? -1 : name_index(MOTION_KINDS, 4, buf);
+    int k = r == 0 ? -1 : name_index(MOTION_KINDS, 5, buf);
     if (k < 0) {
-      napi_throw_type_error(env, NULL, "Config3D: motion.kind must be 'still', 'pour', 'tilt' or 'manual'");
+      napi_throw_type_error(env, NULL, "Config3D: motion.kind must be 'still', 'pour', 'tilt', 'manual' or 'ramp'");
       return 0;
     }
     c->motion.kind = k;
@@ -259,7 +259,7 @@ static napi_value config_to_js(napi_env env, const pl3d_config *c) {
            set_num(env, t, "spoutLength", c->teapot.spout_length) && set_val(env, o, "teapot", t) &&
            set_num(env, o, "h", c->h) && set_num(env, o, "initialVolume", c->initial_volume) &&
            set_str(env, m, "kind",
-                   c->motion.kind >= 0 && c->motion.kind < 4 ? MOTION_KINDS[c->motion.kind] : "pour") &&
+                   c->motion.kind >= 0 && c->motion.kind < 5 ? MOTION_KINDS[c->motion.kind] : "pour") &&
            set_num(env, m, "startDelay", c->motion.start_delay) &&
            set_num(env, m, "tiltDuration", c->motion.tilt_duration) &&
            set_num(env, m, "holdAngleDeg", c->motion.hold_angle_deg) &&

@@ -8,4 +8,15 @@ function gentlePour3D(cellMm, seconds) {
   return load().gentlePour(Number(cellMm), Number(seconds));
 }
 
-module.exports = { gentlePour3D };
+// The gentle pour with a constant-rate ramp to maxAngleDeg (build-only motion kind 'ramp').
+function rampPour3D(cellMm, rateDegS, maxAngleDeg = 90) {
+  const cfg = gentlePour3D(cellMm, 20);
+  cfg.motion = Object.assign({}, cfg.motion, {
+    kind: 'ramp',
+    holdTiltRate: Number(rateDegS),
+    maxAngleDeg: Number(maxAngleDeg),
+  });
+  return cfg;
+}
+
+module.exports = { gentlePour3D, rampPour3D };

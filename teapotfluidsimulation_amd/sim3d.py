@@ -27,6 +27,13 @@ def gentlePour3D(cellMm: float, seconds: float) -> dict:
     return abi.config_to_dict(c)
 
 
+def rampPour3D(cellMm: float, rateDegS: float, maxAngleDeg: float = 90.0) -> dict:
+    """The gentle pour with a constant-rate ramp to maxAngleDeg (build-only motion kind)."""
+    c = abi.Config()
+    check(load().pl3d_ramp_pour(float(cellMm), float(rateDegS), float(maxAngleDeg), C.byref(c)))
+    return abi.config_to_dict(c)
+
+
 def variedPour3D(seed: int, index: int) -> dict:
     """Member `index` of the seeded family of varied pours (pl3d_varied_pour)."""
     c = abi.Config()

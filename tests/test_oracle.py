@@ -460,3 +460,29 @@ def test_poisson_generator_oracle():
     x = PS.solve(n, fl, b)
     A, L = PS.operator(n, fl)
     assert np.abs(A @ x[L] - b[L]).max() < 1e-10
+
+
+def test_ramp_motion_closed_form():
+    """The build-only ramp motion (PL3D_MOTION_RAMP): after startDelay the tilt grows at exactly
+    the ramp rate, angle = rate (t - startDelay), up to the maximum angle, where it holds; the
+    reference's pour, tilt and manual kinds are unchanged (the other tests)."""
+    c = O.gentle_pour(10, 20)
+    c.motion.kind = 4
+    c.motion.hold_tilt_rate = 45.0
+    c.motion.max_angle_deg = 60.0
+    c.motion.target_volume = 10.0  # never reached: no return in this window
+    c.motion.max_hold = 100.0
+    s = O.OracleSim(c)
+    seen_hold = False
+    while s.diagnostics()["time"] < 2.2:
+        s.step(s.stableDt())
+        d = s.diagnostics()
+        t = d["time"]
+        want = 0.0 if t < 0.5 else min(60.0, 45.0 * (t - 0.5))
+        assert d["angleDeg"] == pytest.approx(want, abs=1e-9), t
+        if want >= 60.0:
+            assert d["phase"] == "hold"
+            seen_hold = True
+        elif t > 0.5:
+            assert d["phase"] == "tilt-in"
+    assert seen_hold
