@@ -96,6 +96,17 @@ extern "C" {
 #define PL3D_TRANSFER_FLIP 0
 #define PL3D_TRANSFER_APIC 1
 
+/* spout kinds: the reference's round tube (vessel3d.ts:74-95), or the build-only rim lip */
+#define PL3D_SPOUT_TUBE 0
+/* Rim lip (no counterpart in the reference, whose pot has only the tube): the body wall is
+ * lowered to inlet_height over x > 0, |z| < bore_radius (a notch in the rim on the pouring
+ * side), and an open channel leaves the notch floor at the inner wall face, rising at
+ * spout_angle_deg for spout_length: a flat plate of thickness `wall` and half-width bore_radius
+ * with two side rails (thickness `wall`) reaching height - inlet_height above the plate's face.
+ * No tube and no bore.  The discharge plane, the domain sweep and the spill rule use the
+ * plate's tip as they use the tube's mouth. */
+#define PL3D_SPOUT_LIP 1
+
 /* Teapot3DConfig (sim3d/vessel3d.ts:9-20); lengths in metres */
 typedef struct pl3d_teapot_config {
   double radius;
@@ -105,6 +116,7 @@ typedef struct pl3d_teapot_config {
   double bore_radius;
   double spout_angle_deg;
   double spout_length;
+  int32_t spout_kind; /* PL3D_SPOUT_* (build-only field; 0 = the reference's tube) */
 } pl3d_teapot_config;
 
 /* MotionConfig (sim/types.ts:55-84) */
@@ -290,12 +302,20 @@ int pl3d_recorder_destroy(pl3d_recorder *rec);
 typedef struct pl3d_ensemble pl3d_ensemble;
 /* Member `index` of a family of varied pours drawn from one mulberry32 stream (seed): pot radius
  * 5-8 cm, fill 0.8-1.5 L in three equal-volume layers, spout bore diameter 1.5-2.5 cm with its
- * inlet 1-10 cm above the floor, hold tilt rate 5-20 deg/s, cells 3-6 mm (coarsened where the half
- * pot would hold more than ~11,000 liquid cells), the rest as gentlePour3D(., 20).  Build-only:
- * the reference has one preset (presets3d.ts:9-26). */
+ * inlet 1-10 cm above the floor or (one pot in four) a rim lip (notch 3 cm deep, half-width
+ * 2-3 cm), hold tilt rate 5-20 deg/s, cells 3-6 mm (coarsened where the half pot would hold more
+ * than ~11,000 liquid cells), the rest as gentlePour3D(., 20).  Build-only: the reference has one
+ * preset (presets3d.ts:9-26). */
 int pl3d_varied_pour(uint32_t seed, int32_t index, pl3d_config *out);
 /* the gentle pour with a ramp motion: tilt at rate_deg_s up to max_angle_deg (build-only) */
 int pl3d_ramp_pour(double cell_mm, double rate_deg_s, double max_angle_deg, pl3d_config *out);
+/* the gentle pour from a pot with a rim lip instead of the tube (build-only): notch floor
+ * 12 cm (3 cm below the rim), half-width 2.5 cm, lip channel rising at 15 deg for 4 cm */
+int pl3d_lip_pour(double cell_mm, pl3d_config *out);
+/* Config 0's pot (build-only preset): a 6.2 cm-radius, 14 cm-high cylinder holding 1.5 L, a
+ * 2 cm-diameter tube spout centred 2 cm above the floor, CFL 1, tilted by a constant-rate ramp
+ * at rate_deg_s up to 90 deg; the controller stops at 400 mL as in the gentle pour. */
+int pl3d_low_spout_pour(double cell_mm, double rate_deg_s, pl3d_config *out);
 int pl3d_ensemble_create(const pl3d_config *cfgs, int32_t count, pl3d_ensemble **out);
 int pl3d_ensemble_destroy(pl3d_ensemble *ens);
 int pl3d_ensemble_size(pl3d_ensemble *ens, int32_t *count);

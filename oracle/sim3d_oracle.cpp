@@ -76,7 +76,8 @@ Teapot build_teapot(const pl3d_teapot_config &cfg) {  // vessel3d.ts:46-59
   Teapot t;
   t.cfg = cfg;
   double a = (cfg.spout_angle_deg * PI) / 180;
-  t.ax = cfg.radius - cfg.wall;
+  // rim lip (build-only): the plate's face starts at the inner wall face
+  t.ax = cfg.spout_kind == PL3D_SPOUT_LIP ? cfg.radius : cfg.radius - cfg.wall;
   t.ay = cfg.inlet_height;
   t.dx = std::cos(a);
   t.dy = std::sin(a);
@@ -97,6 +98,30 @@ inline void spout_axis(const Teapot &t, double x, double y, double z, double &di
   dist = std::sqrt(qx * qx + qy * qy + z * z);
 }
 
+// Rim lip (build-only, no reference counterpart): the body's `solid` with the rim notched to
+// inlet_height over x > 0, |z| < bore_radius, united with a channel: a plate of thickness wall
+// whose upper face runs from (ax, ay) along (dx, dy) for spout_length, |z| <= bore_radius, and
+// two side rails (|z| in [bore_radius, bore_radius + wall]) from the plate's underside up to
+// height - inlet_height above the face.
+double lip_sdf(const Teapot &t, double x, double y, double z, double solid) {
+  const pl3d_teapot_config &c = t.cfg;
+  double hw = c.wall / 2;
+  double inNotch = std::max(std::max(std::fabs(z) - c.bore_radius, c.inlet_height - y), -x);
+  solid = std::max(solid, -inNotch);
+  double px = x - t.ax, py = y - t.ay;
+  double s = px * t.dx + py * t.dy;  // along the face
+  double above = py * t.dx - px * t.dy;  // above the face
+  double along = std::max(-s, s - c.spout_length);
+  double slab = std::fabs(above + hw) - hw;
+  double plate = std::max(std::max(slab, along), std::fabs(z) - c.bore_radius);
+  // side rails continue the notch's cut wall along the plate, up to the rim's height
+  double depth = c.height - c.inlet_height;
+  double railH = std::fabs(above - (depth - c.wall) / 2) - (depth + c.wall) / 2;
+  double railZ = std::fabs(std::fabs(z) - (c.bore_radius + hw)) - hw;
+  double rails = std::max(std::max(railH, along), railZ);
+  return std::min(solid, std::min(plate, rails));
+}
+
 // vessel3d.ts:74-95
 double teapot_sdf(const Teapot &t, double x, double y, double z) {
   const pl3d_teapot_config &c = t.cfg;
@@ -107,6 +132,7 @@ double teapot_sdf(const Teapot &t, double x, double y, double z) {
   double side = std::max(ring, yIn);
   double floor_ = std::max(std::fabs(y + hw) - hw, r - (c.radius + c.wall));
   double solid = std::min(side, floor_);
+  if (c.spout_kind == PL3D_SPOUT_LIP) return lip_sdf(t, x, y, z, solid);
   double dist, s;
   spout_axis(t, x, y, z, dist, s);
   double L = c.spout_length;
@@ -121,6 +147,7 @@ double teapot_sdf(const Teapot &t, double x, double y, double z) {
 bool in_cavity(const Teapot &t, double x, double y, double z) {
   const pl3d_teapot_config &c = t.cfg;
   if (y >= 0 && y <= c.height && x * x + z * z <= c.radius * c.radius) return true;
+  if (c.spout_kind == PL3D_SPOUT_LIP) return false;  // no bore
   double dist, s;
   spout_axis(t, x, y, z, dist, s);
   return s >= 0 && s <= c.spout_length && dist <= c.bore_radius;

@@ -18,6 +18,8 @@ from .sim3d import (  # noqa: F401
     Recorder3D,
     Simulation3D,
     gentlePour3D,
+    lipPour3D,
+    lowSpoutPour3D,
     rampPour3D,
     variedPour3D,
 )

@@ -44,6 +44,8 @@ SIGNATURES = {
     "pl3d_varied_pour": (C.c_int, [C.c_uint32, _i32, _P(abi.Config)]),
     "pl3d_p2g_bricks": (C.c_int, [_vp, _i32]),
     "pl3d_ramp_pour": (C.c_int, [_d, _d, _d, _vp]),
+    "pl3d_lip_pour": (C.c_int, [_d, _vp]),
+    "pl3d_low_spout_pour": (C.c_int, [_d, _d, _vp]),
     "pl3d_read_pressure_system": (C.c_int, [_vp, _vp, _vp, _vp]),
     "pl3d_net_create": (C.c_int, [_i32, _i32, _i32, C.c_uint32, _P(_vp)]),
     "pl3d_net_destroy": (C.c_int, [_vp]),

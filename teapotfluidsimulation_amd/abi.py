@@ -18,6 +18,8 @@ MOTION_KINDS = {"still": 0, "pour": 1, "tilt": 2, "manual": 3, "ramp": 4}
 PHASES = ["wait", "tilt-in", "hold", "return", "rest", "done", "manual"]
 PRECONDITIONERS = {"mic0": 0, "mg": 1, "jacobi": 2, "chebyshev": 3}
 TRANSFERS = {"flip": 0, "apic": 1}
+# the reference's round tube spout, or the build-only rim lip (PL3D_SPOUT_*)
+SPOUT_KINDS = {"tube": 0, "lip": 1}
 
 STAT = {
     "inPot": 0,
@@ -48,6 +50,7 @@ class TeapotConfig(C.Structure):
         ("bore_radius", C.c_double),
         ("spout_angle_deg", C.c_double),
         ("spout_length", C.c_double),
+        ("spout_kind", C.c_int32),
     ]
 
 
@@ -191,6 +194,7 @@ def config_to_dict(c: Config) -> dict:
             "boreRadius": t.bore_radius,
             "spoutAngleDeg": t.spout_angle_deg,
             "spoutLength": t.spout_length,
+            "spoutKind": {v: k for k, v in SPOUT_KINDS.items()}[t.spout_kind],
         },
         "h": c.h,
         "initialVolume": c.initial_volume,
@@ -240,6 +244,7 @@ def config_from_dict(d: dict) -> Config:
     c.teapot.bore_radius = t["boreRadius"]
     c.teapot.spout_angle_deg = t["spoutAngleDeg"]
     c.teapot.spout_length = t["spoutLength"]
+    c.teapot.spout_kind = SPOUT_KINDS[t.get("spoutKind", "tube")]
     c.h = d["h"]
     c.initial_volume = d["initialVolume"]
     c.motion.kind = MOTION_KINDS[m["kind"]]

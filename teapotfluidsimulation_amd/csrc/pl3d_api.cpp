@@ -52,7 +52,8 @@ Pot make_pot(const pl3d_teapot_config &c) {  // buildTeapot3D, vessel3d.ts:46-59
   Pot t;
   t.c = c;
   const double a = (c.spout_angle_deg * PI) / 180;
-  t.ax = c.radius - c.wall;
+  // the tube's axis starts inside the body wall; the lip's face starts at the inner wall face
+  t.ax = c.spout_kind == PL3D_SPOUT_LIP ? c.radius : c.radius - c.wall;
   t.ay = c.inlet_height;
   t.dx = std::cos(a);
   t.dy = std::sin(a);
@@ -70,6 +71,26 @@ void axis(const Pot &t, double x, double y, double z, double &dist, double &s) {
   dist = std::sqrt(qx * qx + qy * qy + z * z);
 }
 
+// Rim lip (build-only): the body `solid` with a notch carved from the rim on the pouring side,
+// united with the lip channel: a plate (upper face through (ax, ay) along (dx, dy), thickness
+// wall) and two side rails up to the rim's height above the face.
+double lip_sdf(const Pot &t, double x, double y, double z, double solid) {
+  const pl3d_teapot_config &c = t.c;
+  const double hw = c.wall / 2;
+  const double notch = std::max(std::max(std::fabs(z) - c.bore_radius, c.inlet_height - y), -x);
+  solid = std::max(solid, -notch);
+  const double px = x - t.ax, py = y - t.ay;
+  const double s = px * t.dx + py * t.dy;
+  const double h = py * t.dx - px * t.dy;  // height above the face
+  const double along = std::max(-s, s - c.spout_length);
+  const double plate = std::max(std::max(std::fabs(h + hw) - hw, along), std::fabs(z) - c.bore_radius);
+  const double depth = c.height - c.inlet_height;
+  const double rails =
+      std::max(std::max(std::fabs(h - (depth - c.wall) / 2) - (depth + c.wall) / 2, along),
+               std::fabs(std::fabs(z) - (c.bore_radius + hw)) - hw);
+  return std::min(solid, std::min(plate, rails));
+}
+
 double sdf(const Pot &t, double x, double y, double z) {  // teapotSdf, vessel3d.ts:74-95
   const pl3d_teapot_config &c = t.c;
   const double hw = c.wall / 2;
@@ -78,6 +99,7 @@ double sdf(const Pot &t, double x, double y, double z) {  // teapotSdf, vessel3d
   const double side = std::max(ring, std::max(-(y + c.wall), y - c.height));
   const double fl = std::max(std::fabs(y + hw) - hw, r - (c.radius + c.wall));
   double solid = std::min(side, fl);
+  if (c.spout_kind == PL3D_SPOUT_LIP) return lip_sdf(t, x, y, z, solid);
   double dist, s;
   axis(t, x, y, z, dist, s);
   const double L = c.spout_length;
@@ -90,6 +112,7 @@ double sdf(const Pot &t, double x, double y, double z) {  // teapotSdf, vessel3d
 bool cavity(const Pot &t, double x, double y, double z) {  // inCavity, vessel3d.ts:98-103
   const pl3d_teapot_config &c = t.c;
   if (y >= 0 && y <= c.height && x * x + z * z <= c.radius * c.radius) return true;
+  if (c.spout_kind == PL3D_SPOUT_LIP) return false;
   double dist, s;
   axis(t, x, y, z, dist, s);
   return s >= 0 && s <= c.spout_length && dist <= c.bore_radius;
@@ -414,6 +437,32 @@ int pl3d_ramp_pour(double cell_mm, double rate_deg_s, double max_angle_deg, pl3d
   return PL3D_OK;
 }
 
+int pl3d_lip_pour(double cell_mm, pl3d_config *out) {
+  if (!out || !(cell_mm > 0)) return fail(PL3D_EINVAL, "bad lip pour arguments");
+  fill_gentle(cell_mm, 20, out);
+  out->teapot.spout_kind = PL3D_SPOUT_LIP;
+  out->teapot.inlet_height = 0.12;
+  out->teapot.bore_radius = 0.025;
+  out->teapot.spout_angle_deg = 15;
+  out->teapot.spout_length = 0.04;
+  return PL3D_OK;
+}
+
+int pl3d_low_spout_pour(double cell_mm, double rate_deg_s, pl3d_config *out) {
+  if (!out || !(cell_mm > 0) || !(rate_deg_s > 0))
+    return fail(PL3D_EINVAL, "bad low-spout pour arguments");
+  fill_gentle(cell_mm, 20, out);
+  out->teapot.radius = 0.062;
+  out->teapot.height = 0.14;
+  out->teapot.inlet_height = 0.02;
+  out->teapot.bore_radius = 0.01;
+  out->cfl = 1.0;
+  out->motion.kind = PL3D_MOTION_RAMP;
+  out->motion.hold_tilt_rate = rate_deg_s;
+  out->motion.max_angle_deg = 90;
+  return PL3D_OK;
+}
+
 int pl3d_varied_pour(uint32_t seed, int32_t index, pl3d_config *out) {
   if (!out || index < 0) return fail(PL3D_EINVAL, "bad varied pour arguments");
   // one mulberry32 stream from `seed`; member `index` takes the 8 draws after those of the
@@ -432,6 +481,13 @@ int pl3d_varied_pour(uint32_t seed, int32_t index, pl3d_config *out) {
   out->teapot.inlet_height =
       std::min(0.01 + 0.09 * u[3], out->teapot.height - 0.03);  // spout 1-10 cm up
   out->initial_volume = volume;
+  if (u[6] < 0.25) {  // a quarter of the pots pour over a rim lip instead of through a tube
+    out->teapot.spout_kind = PL3D_SPOUT_LIP;
+    out->teapot.inlet_height = out->teapot.height - 0.03;  // notch 3 cm deep
+    out->teapot.bore_radius = 0.02 + 0.01 * u[7];          // half-width 2-3 cm
+    out->teapot.spout_angle_deg = 15;
+    out->teapot.spout_length = 0.04;
+  }
   out->motion.hold_tilt_rate = 5 + 15 * u[4];  // tilt rate 5-20 deg/s while holding
   // cells 3-6 mm, coarsened where the liquid (half pot) would exceed ~11,000 cells so that every
   // member stays within the one-workgroup solver's capacity with room for the poured stream
@@ -463,6 +519,14 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
     return fail(PL3D_EINVAL, "p2g_brick must be in 0..%d", PL3D_P2G_BRICKS - 1);
   if (c.transfer != PL3D_TRANSFER_FLIP && c.transfer != PL3D_TRANSFER_APIC)
     return fail(PL3D_EINVAL, "transfer %d is not available", c.transfer);
+  if (c.teapot.spout_kind != PL3D_SPOUT_TUBE && c.teapot.spout_kind != PL3D_SPOUT_LIP)
+    return fail(PL3D_EINVAL, "spout kind %d is not available", int(c.teapot.spout_kind));
+  if (c.teapot.spout_kind == PL3D_SPOUT_LIP &&
+      !(c.teapot.inlet_height > 0 && c.teapot.inlet_height <= c.teapot.height &&
+        c.teapot.bore_radius > 0 && c.teapot.bore_radius < c.teapot.radius &&
+        c.teapot.spout_length > 0))
+    return fail(PL3D_EINVAL, "a rim lip needs 0 < inlet_height <= height, 0 < bore_radius < "
+                             "radius and spout_length > 0");
   auto *s = new pl3d_sim();
   s->cfg = c;
   s->pot = make_pot(c.teapot);
@@ -565,6 +629,7 @@ int pl3d_create(const pl3d_config *cfg, pl3d_sim **out) {
   P.inletHeight = t.inlet_height;
   P.boreRadius = t.bore_radius;
   P.spoutLength = t.spout_length;
+  P.spoutKind = t.spout_kind;
   P.ax = s->pot.ax;
   P.ay = s->pot.ay;
   P.dx = s->pot.dx;

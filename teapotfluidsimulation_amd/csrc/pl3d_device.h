@@ -17,6 +17,7 @@
 #define PL3D_AIR_ 0
 #define PL3D_FLUID_ 1
 #define PL3D_SOLID_ 2
+#define PL3D_SPOUT_LIP_ 1  // PL3D_SPOUT_LIP of the public header
 #define PL3D_TRANSFER_APIC_ 1  // PL3D_TRANSFER_APIC of the C ABI
 #define ST_IN_POT 0
 #define ST_DISCHARGED 1
@@ -104,6 +105,7 @@ struct DevParams {
   double hd;
   // geometry (Teapot3D, vessel3d.ts:34-59)
   double radius, height, wall, inletHeight, boreRadius, spoutLength;
+  int spoutKind;  // PL3D_SPOUT_TUBE / PL3D_SPOUT_LIP
   double ax, ay, dx, dy, tipX, tipY;
   // frames (solver3d.ts:71-72)
   double pivotX, pivotY;        // cfg.pivot (pot-local)
@@ -136,6 +138,20 @@ __device__ __forceinline__ T teapot_sdf(const DevParams &P, T x, T y, T z) {
   const T px = x - T(P.ax), py = y - T(P.ay);
   const T s = px * T(P.dx) + py * T(P.dy);
   const T L = T(P.spoutLength);
+  if (P.spoutKind == PL3D_SPOUT_LIP_) {
+    // rim lip: a notch carved from the rim on the pouring side (x > 0, |z| < half-width, above
+    // the notch floor), united with the lip channel: the plate below the face through (ax, ay)
+    // and two side rails up to the rim's height above the face
+    const T notch = fmax(fmax(fabs(z) - T(P.boreRadius), T(P.inletHeight) - y), -x);
+    solid = fmax(solid, -notch);
+    const T hf = py * T(P.dx) - px * T(P.dy);
+    const T along = fmax(-s, s - L);
+    const T plate = fmax(fmax(fabs(hf + hw) - hw, along), fabs(z) - T(P.boreRadius));
+    const T depth = height - T(P.inletHeight);
+    const T rails = fmax(fmax(fabs(hf - (depth - wall) / T(2)) - (depth + wall) / T(2), along),
+                         fabs(fabs(z) - (T(P.boreRadius) + hw)) - hw);
+    return fmin(solid, fmin(plate, rails));
+  }
   const T sc = s < T(0) ? T(0) : (s > L ? L : s);
   const T qx = px - sc * T(P.dx), qy = py - sc * T(P.dy);
   const T dist = sqrt(qx * qx + qy * qy + z * z);

@@ -123,6 +123,7 @@ static int get_opt_str(napi_env env, napi_value obj, const char *key, char *buf,
 static const char *MOTION_KINDS[] = {"still", "pour", "tilt", "manual", "ramp"};
 static const char *PRECONDS[] = {"mic0", "mg", "jacobi", "chebyshev"};
 static const char *TRANSFERS[] = {"flip", "apic"};
+static const char *SPOUT_KINDS[] = {"tube", "lip"}; /* PL3D_SPOUT_* (build-only key) */
 
 static int name_index(const char *const *names, int n, const char *s) {
   for (int i = 0; i < n; i++)
@@ -134,7 +135,8 @@ static int name_index(const char *const *names, int n, const char *s) {
 
 /* Config3D (sim3d/solver3d.ts:25-43) as a JS object -> pl3d_config.  Device-only keys
  * 'preconditioner' and 'transfer' are optional strings, 'chebDegree', 'chebLower' and
- * 'p2gBrick' optional numbers (0 or absent: the library's defaults). */
+ * 'p2gBrick' optional numbers (0 or absent: the library's defaults); teapot.spoutKind is an
+ * optional 'tube' (default) or 'lip'. */
 static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
   napi_value t, m, pv, x;
   double d;
@@ -185,6 +187,16 @@ static int config_from_js(napi_env env, napi_value o, pl3d_config *c) {
       return 0;
     }
     c->motion.kind = k;
+  }
+  {
+    int r = get_opt_str(env, t, "spoutKind", buf, sizeof buf);
+    if (r < 0) return 0;
+    int k = r == 0 ? PL3D_SPOUT_TUBE : name_index(SPOUT_KINDS, 2, buf);
+    if (k < 0) {
+      napi_throw_type_error(env, NULL, "Config3D: teapot.spoutKind must be 'tube' or 'lip'");
+      return 0;
+    }
+    c->teapot.spout_kind = k;
   }
   {
     bool isArr = false;
@@ -256,7 +268,10 @@ static napi_value config_to_js(napi_env env, const pl3d_config *c) {
            set_num(env, t, "inletHeight", c->teapot.inlet_height) &&
            set_num(env, t, "boreRadius", c->teapot.bore_radius) &&
            set_num(env, t, "spoutAngleDeg", c->teapot.spout_angle_deg) &&
-           set_num(env, t, "spoutLength", c->teapot.spout_length) && set_val(env, o, "teapot", t) &&
+           set_num(env, t, "spoutLength", c->teapot.spout_length) &&
+           set_str(env, t, "spoutKind",
+                   c->teapot.spout_kind == PL3D_SPOUT_LIP ? SPOUT_KINDS[1] : SPOUT_KINDS[0]) &&
+           set_val(env, o, "teapot", t) &&
            set_num(env, o, "h", c->h) && set_num(env, o, "initialVolume", c->initial_volume) &&
            set_str(env, m, "kind",
                    c->motion.kind >= 0 && c->motion.kind < 5 ? MOTION_KINDS[c->motion.kind] : "pour") &&
@@ -416,6 +431,28 @@ static napi_value js_gentle_pour(napi_env env, napi_callback_info info) {
       !get_double(env, argv[1], &seconds, "seconds"))
     return NULL;
   CHECK_PL3D(pl3d_gentle_pour(cell, seconds, &c));
+  return config_to_js(env, &c);
+}
+
+/* lipPour(cellMm) -> Config3D: the gentle pour from the build-only rim-lip pot */
+static napi_value js_lip_pour(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  double cell;
+  pl3d_config c;
+  if (!args(env, info, 1, argv) || !get_double(env, argv[0], &cell, "cellMm")) return NULL;
+  CHECK_PL3D(pl3d_lip_pour(cell, &c));
+  return config_to_js(env, &c);
+}
+
+/* lowSpoutPour(cellMm, rateDegS) -> Config3D: the build-only low-spout ramp preset */
+static napi_value js_low_spout_pour(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  double cell, rate;
+  pl3d_config c;
+  if (!args(env, info, 2, argv) || !get_double(env, argv[0], &cell, "cellMm") ||
+      !get_double(env, argv[1], &rate, "rateDegS"))
+    return NULL;
+  CHECK_PL3D(pl3d_low_spout_pour(cell, rate, &c));
   return config_to_js(env, &c);
 }
 
@@ -817,6 +854,8 @@ static napi_value init(napi_env env, napi_value exports) {
   } fns[] = {
       {"version", js_version},
       {"gentlePour", js_gentle_pour},
+      {"lipPour", js_lip_pour},
+      {"lowSpoutPour", js_low_spout_pour},
       {"create", js_create},
       {"destroy", js_destroy},
       {"info", js_info},

@@ -19,4 +19,15 @@ function rampPour3D(cellMm, rateDegS, maxAngleDeg = 90) {
   return cfg;
 }
 
-module.exports = { gentlePour3D, rampPour3D };
+// The gentle pour from a pot with a rim lip in place of the tube spout (build-only geometry).
+function lipPour3D(cellMm) {
+  return load().lipPour(Number(cellMm));
+}
+
+// The 6.2 x 14 cm pot with a 2 cm tube spout 2 cm above the floor, ramped to 90 deg, CFL 1
+// (build-only preset).
+function lowSpoutPour3D(cellMm, rateDegS = 9) {
+  return load().lowSpoutPour(Number(cellMm), Number(rateDegS));
+}
+
+module.exports = { gentlePour3D, rampPour3D, lipPour3D, lowSpoutPour3D };

@@ -34,6 +34,21 @@ def rampPour3D(cellMm: float, rateDegS: float, maxAngleDeg: float = 90.0) -> dic
     return abi.config_to_dict(c)
 
 
+def lipPour3D(cellMm: float) -> dict:
+    """The gentle pour from a pot with a rim lip in place of the tube spout (build-only)."""
+    c = abi.Config()
+    check(load().pl3d_lip_pour(float(cellMm), C.byref(c)))
+    return abi.config_to_dict(c)
+
+
+def lowSpoutPour3D(cellMm: float, rateDegS: float = 9.0) -> dict:
+    """The 6.2 x 14 cm pot with a 2 cm tube spout 2 cm above the floor, ramped to 90 deg at
+    rateDegS, CFL 1 (build-only preset)."""
+    c = abi.Config()
+    check(load().pl3d_low_spout_pour(float(cellMm), float(rateDegS), C.byref(c)))
+    return abi.config_to_dict(c)
+
+
 def variedPour3D(seed: int, index: int) -> dict:
     """Member `index` of the seeded family of varied pours (pl3d_varied_pour)."""
     c = abi.Config()

@@ -1,0 +1,41 @@
+"""The library's build-only geometry and motion presets (pl3d_lip_pour, pl3d_low_spout_pour)
+equal the configurations the oracle's golden pours were computed from, and the host rejects
+malformed lip geometry before any device work."""
+import pytest
+
+from golden.make_geometry_golden import lip_cfg, low_spout_cfg
+from teapotfluidsimulation_amd import lipPour3D, lowSpoutPour3D
+from teapotfluidsimulation_amd.abi import config_to_dict
+
+DEVICE_KEYS = ("preconditioner", "transfer", "chebDegree", "chebLower", "p2gBrick", "warmStart")
+
+
+@pytest.mark.parametrize("cell", [10, 5, 4])
+def test_presets_equal_oracle_configs(cell):
+    for mine, theirs in ((lipPour3D(cell), lip_cfg(cell)),
+                         (lowSpoutPour3D(cell, 12.0), low_spout_cfg(cell, 12.0))):
+        o = config_to_dict(theirs)
+        for k in DEVICE_KEYS:
+            mine.pop(k)
+            o.pop(k)
+        assert mine == o
+    assert lipPour3D(cell)["teapot"]["spoutKind"] == "lip"
+    assert lowSpoutPour3D(cell)["motion"]["holdTiltRate"] == 9.0
+
+
+def test_bad_spout_geometry_is_rejected_before_device_work():
+    import ctypes as C
+
+    from teapotfluidsimulation_amd import _lib, abi
+    lib = _lib.load()
+    h = C.c_void_p()
+    c = abi.config_from_dict(lipPour3D(5))
+    c.teapot.spout_kind = 7
+    assert lib.pl3d_create(C.byref(c), C.byref(h)) == abi.EINVAL
+    assert b"spout kind" in lib.pl3d_last_error()
+    c = abi.config_from_dict(lipPour3D(5))
+    c.teapot.bore_radius = 0.07  # wider than the pot
+    assert lib.pl3d_create(C.byref(c), C.byref(h)) == abi.EINVAL
+    assert b"rim lip" in lib.pl3d_last_error()
+    assert lib.pl3d_lip_pour(0, C.byref(c)) == abi.EINVAL
+    assert lib.pl3d_low_spout_pour(5, -1, C.byref(c)) == abi.EINVAL

@@ -1,0 +1,99 @@
+"""The build-only geometry variants (rim lip, config 0's low-spout pot with a ramp) on the device
+against the FP64 oracle:
+- construction is exact (grid, parcels, bands, positions rounded to FP32), which pins the host's
+  lip SDF, cavity test and domain sweep;
+- the device's cell classification (teapot_sdf in FP64) equals the oracle's SDF sign at every
+  cell centre, at rest and tilted;
+- whole pours land on the oracle's (tests/golden/geometry_pours.json, written by
+  tests/golden/make_geometry_golden.py).  The flow is chaotic, so these bars are statistical and
+  stated at their assertions.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from golden.make_geometry_golden import lip_cfg, low_spout_cfg, run
+from oracle import oracle3d as O
+from teapotfluidsimulation_amd import Simulation3D
+from teapotfluidsimulation_amd.abi import config_to_dict
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "geometry_pours.json")
+CASES = {"lip": lip_cfg, "lowSpout": low_spout_cfg}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_construction_matches_oracle(case):
+    cfg = CASES[case](5)
+    cpu, gpu = O.OracleSim(cfg), Simulation3D(config_to_dict(cfg))
+    assert (gpu.n, gpu.nx, gpu.ny, gpu.nz) == (cpu.n, cpu.nx, cpu.ny, cpu.nz)
+    assert gpu.pivotWorld == cpu.pivotWorld and gpu.bandCounts == cpu.bandCounts
+    a, b = cpu.state(), gpu.state()
+    for k in ("x", "y", "z"):
+        assert np.array_equal(a[k].astype(np.float32).astype(np.float64), b[k]), k
+    assert np.array_equal(a["origin"], b["origin"]) and np.array_equal(a["y0"], b["y0"])
+    gpu.destroy()
+
+
+def solid_by_oracle(cfg, sim, angle_deg):
+    """SOLID flags of the classification rule (border cells, or teapotSdf < 0 at the cell
+    centre in the pot frame) evaluated with the oracle's FP64 SDF; also |sdf| per cell."""
+    nx, ny, nz, h = sim.nx, sim.ny, sim.nz, cfg.h
+    phi = -angle_deg * math.pi / 180
+    c, s = math.cos(phi), math.sin(phi)
+    solid = np.zeros((nz, ny, nx), bool)
+    mag = np.full((nz, ny, nx), np.inf)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                if i in (0, nx - 1) or j in (0, ny - 1) or k in (0, nz - 1):
+                    solid[k, j, i] = True
+                    continue
+                rx, ry = (i + 0.5) * h - sim.pivotWorld[0], (j + 0.5) * h - sim.pivotWorld[1]
+                lx, ly = c * rx + s * ry + cfg.pivot[0], -s * rx + c * ry + cfg.pivot[1]
+                d = O.teapot_sdf(cfg.teapot, lx, ly, (k + 0.5) * h - h)
+                solid[k, j, i] = d < 0
+                mag[k, j, i] = abs(d)
+    return solid, mag
+
+
+def test_lip_classification_matches_oracle_sdf():
+    cfg = lip_cfg(8)
+    cfg.motion.kind, cfg.motion.start_delay, cfg.motion.tilt_duration = 2, 0.0, 0.5
+    cfg.motion.hold_angle_deg = cfg.motion.max_angle_deg = 35.0
+    cfg.motion.hold_tilt_rate, cfg.motion.hold_time = 0.0, 5.0
+    gpu = Simulation3D(config_to_dict(cfg))
+    for until in (0.0, 0.7):  # at rest, then held at 35 deg
+        while gpu.diagnostics()["time"] < until:
+            gpu.step(gpu.stableDt())
+        angle = gpu.diagnostics()["angleDeg"]
+        gpu.step(gpu.stableDt())
+        cell, _, _ = gpu.readPressureSystem()
+        want, mag = solid_by_oracle(cfg, gpu, angle)
+        sure = mag > 1e-12
+        assert np.array_equal((cell == 2)[sure], want[sure]), angle
+    assert angle == pytest.approx(35.0)
+    gpu.destroy()
+
+
+def golden(case, cell):
+    rows = json.load(open(GOLDEN))["rows"]
+    return next(r for r in rows if r["case"] == case and r["cellMm"] == cell)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_pour_against_oracle_5mm(case):
+    ref = golden(case, 5.0)
+    gpu = Simulation3D(config_to_dict(CASES[case](5)))
+    d = run(gpu)
+    f = gpu.dischargedFractions()
+    print(f"{case}: device {d['dischargedMl']:.1f} mL poured, {d['spilledMl']:.1f} spilled, "
+          f"shares {np.round(f, 3)}; oracle {ref['dischargedMl']:.1f}, {ref['spilledMl']:.1f}, "
+          f"{np.round(ref['dischargedFractions'], 3)}")
+    assert d["lostMl"] == 0 and d["pressureFailures"] == 0
+    assert abs(d["dischargedMl"] / ref["dischargedMl"] - 1) <= 0.03
+    assert np.max(np.abs(np.array(f) - ref["dischargedFractions"])) <= 0.05
+    gpu.destroy()

@@ -9,7 +9,7 @@ import subprocess
 
 import pytest
 
-from teapotfluidsimulation_amd import gentlePour3D
+from teapotfluidsimulation_amd import gentlePour3D, lipPour3D, lowSpoutPour3D
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 NODE_DIR = os.path.join(ROOT, "teapotfluidsimulation_amd", "node")
@@ -19,7 +19,7 @@ NODE = shutil.which("node")
 pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
                                 reason="node or the built binding is absent")
 
-EXPORTS = ["version", "gentlePour", "create", "destroy", "info", "step", "advance", "diagnostics",
+EXPORTS = ["version", "gentlePour", "lipPour", "lowSpoutPour", "create", "destroy", "info", "step", "advance", "diagnostics",
            "stableDt", "finished", "dischargedFractions", "stats", "readState", "dischargeOrder",
            "readParticles", "readRenderPositions", "readAffine", "setTarget", "pcgSolve",
            "recorderCreate", "recorderParcels", "recorderCapture", "recorderDestroy"]
@@ -47,6 +47,16 @@ def test_preset_matches_python_host():
         assert js == py
 
 
+def test_geometry_presets_match_python_host():
+    drop = ("preconditioner", "transfer", "chebDegree", "chebLower", "p2gBrick", "warmStart")
+    for script, py in ((".lipPour3D(5)", lipPour3D(5)), (".lowSpoutPour3D(5, 12)", lowSpoutPour3D(5, 12))):
+        js = node_json(f"console.log(JSON.stringify(require('./src/sim3d/presets3d'){script}))")
+        for k in drop:
+            py.pop(k)
+        assert js == py
+    assert js["teapot"]["spoutKind"] == "tube" and js["motion"]["kind"] == "ramp"
+
+
 def test_argument_errors_are_thrown_before_device_work():
     out = node_json(r"""
 const a = require('./src/addon').load();
@@ -55,6 +65,7 @@ const res = {};
 function t(name, f) { try { f(); res[name] = 'no error'; } catch (e) { res[name] = e.constructor.name + ': ' + e.message; } }
 t('empty', () => a.create({}));
 t('kind', () => a.create(Object.assign({}, cfg, { motion: Object.assign({}, cfg.motion, { kind: 'spin' }) })));
+t('spout', () => a.create(Object.assign({}, cfg, { teapot: Object.assign({}, cfg.teapot, { spoutKind: 'funnel' }) })));
 t('pivot', () => a.create(Object.assign({}, cfg, { pivot: [1] })));
 t('precond', () => a.create(Object.assign({}, cfg, { preconditioner: 'ilu' })));
 t('handle', () => a.advance({}, 0.5));
@@ -64,6 +75,7 @@ console.log(JSON.stringify(res));
 """)
     assert out["empty"].startswith("TypeError") and "teapot" in out["empty"]
     assert out["kind"].startswith("TypeError") and "motion.kind" in out["kind"]
+    assert out["spout"].startswith("TypeError") and "spoutKind" in out["spout"]
     assert out["pivot"].startswith("TypeError") and "pivot" in out["pivot"]
     assert out["precond"].startswith("TypeError") and "preconditioner" in out["precond"]
     assert out["handle"].startswith("TypeError") and "handle" in out["handle"]

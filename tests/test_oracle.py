@@ -486,3 +486,98 @@ def test_ramp_motion_closed_form():
         elif t > 0.5:
             assert d["phase"] == "tilt-in"
     assert seen_hold
+
+
+def _lip_teapot():
+    """The rim-lip pot of pl3d_lip_pour: notch floor 12 cm, half-width 2.5 cm, lip channel
+    rising at 15 deg for 4 cm."""
+    t = O.default_teapot()
+    t.spout_kind = 1
+    t.inlet_height, t.bore_radius, t.spout_angle_deg, t.spout_length = 0.12, 0.025, 15.0, 0.04
+    return t
+
+
+def test_lip_geometry_closed_forms():
+    """The build-only rim lip: the notch opens the rim on the pouring side only, the plate's
+    and the rails' signed distances are exact at their mid-planes, and the tube's bore is gone."""
+    t = _lip_teapot()
+    R, w, hw, Hn, W = t.radius, t.wall, t.wall / 2, 0.12, 0.025
+    p = O.build_teapot(t)
+    a = math.radians(15.0)
+    assert (p["ax"], p["ay"]) == (R, Hn)
+    assert p["tipX"] == pytest.approx(R + 0.04 * math.cos(a), abs=1e-16)
+    assert p["tipY"] == pytest.approx(Hn + 0.04 * math.sin(a), abs=1e-16)
+    # the wall inside the notch is air, beside it and on the back it is solid up to the rim
+    assert O.teapot_sdf(t, R + hw, 0.125, 0) > 0
+    assert O.teapot_sdf(O.default_teapot(), R + hw, 0.125, 0) < 0
+    z = 0.03
+    assert O.teapot_sdf(t, math.sqrt((R + hw) ** 2 - z * z), 0.125, z) < 0
+    assert O.teapot_sdf(t, -(R + hw), 0.145, 0) < 0
+    assert O.teapot_sdf(t, R + hw, 0.115, 0) < 0  # below the notch floor
+    # plate mid-plane: exactly half a wall deep; 1 mm above the face: exactly 1 mm
+    u, n = (math.cos(a), math.sin(a)), (-math.sin(a), math.cos(a))
+
+    def at(s_, h_):
+        return R + s_ * u[0] + h_ * n[0], Hn + s_ * u[1] + h_ * n[1]
+
+    assert O.teapot_sdf(t, *at(0.02, -hw), 0.01) == pytest.approx(-hw, abs=1e-15)
+    assert O.teapot_sdf(t, *at(0.02, 1e-3), 0) == pytest.approx(1e-3, abs=1e-15)
+    # rail mid-plane (|z| = W + w/2) at half the rim's height above the face: half a wall deep
+    assert O.teapot_sdf(t, *at(0.03, 0.015), W + hw) == pytest.approx(-hw, abs=1e-15)
+    # above the rails (2 mm over the rim's height) the channel is open
+    assert O.teapot_sdf(t, *at(0.03, 0.15 - Hn + 2e-3), W + hw) == pytest.approx(2e-3, abs=1e-15)
+    # the tube's bore is not liquid space any more; the body still is
+    s = 0.08
+    tube = O.build_teapot(O.default_teapot())
+    bx, by = tube["ax"] + s * tube["dx"], tube["ay"] + s * tube["dy"]
+    assert O.in_cavity(O.default_teapot(), bx, by, 0) and not O.in_cavity(t, bx, by, 0)
+    assert O.in_cavity(t, 0.03, 0.1, 0.02) and not O.in_cavity(t, R + hw, 0.125, 0)
+
+
+def test_lip_sdf_sign_matches_explicit_solid():
+    """Sign of the lip pot's SDF at 40,000 random points against a membership test written
+    directly from the solid's description (wall annulus up to the rim minus the notch, floor
+    disk, plate slab, two rail slabs), with no min/max algebra."""
+    t = _lip_teapot()
+    R, w, H, Hn, W, L = t.radius, t.wall, t.height, 0.12, 0.025, 0.04
+    a = math.radians(15.0)
+    rng = np.random.default_rng(5)
+    pts = rng.uniform([-0.09, -0.02, -0.09], [0.13, 0.18, 0.09], (40000, 3))
+    mism = solid_n = 0
+    for x, y, z in pts:
+        r = math.hypot(x, z)
+        in_notch = x > 0 and abs(z) < W and y > Hn
+        wall = R <= r <= R + w and -w <= y <= H and not in_notch
+        floor = -w <= y <= 0 and r <= R + w
+        px, py = x - R, y - Hn
+        s, above = px * math.cos(a) + py * math.sin(a), py * math.cos(a) - px * math.sin(a)
+        plate = 0 <= s <= L and -w <= above <= 0 and abs(z) <= W
+        rail = 0 <= s <= L and -w <= above <= H - Hn and W <= abs(z) <= W + w
+        inside = wall or floor or plate or rail
+        solid_n += inside
+        d = O.teapot_sdf(t, x, y, z)
+        if abs(d) > 1e-12:
+            mism += (d < 0) != inside
+    assert mism == 0 and solid_n > 1000
+
+
+def test_lip_pour_behaviour():
+    """Tilted by 8 deg the lip pot holds everything; held at 35 deg, liquid leaves through the
+    notch and down the lip channel and is all counted as poured (none spilled)."""
+
+    def run(angle):
+        c = _cfg8(kind=2, start_delay=0, tilt_duration=1.0, hold_angle_deg=angle,
+                  max_angle_deg=angle, hold_time=1.5, hold_tilt_rate=0)
+        c.teapot = _lip_teapot()
+        c.duration = 2.5
+        s = O.OracleSim(c)
+        while s.diagnostics()["time"] < 2.5:
+            s.advance(0.5)
+        return s.diagnostics()
+
+    d = run(8.0)
+    assert d["dischargedMl"] + d["spilledMl"] + d["lostMl"] == 0
+    d = run(35.0)
+    assert d["dischargedMl"] > 30 and d["spilledMl"] == 0 and d["lostMl"] == 0
+    assert d["pressureFailures"] == 0
+    assert d["inPotMl"] + d["dischargedMl"] == pytest.approx(1500)
