@@ -267,10 +267,12 @@ def poisson_bench(a):
     kms = r[0]["kernelMs"]
     cells = n ** 3 * P
     # bytes of the lockstep iterations actually run (a problem stops streaming once converged);
-    # each Chebyshev step after the first streams z (neighbours), r, d and writes d, z (20 B per
-    # FLUID cell) plus the cells' info words, the first step 12 B, and z.r reads z and r (8 B)
+    # the three-term Chebyshev step streams z_k (neighbours), z_{k-1}, r and writes z_{k+1} (16 B
+    # per FLUID cell, 12 B for the second step, whose z_{k-1} is zero) plus the cells' info words;
+    # the first step reads r and writes z (8 B), and z.r reads z and r (8 B)
     deg = a.poisson_degree if pc == "chebyshev" else 0
-    per_cell = PS_VECTOR_BYTES + (12.0 + 20.0 * (deg - 1) + 8.0 if deg else 0.0)
+    steps_b = 8.0 + (12.0 + 16.0 * (deg - 2) if deg >= 2 else 0.0)
+    per_cell = PS_VECTOR_BYTES + (steps_b + 8.0 if deg else 0.0)
     per_flag = PS_FLAG_BYTES + (2.0 * (deg + 1) if deg else 0.0)
     it_bytes = sum((per_cell * mi) * x["iterations"] for mi, x in zip(m, r)) + \
         per_flag * cells * iters
@@ -308,7 +310,9 @@ def poisson_bench(a):
                    f"chebyshev degree {a.poisson_degree} on [{a.poisson_lower}, 2]"},
         "precisions": summary,
         "roofline": {"kernel": "batched PCG iteration (k_ps_spmv, k_ps_update, k_ps_dir + "
-                               "per-problem reductions)", "bound": "hbm",
+                               "per-problem reductions" +
+                               (", k_ps_cheb0, k_ps_cheb, k_ps_zdot, k_ps_dirz)" if deg else ")"),
+                     "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_model": f"{per_cell:.0f} B per FLUID cell per iteration + "

@@ -541,13 +541,16 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_sigma(const double *partial, 
   }
 }
 
-// ---- Chebyshev preconditioner (the polynomial of the single-pour solver, on all problems):
-// step 0: d = r / (diag theta), z = d; step j: d = c1 d + c2 (r - A z) / diag, z' = z + d with
-// z read from zin (neighbours) and written to zout (the two z buffers alternate).
+// ---- Chebyshev preconditioner (the polynomial of the single-pour solver, on all problems), in
+// its three-term form: step 0: z1 = r / (diag theta); step k: z_{k+1} = z_k + c1 (z_k - z_{k-1})
+// + c2 (r - A z_k) / diag, the same polynomial as the two-term d/z recurrence with d_k = z_k -
+// z_{k-1} (z_0 = 0).  z_k (and its neighbours) is read from zin; z_{k-1} is read from and
+// z_{k+1} written to zio in place (each cell touches only its own entry there), so two buffers
+// alternate and no correction vector is stored: 16 B per FLUID cell per step in FP32.
 template <class T>
 __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb0(const uint16_t *__restrict__ infov,
                                                          const T *__restrict__ rv,
-                                                         T *__restrict__ zv, T *__restrict__ dv,
+                                                         T *__restrict__ zv,
                                                          const Scal *__restrict__ sc, int n,
                                                          float theta) {
   const int p = blockIdx.y;
@@ -556,7 +559,7 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb0(const uint16_t *__restr
   const int64_t safe = 1 + n + int64_t(n) * n;
   const uint16_t *info = infov + base;
   const T *r = rv + base;
-  T *z = zv + base, *d = dv + base;
+  T *z = zv + base;
   PS_LOAD_INFO();
   if (!waveLiquid) return;
   T rs[PS_CPT];
@@ -564,36 +567,32 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb0(const uint16_t *__restr
   for (int q = 0; q < PS_CPT; q++) rs[q] = r[cc[q]];
 #pragma unroll
   for (int q = 0; q < PS_CPT; q++)
-    if (inf[q] & 0x8000u) {
-      const T v = rs[q] / (T(PS_DIAG(inf[q])) * T(theta));
-      z[cc[q]] = v;
-      d[cc[q]] = v;
-    }
+    if (inf[q] & 0x8000u) z[cc[q]] = rs[q] / (T(PS_DIAG(inf[q])) * T(theta));
 }
 
 template <class T>
 __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb(const uint16_t *__restrict__ infov,
                                                         const T *__restrict__ zin,
                                                         const T *__restrict__ rv,
-                                                        T *__restrict__ dv, T *__restrict__ zout,
+                                                        T *__restrict__ zio,
                                                         const Scal *__restrict__ sc, int n,
-                                                        float c1, float c2) {
+                                                        float c1, float c2, int first) {
   const int p = blockIdx.y;
   if (!sc[p].active) return;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
   const int64_t safe = 1 + n + int64_t(n) * n;
   const uint16_t *info = infov + base;
   const T *z = zin + base, *r = rv + base;
-  T *d = dv + base, *zo = zout + base;
+  T *zp = zio + base;
   const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
   PS_LOAD_INFO();
   if (!waveLiquid) return;
-  T z0[PS_CPT], nb[PS_CPT][6], rs[PS_CPT], ds[PS_CPT];
+  T z0[PS_CPT], nb[PS_CPT][6], rs[PS_CPT], ps[PS_CPT];
 #pragma unroll
   for (int q = 0; q < PS_CPT; q++) {
     z0[q] = z[cc[q]];
     rs[q] = r[cc[q]];
-    ds[q] = d[cc[q]];
+    ps[q] = first ? T(0) : zp[cc[q]];
 #pragma unroll
     for (int o = 0; o < 6; o++) nb[q][o] = z[cc[q] + off[o]];
   }
@@ -603,11 +602,8 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb(const uint16_t *__restri
 #pragma unroll
     for (int o = 0; o < 6; o++) sum += PS_NB(inf[q], o) ? nb[q][o] : T(0);
     const T dg = T(PS_DIAG(inf[q]));
-    const T dn = T(c1) * ds[q] + T(c2) * (rs[q] - (dg * z0[q] - sum)) / dg;
-    if (inf[q] & 0x8000u) {
-      d[cc[q]] = dn;
-      zo[cc[q]] = z0[q] + dn;
-    }
+    const T dn = T(c1) * (z0[q] - ps[q]) + T(c2) * (rs[q] - (dg * z0[q] - sum)) / dg;
+    if (inf[q] & 0x8000u) zp[cc[q]] = z0[q] + dn;
   }
 }
 
@@ -780,14 +776,14 @@ struct pl3d_poisson {
   // Chebyshev preconditioner (PL3D_PRECOND_CHEBYSHEV) or Jacobi (PL3D_PRECOND_JACOBI)
   int precond = PL3D_PRECOND_JACOBI, chebDegree = 6;
   double chebLower = 0.05;
-  void *za = nullptr, *zb = nullptr, *d = nullptr;  // its work vectors (element size workElem)
+  void *za = nullptr, *zb = nullptr;  // its two alternating z vectors (element size workElem)
   bool destroyed = false;
 };
 
 namespace {
 
 void free_work(pl3d_poisson *ps) {
-  for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As, &ps->za, &ps->zb, &ps->d}) {
+  for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As, &ps->za, &ps->zb}) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
@@ -804,7 +800,7 @@ int alloc_work(pl3d_poisson *ps, size_t elem, bool hi) {
     ps->workElem = elem;
   }
   if (ps->precond == PL3D_PRECOND_CHEBYSHEV && !ps->za)
-    for (void **p : {&ps->za, &ps->zb, &ps->d}) PS_HIP(hipMalloc(p, cells * elem));
+    for (void **p : {&ps->za, &ps->zb}) PS_HIP(hipMalloc(p, cells * elem));
   if (hi && !ps->xhi) PS_HIP(hipMalloc((void **)&ps->xhi, cells * 8));
   return PL3D_OK;
 }
@@ -823,19 +819,19 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
     *As = static_cast<T *>(ps->As);
   const dim3 grid(bpp, P), blk(PS_THREADS);
   const bool cheb = ps->precond == PL3D_PRECOND_CHEBYSHEV;
-  T *za = static_cast<T *>(ps->za), *zb = static_cast<T *>(ps->zb), *dd = static_cast<T *>(ps->d);
+  T *za = static_cast<T *>(ps->za), *zb = static_cast<T *>(ps->zb);
   // z = M r for the active problems; returns the buffer holding z
   auto chebyshev = [&]() -> T * {
     const double theta = 0.5 * (2.0 + ps->chebLower), delta = 0.5 * (2.0 - ps->chebLower);
     const double sigma1 = theta / delta;
     double rho = 1.0 / sigma1;
-    hipLaunchKernelGGL(k_ps_cheb0<T>, grid, blk, 0, st, ps->info, r, za, dd, ps->sc, n,
+    hipLaunchKernelGGL(k_ps_cheb0<T>, grid, blk, 0, st, ps->info, r, za, ps->sc, n,
                        float(theta));
     T *cur = za, *nxt = zb;
     for (int k = 1; k < ps->chebDegree; k++) {
       const double rhoN = 1.0 / (2.0 * sigma1 - rho);
-      hipLaunchKernelGGL(k_ps_cheb<T>, grid, blk, 0, st, ps->info, cur, r, dd, nxt, ps->sc, n,
-                         float(rhoN * rho), float(2.0 * rhoN / delta));
+      hipLaunchKernelGGL(k_ps_cheb<T>, grid, blk, 0, st, ps->info, cur, r, nxt, ps->sc, n,
+                         float(rhoN * rho), float(2.0 * rhoN / delta), k == 1 ? 1 : 0);
       std::swap(cur, nxt);
       rho = rhoN;
     }
