@@ -269,11 +269,12 @@ def poisson_bench(a):
     # bytes of the lockstep iterations actually run (a problem stops streaming once converged);
     # the three-term Chebyshev step streams z_k (neighbours), z_{k-1}, r and writes z_{k+1} (16 B
     # per FLUID cell, 12 B for the second step, whose z_{k-1} is zero) plus the cells' info words;
-    # the first step reads r and writes z (8 B), and z.r reads z and r (8 B)
+    # the first step is the update kernel's extra z store (4 B), and the last step forms z.r
+    # from values it holds (degree 1 keeps a separate z.r pass, 8 B)
     deg = a.poisson_degree if pc == "chebyshev" else 0
-    steps_b = 8.0 + (12.0 + 16.0 * (deg - 2) if deg >= 2 else 0.0)
-    per_cell = PS_VECTOR_BYTES + (steps_b + 8.0 if deg else 0.0)
-    per_flag = PS_FLAG_BYTES + (2.0 * (deg + 1) if deg else 0.0)
+    steps_b = 4.0 + (12.0 + 16.0 * (deg - 2) if deg >= 2 else 8.0)
+    per_cell = PS_VECTOR_BYTES + (steps_b if deg else 0.0)
+    per_flag = PS_FLAG_BYTES + (2.0 * max(deg - 1, 1) if deg else 0.0)
     it_bytes = sum((per_cell * mi) * x["iterations"] for mi, x in zip(m, r)) + \
         per_flag * cells * iters
     achieved = it_bytes / (kms / 1000) / 1e9

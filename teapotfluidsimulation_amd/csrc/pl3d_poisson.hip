@@ -318,7 +318,8 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_alpha(const double *partial, 
   }
 }
 
-// x += alpha s, r -= alpha As; partial z.r (z = r / diag) and max|r|
+// x += alpha s, r -= alpha As; partial z.r (z = r / diag) and max|r|; with zv, also the first
+// Chebyshev step z1 = r / (diag theta) of the new r (k_ps_cheb0 fused)
 template <class T>
 __global__ __launch_bounds__(PS_THREADS) void k_ps_update(const uint16_t *__restrict__ infov,
                                                           T *__restrict__ xv, T *__restrict__ rv,
@@ -327,7 +328,8 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_update(const uint16_t *__rest
                                                           double *__restrict__ partial,
                                                           double *__restrict__ partial2,
                                                           const Scal *__restrict__ sc, int n,
-                                                          int bpp) {
+                                                          int bpp, T *__restrict__ zv,
+                                                          float theta) {
   __shared__ double sh[PS_THREADS / 64];
   const int p = blockIdx.y;
   if (!sc[p].active) return;
@@ -355,6 +357,7 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_update(const uint16_t *__rest
         const T rr = rs[q] - alpha * as[q];
         r[cc[q]] = rr;
         const T z = rr / T(PS_DIAG(inf[q]));
+        if (zv) zv[base + cc[q]] = rr / (T(PS_DIAG(inf[q])) * T(theta));
         acc += double(z) * double(rr);
         mx = fmax(mx, fabs(double(rr)));
       }
@@ -570,13 +573,16 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb0(const uint16_t *__restr
     if (inf[q] & 0x8000u) z[cc[q]] = rs[q] / (T(PS_DIAG(inf[q])) * T(theta));
 }
 
+// With partial (the last step), also the per-block partial z.r of the new z (k_ps_zdot fused).
 template <class T>
 __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb(const uint16_t *__restrict__ infov,
                                                         const T *__restrict__ zin,
                                                         const T *__restrict__ rv,
                                                         T *__restrict__ zio,
                                                         const Scal *__restrict__ sc, int n,
-                                                        float c1, float c2, int first) {
+                                                        float c1, float c2, int first,
+                                                        double *__restrict__ partial, int bpp) {
+  __shared__ double sh[PS_THREADS / 64];
   const int p = blockIdx.y;
   if (!sc[p].active) return;
   const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
@@ -586,24 +592,34 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb(const uint16_t *__restri
   T *zp = zio + base;
   const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
   PS_LOAD_INFO();
-  if (!waveLiquid) return;
-  T z0[PS_CPT], nb[PS_CPT][6], rs[PS_CPT], ps[PS_CPT];
+  double acc = 0.0;
+  if (waveLiquid) {
+    T z0[PS_CPT], nb[PS_CPT][6], rs[PS_CPT], ps[PS_CPT];
 #pragma unroll
-  for (int q = 0; q < PS_CPT; q++) {
-    z0[q] = z[cc[q]];
-    rs[q] = r[cc[q]];
-    ps[q] = first ? T(0) : zp[cc[q]];
+    for (int q = 0; q < PS_CPT; q++) {
+      z0[q] = z[cc[q]];
+      rs[q] = r[cc[q]];
+      ps[q] = first ? T(0) : zp[cc[q]];
 #pragma unroll
-    for (int o = 0; o < 6; o++) nb[q][o] = z[cc[q] + off[o]];
+      for (int o = 0; o < 6; o++) nb[q][o] = z[cc[q] + off[o]];
+    }
+#pragma unroll
+    for (int q = 0; q < PS_CPT; q++) {
+      T sum = T(0);
+#pragma unroll
+      for (int o = 0; o < 6; o++) sum += PS_NB(inf[q], o) ? nb[q][o] : T(0);
+      const T dg = T(PS_DIAG(inf[q]));
+      const T dn = T(c1) * (z0[q] - ps[q]) + T(c2) * (rs[q] - (dg * z0[q] - sum)) / dg;
+      if (inf[q] & 0x8000u) {
+        const T zn = z0[q] + dn;
+        zp[cc[q]] = zn;
+        acc += double(zn) * double(rs[q]);
+      }
+    }
   }
-#pragma unroll
-  for (int q = 0; q < PS_CPT; q++) {
-    T sum = T(0);
-#pragma unroll
-    for (int o = 0; o < 6; o++) sum += PS_NB(inf[q], o) ? nb[q][o] : T(0);
-    const T dg = T(PS_DIAG(inf[q]));
-    const T dn = T(c1) * (z0[q] - ps[q]) + T(c2) * (rs[q] - (dg * z0[q] - sum)) / dg;
-    if (inf[q] & 0x8000u) zp[cc[q]] = z0[q] + dn;
+  if (partial) {  // uniform over the grid: every thread of the block reaches the reduction
+    const double t = block_sum_d(acc, sh);
+    if (threadIdx.x == 0) partial[int64_t(p) * bpp + blockIdx.x] = t;
   }
 }
 
@@ -821,20 +837,29 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
   const bool cheb = ps->precond == PL3D_PRECOND_CHEBYSHEV;
   T *za = static_cast<T *>(ps->za), *zb = static_cast<T *>(ps->zb);
   // z = M r for the active problems; returns the buffer holding z
-  auto chebyshev = [&]() -> T * {
-    const double theta = 0.5 * (2.0 + ps->chebLower), delta = 0.5 * (2.0 - ps->chebLower);
+  const double chebTheta = 0.5 * (2.0 + ps->chebLower);
+  // z = M r for the active problems and the partial z.r per block (in ps->partial); returns the
+  // buffer holding z.  With haveZ1 the first step (z1 in za) was done by the update kernel.
+  auto chebyshev = [&](bool haveZ1) -> T * {
+    const double theta = chebTheta, delta = 0.5 * (2.0 - ps->chebLower);
     const double sigma1 = theta / delta;
     double rho = 1.0 / sigma1;
-    hipLaunchKernelGGL(k_ps_cheb0<T>, grid, blk, 0, st, ps->info, r, za, ps->sc, n,
-                       float(theta));
+    if (!haveZ1)
+      hipLaunchKernelGGL(k_ps_cheb0<T>, grid, blk, 0, st, ps->info, r, za, ps->sc, n,
+                         float(theta));
     T *cur = za, *nxt = zb;
     for (int k = 1; k < ps->chebDegree; k++) {
       const double rhoN = 1.0 / (2.0 * sigma1 - rho);
+      const bool last = k == ps->chebDegree - 1;
       hipLaunchKernelGGL(k_ps_cheb<T>, grid, blk, 0, st, ps->info, cur, r, nxt, ps->sc, n,
-                         float(rhoN * rho), float(2.0 * rhoN / delta), k == 1 ? 1 : 0);
+                         float(rhoN * rho), float(2.0 * rhoN / delta), k == 1 ? 1 : 0,
+                         last ? ps->partial : nullptr, bpp);
       std::swap(cur, nxt);
       rho = rhoN;
     }
+    if (ps->chebDegree == 1)
+      hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, cur, r, ps->partial, ps->sc,
+                         n, bpp, 0);
     return cur;
   };
   hipEvent_t e0, e1;
@@ -845,10 +870,8 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
                      refined ? ps->xhi : nullptr, ps->partial, n, bpp);
   hipLaunchKernelGGL(k_ps_start, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, tol);
   if (cheb) {
-    T *z = chebyshev();
+    T *z = chebyshev(false);  // also z.r partials
     hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, z, s, ps->sc, n, 2);  // s = z
-    hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, z, r, ps->partial, ps->sc, n,
-                       bpp, 0);
   } else {
     hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 2);  // s = z
     hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n, bpp, 0);
@@ -862,13 +885,9 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
                        bpp);
     hipLaunchKernelGGL(k_ps_alpha, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp);
     hipLaunchKernelGGL(k_ps_update<T>, grid, blk, 0, st, ps->info, x, r, s, As, ps->partial,
-                       ps->partial2, ps->sc, n, bpp);
+                       ps->partial2, ps->sc, n, bpp, cheb ? za : nullptr, float(chebTheta));
     T *z = nullptr;
-    if (cheb) {  // z = M r replaces the update's Jacobi z.r partials
-      z = chebyshev();
-      hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, z, r, ps->partial, ps->sc, n,
-                         bpp, 0);
-    }
+    if (cheb) z = chebyshev(true);  // z = M r; its z.r partials replace the update's Jacobi ones
     hipLaunchKernelGGL(k_ps_beta, dim3(P), blk, 0, st, ps->partial, ps->partial2, ps->sc, bpp,
                        refined ? 1 : 0);
     if (cheb)
@@ -900,11 +919,9 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
         hipLaunchKernelGGL(k_ps_refresh_verdict, dim3(P), blk, 0, st, ps->partial, ps->partial2,
                            ps->sc, bpp);
         // restart where flagged: s = z, sigma = z.r
-        if (cheb) {
-          T *zr = chebyshev();
+        if (cheb) {  // z.r partials of every active problem; k_ps_sigma reads the flagged ones
+          T *zr = chebyshev(false);
           hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, zr, s, ps->sc, n, 1);
-          hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, zr, r, ps->partial,
-                             ps->sc, n, bpp, 1);
         } else {
           hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 1);
           hipLaunchKernelGGL(k_ps_zr<T>, grid, blk, 0, st, ps->info, r, ps->partial, ps->sc, n,
