@@ -44,26 +44,47 @@ def bytes_per_cell_iter(settings):
     return BYTES_PER_CELL_ITER[settings["preconditioner"]]
 
 
+def progress(msg):
+    """One line on stderr per finished phase (the JSON line alone goes to stdout)."""
+    print(msg, file=sys.stderr, flush=True)
+
+
 def learned_comparison(a):
-    """The metric's second part: pressure-solve time at the 1e-6 residual on systems logged from
-    this pour, the learned preconditioner (flexible CG, trained now on the device from the other
-    logged systems) against plain MIC(0)-PCG and Chebyshev-PCG; solves through the standalone
-    solve entry points with host arrays, device time by HIP events."""
+    """The metric's second part: pressure-solve time at the 1e-6 residual on held-out systems,
+    the learned preconditioner (flexible CG, trained now on the device) against plain MIC(0)-PCG
+    and Chebyshev-PCG.  Data (--learned-data): 'gentle' logs this pour every 0.05 simulated s and
+    holds out every fourth system; 'ramp' logs six seeded ramp pours on this pour's grid (tilt
+    rates 5-20 deg/s to 90 deg, fills 1.0-1.5 L), trains on four and evaluates on every third
+    system of the other two.  Solves go through the standalone solve entry points with host
+    arrays; device time by HIP events."""
     from teapotfluidsimulation_amd import learned
     t0 = time.perf_counter()
-    dims, snaps = learned.log_pour(a.cell, 0.05, a.transfer)
-    tr, te = learned.split(snaps)
-    net, info = learned.train(dims, tr, steps=4000)
+    if a.learned_data == "ramp":
+        dims, pours = learned.log_ramp_pours(1, 6, a.cell, 0.05, a.transfer, progress=True)
+        tr, te = learned.ramp_split(pours, held_out=2, eval_every=3)
+        logged = sum(len(p) for p in pours)
+        data = {"data": "6 seeded ramp pours (5-20 deg/s to 90 deg, 1.0-1.5 L), 4 train, 2 held out",
+                "rates_deg_s": [c["motion"]["holdTiltRate"]
+                                for c in learned.ramp_configs(1, 6, a.cell)]}
+    else:
+        dims, snaps = learned.log_pour(a.cell, 0.05, a.transfer)
+        tr, te = learned.split(snaps)
+        logged = len(snaps)
+        data = {"data": "this pour's systems every 0.05 s, every fourth held out"}
+    progress(f"learned: {logged} systems logged, training on {len(tr)}")
+    net, info = learned.train(dims, tr, steps=4000, progress=True)
+    progress(f"learned: evaluating {len(te)} held-out systems")
     _, summary = learned.evaluate(net, dims, te, baselines=("mic0", "chebyshev"))
     net.destroy()
     ms = {k: v["mean_device_ms"] for k, v in summary.items()}
-    return {"systems_logged": len(snaps), "held_out": len(te), "train": info,
-            "solve_ms": ms,
-            "iterations": {k: v["mean_iterations"] for k, v in summary.items()},
-            "all_converged": {k: v["all_converged"] for k, v in summary.items()},
-            "speedup_vs_mic0": ms["mic0"] / ms["learned"],
-            "speedup_vs_chebyshev": ms["chebyshev"] / ms["learned"],
-            "wall_s": time.perf_counter() - t0}
+    return dict(data, **{
+        "systems_logged": logged, "held_out": len(te), "train": info,
+        "solve_ms": ms,
+        "iterations": {k: v["mean_iterations"] for k, v in summary.items()},
+        "all_converged": {k: v["all_converged"] for k, v in summary.items()},
+        "speedup_vs_mic0": ms["mic0"] / ms["learned"],
+        "speedup_vs_chebyshev": ms["chebyshev"] / ms["learned"],
+        "wall_s": time.perf_counter() - t0})
 
 
 def solver_settings(a):
@@ -338,6 +359,8 @@ def main():
                     help="default: apic for the 4 mm pour, flip for --ensemble (APIC is not "
                          "stable in the fast-tilting varied pours)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--learned-data", default="gentle", choices=["gentle", "ramp"],
+                    help="systems the learned comparison trains and evaluates on")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-preconditioner comparison (log, train, held-out solves)")
     ap.add_argument("--cpu-sample", type=float, default=20.0)
@@ -380,6 +403,7 @@ def main():
     for _ in range(a.warmup):
         warm.advance(0.5)
     warm.destroy()
+    progress("warmup pour done")
 
     sim = make()
     n_parcels, grid = sim.n, (sim.nx, sim.ny, sim.nz)
@@ -390,6 +414,7 @@ def main():
     wall = time.perf_counter() - t0
     sim_s = d["time"]
     substeps = d["steps"]
+    progress(f"timed pour: {sim_s:.2f} sim-s in {wall:.2f} s")
     cnt = sim.counters()
     result = {
         "dischargedMl": d["dischargedMl"],
@@ -412,6 +437,7 @@ def main():
         kt = rep.kernel_times()
         rc = rep.counters()
         rep.destroy()
+        progress("kernel-timed replay done")
         kernels = {k: {"total_ms": v[0], "launches": v[1],
                        "avg_us": 1000 * v[0] / max(v[1], 1)} for k, v in kt.items()}
         dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
@@ -443,9 +469,11 @@ def main():
     learned_obj = None
     if not a.no_learned:
         learned_obj = learned_comparison(a)
+        progress(f"learned comparison done ({learned_obj['wall_s']:.0f} s)")
 
     cpu = None
     if not a.no_cpu_baseline:
+        progress("CPU baseline: the oracle on one core")
         cpu = cpu_baseline(int(a.cell) if a.cell == int(a.cell) else a.cell, a.cpu_sample,
                            a.transfer)
 

@@ -1,11 +1,12 @@
 """Training and evaluation of the learned pressure preconditioner on logged pour snapshots.
 
-A pour is run with the plain device solver and, after every `every_s` simulated seconds, the
+Pours are run with the plain device solver and, after every `every_s` simulated seconds, the
 last substep's pressure system (cell flags, rhs b, solution q) is logged; those triples, plus
 synthetic (A e, e) pairs on the same masks, train the network (LearnedPreconditioner,
-pl3d_net_*), and held-out snapshots compare the flexible CG with the network against the plain
-PCG solves (pl3d_pcg_solve) at the same stop rule.  No counterpart
-in the reference.
+pl3d_net_*), and held-out systems compare the flexible CG with the network against the plain
+PCG solves (pl3d_pcg_solve) at the same stop rule.  Two data sets: the snapshots of one gentle
+pour split by index (log_pour, split), or seeded ramp pours (tilt rates 5-20 deg/s, fills
+1.0-1.5 L) with whole pours held out (log_ramp_pours).  No counterpart in the reference.
 
 Usage: python -m teapotfluidsimulation_amd.learned [cellMm] [--steps N] [--out FILE]
 """
@@ -16,15 +17,51 @@ import time
 
 import numpy as np
 
-from .sim3d import LearnedPreconditioner, Pcg3D, Simulation3D, gentlePour3D
+from .sim3d import LearnedPreconditioner, Pcg3D, Simulation3D, gentlePour3D, rampPour3D
 
 
 def log_pour(cell_mm=4.0, every_s=0.05, transfer="apic", preconditioner="chebyshev", seed=1,
              progress=False):
-    """[(cell, b, q)] of the pour's pressure solves, one per every_s simulated seconds, while
-    the pot holds liquid (the run stops 1.5 s after the controller is done, as run3d.ts)."""
+    """[(cell, b, q)] of the gentle pour's pressure solves, one per every_s simulated seconds,
+    while the pot holds liquid (the run stops 1.5 s after the controller is done, as run3d.ts)."""
     cfg = gentlePour3D(cell_mm, 20)
     cfg.update(transfer=transfer, preconditioner=preconditioner, seed=seed)
+    return log_config(cfg, every_s, progress)
+
+
+def ramp_configs(seed=1, count=6, cell_mm=4.0, transfer="apic", preconditioner="chebyshev"):
+    """`count` ramp pours drawn from one seed: tilt rate 5-20 deg/s up to 90 deg (the ramp motion),
+    fill 1.0-1.5 L in three layers, each with its own parcel seed.  The pot and cell size are the
+    gentle pour's, so every pour has the same grid (the network is bound to one grid)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        c = rampPour3D(cell_mm, float(rng.uniform(5.0, 20.0)))
+        c["initialVolume"] = float(rng.uniform(1.0, 1.5)) * 1e-3
+        c.update(transfer=transfer, preconditioner=preconditioner, seed=int(1000 * seed + k + 1))
+        out.append(c)
+    return out
+
+
+def log_ramp_pours(seed=1, count=6, cell_mm=4.0, every_s=0.05, transfer="apic", progress=False):
+    """(dims, [snapshots of pour k]) for the pours of ramp_configs."""
+    dims, pours = None, []
+    for cfg in ramp_configs(seed, count, cell_mm, transfer):
+        d, snaps = log_config(cfg, every_s)
+        if dims is not None and d != dims:
+            raise ValueError(f"ramp pours on different grids: {dims} vs {d}")
+        dims = d
+        pours.append(snaps)
+        if progress:
+            sys.stderr.write(f"ramp pour {len(pours)}/{count} ({cfg['motion']['holdTiltRate']:.1f} "
+                             f"deg/s, {1e3 * cfg['initialVolume']:.2f} L): {len(snaps)} systems\n")
+            sys.stderr.flush()
+    return dims, pours
+
+
+def log_config(cfg, every_s=0.05, progress=False):
+    """(dims, [(cell, b, q)]) of the pressure solves of the pour `cfg`, one per every_s simulated
+    seconds, while the pot holds liquid, until 1.5 s after the controller is done."""
     sim = Simulation3D(cfg)
     out = []
     done_at = -1.0
@@ -153,15 +190,30 @@ def evaluate(net, dims, snaps, tol=1e-6, max_iter=400, baselines=("mic0", "cheby
     return rows, summary
 
 
+def ramp_split(pours, held_out=2, eval_every=3):
+    """Training systems: every system of all but the last `held_out` pours; evaluation systems:
+    every eval_every-th system of the held-out pours."""
+    tr = [s for snaps in pours[:len(pours) - held_out] for s in snaps]
+    te = [s for snaps in pours[len(pours) - held_out:] for s in snaps[::eval_every]]
+    return tr, te
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cell", nargs="?", type=float, default=4.0)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--every", type=float, default=0.05)
+    ap.add_argument("--ramp", type=int, default=0,
+                    help="log this many seeded ramp pours and hold out the last two")
     ap.add_argument("--out", default=None)
     a = ap.parse_args()
-    dims, snaps = log_pour(a.cell, a.every, progress=True)
-    tr, te = split(snaps)
+    if a.ramp:
+        dims, pours = log_ramp_pours(1, a.ramp, a.cell, a.every, progress=True)
+        snaps = [s for p in pours for s in p]
+        tr, te = ramp_split(pours)
+    else:
+        dims, snaps = log_pour(a.cell, a.every, progress=True)
+        tr, te = split(snaps)
     net, info = train(dims, tr, a.steps, progress=True)
     rows, summary = evaluate(net, dims, te)
     res = {"grid": dims, "logged": len(snaps), "train": info, "heldout": len(te), "summary": summary}
