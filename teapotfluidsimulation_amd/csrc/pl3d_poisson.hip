@@ -217,6 +217,16 @@ __device__ __forceinline__ double block_max_d(double v, double *sh) {
   return t;
 }
 
+// The chunk of PS_CPB cells a block of a per-problem kernel takes.  Workgroups go to the eight
+// XCDs round-robin in dispatch order, so block i of a row of G blocks runs on XCD i % 8 (G a
+// multiple of 8); mapping it to chunk (i % 8) (G / 8) + i / 8 gives each XCD one contiguous
+// eighth of the problem, walked in order, so the +-z neighbour planes a stencil gathers were
+// read through the same XCD's L2 shortly before.  Other G: the identity.  A bijection on [0, G).
+__device__ __forceinline__ int64_t ps_block() {
+  const unsigned G = gridDim.x, i = blockIdx.x;
+  return (G % 8u) ? int64_t(i) : int64_t(i % 8u) * (G / 8u) + i / 8u;
+}
+
 // Each thread takes PS_CPT cells, PS_THREADS apart.  The cells' info words are loaded first, all
 // together; the body then runs for the FLUID cells with `c` (cell index) and `inf` (its word:
 // bit 15 FLUID, bits 0-5 the -x,+x,-y,+y,-z,+z neighbours that are FLUID, bits 8-10 the
@@ -226,11 +236,11 @@ __device__ __forceinline__ double block_max_d(double v, double *sh) {
   {                                                                                       \
     uint32_t inf_[PS_CPT];                                                                \
     _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
-      const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+      const int64_t c = ps_block() * PS_CPB + q * PS_THREADS + threadIdx.x;      \
       inf_[q] = c < N ? uint32_t(info[c]) : 0u;                                           \
     }                                                                                     \
     _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
-      const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+      const int64_t c = ps_block() * PS_CPB + q * PS_THREADS + threadIdx.x;      \
       const uint32_t inf = inf_[q];                                                       \
       if (inf & 0x8000u) {                                                                \
         __VA_ARGS__                                                                       \
@@ -248,11 +258,11 @@ __device__ __forceinline__ double block_max_d(double v, double *sh) {
   uint32_t inf[PS_CPT];                                                                 \
   int64_t cc[PS_CPT];                                                                   \
   _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
-    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+    const int64_t c = ps_block() * PS_CPB + q * PS_THREADS + threadIdx.x;      \
     inf[q] = c < N ? uint32_t(info[c]) : 0u;                                            \
   }                                                                                     \
   _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) {                                  \
-    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;      \
+    const int64_t c = ps_block() * PS_CPB + q * PS_THREADS + threadIdx.x;      \
     cc[q] = (inf[q] & 0x8000u) ? c : safe;                                              \
   }                                                                                     \
   /* a wave with no FLUID cell (air or solid regions) skips its loads and stores */     \
@@ -685,7 +695,7 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_init(const uint8_t *flags, co
   const uint8_t *fl = flags + base;
   double mx = 0.0;
   for (int q = 0; q < PS_CPT; q++) {
-    const int64_t c = int64_t(blockIdx.x) * PS_CPB + q * PS_THREADS + threadIdx.x;
+    const int64_t c = ps_block() * PS_CPB + q * PS_THREADS + threadIdx.x;
     if (c >= N) continue;
     const bool liq = fl[c] == FLUID;
     const float bb = liq ? bv[base + c] : 0.f;
