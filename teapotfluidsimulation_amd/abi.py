@@ -182,9 +182,18 @@ def diagnostics_dict(d: Diagnostics) -> dict:
 
 
 def config_to_dict(c: Config) -> dict:
-    """Config3D (solver3d.ts:25-43) as a nested camelCase dict."""
+    """Config3D (solver3d.ts:25-43) as a nested camelCase dict.  The build-only key
+    teapot.spoutKind appears only for a non-tube pot, so the reference's configurations keep the
+    reference's shape (recording headers embed them byte for byte)."""
     t, m = c.teapot, c.motion
     kinds = {v: k for k, v in MOTION_KINDS.items()}
+    d = _config_dict(c, t, m, kinds)
+    if t.spout_kind != SPOUT_KINDS["tube"]:
+        d["teapot"]["spoutKind"] = {v: k for k, v in SPOUT_KINDS.items()}[t.spout_kind]
+    return d
+
+
+def _config_dict(c, t, m, kinds) -> dict:
     return {
         "teapot": {
             "radius": t.radius,
@@ -194,7 +203,6 @@ def config_to_dict(c: Config) -> dict:
             "boreRadius": t.bore_radius,
             "spoutAngleDeg": t.spout_angle_deg,
             "spoutLength": t.spout_length,
-            "spoutKind": {v: k for k, v in SPOUT_KINDS.items()}[t.spout_kind],
         },
         "h": c.h,
         "initialVolume": c.initial_volume,

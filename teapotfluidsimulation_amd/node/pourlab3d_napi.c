@@ -269,8 +269,8 @@ static napi_value config_to_js(napi_env env, const pl3d_config *c) {
            set_num(env, t, "boreRadius", c->teapot.bore_radius) &&
            set_num(env, t, "spoutAngleDeg", c->teapot.spout_angle_deg) &&
            set_num(env, t, "spoutLength", c->teapot.spout_length) &&
-           set_str(env, t, "spoutKind",
-                   c->teapot.spout_kind == PL3D_SPOUT_LIP ? SPOUT_KINDS[1] : SPOUT_KINDS[0]) &&
+           /* build-only key, only for a non-tube pot: reference configs keep their shape */
+           (c->teapot.spout_kind != PL3D_SPOUT_LIP || set_str(env, t, "spoutKind", SPOUT_KINDS[1])) &&
            set_val(env, o, "teapot", t) &&
            set_num(env, o, "h", c->h) && set_num(env, o, "initialVolume", c->initial_volume) &&
            set_str(env, m, "kind",

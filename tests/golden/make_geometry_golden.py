@@ -4,8 +4,8 @@ so the oracle's restatement, pinned by the geometry and ramp tests on the CPU, j
 device's pours).  The configurations are built here from the oracle's gentle pour, and the
 library's presets (pl3d_lip_pour, pl3d_low_spout_pour) must equal them.
 
-Usage: python tests/golden/make_geometry_golden.py [cellMm ...]   (default 10 5; ~2 min per pour
-at 5 mm)
+Usage: python tests/golden/make_geometry_golden.py [cellMm ...] [--case lip|lowSpout|ramp]
+(default 10 5, every case; ~2 min per pour at 5 mm)
 """
 import json
 import os
@@ -41,7 +41,16 @@ def low_spout_cfg(cell_mm, rate=9.0):
     return c
 
 
-CASES = {"lip": lip_cfg, "lowSpout": low_spout_cfg}
+def ramp_cfg(cell_mm, rate=15.0):
+    """pl3d_ramp_pour(cell, rate, 90): the gentle pour with a constant-rate tilt to 90 deg."""
+    c = O.gentle_pour(cell_mm, 20)
+    c.motion.kind = 4
+    c.motion.hold_tilt_rate = rate
+    c.motion.max_angle_deg = 90.0
+    return c
+
+
+CASES = {"lip": lip_cfg, "lowSpout": low_spout_cfg, "ramp": ramp_cfg}
 
 
 def run(sim, chunk=0.5):
@@ -57,12 +66,20 @@ def run(sim, chunk=0.5):
 
 
 def main():
-    cells = [float(c) for c in sys.argv[1:]] or [10.0, 5.0]
+    args = sys.argv[1:]
+    only = None
+    if "--case" in args:  # --case NAME: only that case
+        k = args.index("--case")
+        only = args[k + 1]
+        args = args[:k] + args[k + 2:]
+    cells = [float(c) for c in args] or [10.0, 5.0]
     path = os.path.join(HERE, "geometry_pours.json")
     out = json.load(open(path)) if os.path.exists(path) else {"source": __doc__.split("\n")[0],
                                                                  "rows": []}
     O.build()
     for case, make in CASES.items():
+        if only and case != only:
+            continue
         for cell in cells:
             t0 = time.perf_counter()
             sim = O.OracleSim(make(cell))

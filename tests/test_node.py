@@ -54,7 +54,9 @@ def test_geometry_presets_match_python_host():
         for k in drop:
             py.pop(k)
         assert js == py
-    assert js["teapot"]["spoutKind"] == "tube" and js["motion"]["kind"] == "ramp"
+    assert "spoutKind" not in js["teapot"] and js["motion"]["kind"] == "ramp"
+    js = node_json("console.log(JSON.stringify(require('./src/sim3d/presets3d').lipPour3D(4)))")
+    assert js["teapot"]["spoutKind"] == "lip"
 
 
 def test_argument_errors_are_thrown_before_device_work():
