@@ -59,3 +59,23 @@ struct WgradArgs {
 // flexible CG: damped-Jacobi steps applied to the network's output (z += w D^-1 (r - A z))
 #define NET_POST_SMOOTH 4
 #define NET_SMOOTH_OMEGA 0.6f
+// the host reads the flexible CG's state every NET_FCG_POLL iterations (the iterations enqueued
+// after convergence find `done` set and change nothing)
+#define NET_FCG_POLL 4
+
+// Scalars of the flexible CG, kept on the device so an iteration needs no host round trip; the
+// single-block kernels k_fcg_alpha / k_fcg_rcheck / k_fcg_tcheck / k_fcg_dcheck update them
+// between the vector kernels, which read alpha, beta and the input scale from here.
+struct FcgState {
+  double zr;         // z.r of the current search direction's z
+  double alpha, beta;
+  double rmax;       // max|r|, recursive or after a true-residual refresh (the network's scale)
+  double target;     // stop when max|r| <= target
+  int it;            // iterations taken
+  int lastTrue;      // iteration of the last FP64 true-residual refresh
+  int done, converged;
+  int needTrue;      // this iteration refreshes the true residual
+  int fallback;      // this iteration's z is the Jacobi fallback
+  int fallbacks;     // count of those
+  int init;          // the first z: beta = 0
+};

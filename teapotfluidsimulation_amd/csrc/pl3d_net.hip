@@ -325,7 +325,9 @@ __device__ __forceinline__ void block_reduce2(double a, double b, double *pa, do
 
 // Ap = A p and partial p.Ap
 __global__ __launch_bounds__(256) void k_fcg_spmv(const float *p, const uint8_t *fl, int nx,
-                                                  int ny, int V, float *Ap, double *partial) {
+                                                  int ny, int V, float *Ap, double *partial,
+                                                  const FcgState *st) {
+  if (st->done) return;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   double d = 0.0;
   if (v < V) {
@@ -338,11 +340,46 @@ __global__ __launch_bounds__(256) void k_fcg_spmv(const float *p, const uint8_t 
   block_reduce2(d, 0.0, &partial[blockIdx.x], &dummy);
 }
 
+// sum (op 0) or max (op 1) of nb per-block partials, in a fixed order; valid in thread 0
+template <class T>
+__device__ __forceinline__ double fixed_reduce(const T *part, int nb, int stride, int op) {
+  __shared__ double red[256];
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    const double x = double(part[size_t(b) * stride]);
+    a = op ? fmax(a, x) : a + x;
+  }
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      red[threadIdx.x] = op ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                            : red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// alpha = z.r / p.Ap (one block); s.Ap <= 0 ends the solve unconverged (solver3d.ts:916)
+__global__ __launch_bounds__(256) void k_fcg_alpha(const double *partial, int nb, FcgState *st) {
+  if (st->done) return;
+  const double pap = fixed_reduce(partial, nb, 1, 0);
+  if (threadIdx.x == 0) {
+    st->it += 1;
+    if (!(pap > 0.0))
+      st->done = 1;
+    else
+      st->alpha = st->zr / pap;
+  }
+}
+
 // x += alpha p (FP64), rold = r, r -= alpha Ap; partial max|r|
-__global__ __launch_bounds__(256) void k_fcg_update(double alpha, const float *p, const float *Ap,
-                                                    double *x, float *r, float *rold, int V,
-                                                    float *pmax) {
+__global__ __launch_bounds__(256) void k_fcg_update(const float *p, const float *Ap, double *x,
+                                                    float *r, float *rold, int V, float *pmax,
+                                                    const FcgState *st) {
+  if (st->done) return;
   __shared__ float red[256];
+  const double alpha = st->alpha;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   float m = 0.f;
   if (v < V) {
@@ -362,80 +399,22 @@ __global__ __launch_bounds__(256) void k_fcg_update(double alpha, const float *p
   if (threadIdx.x == 0) pmax[blockIdx.x] = red[0];
 }
 
-// z = s * znet (the net's output for r / s), or with jacobi != 0 the fallback z = r / diag;
-// partials z.r, z.rold, z.z and r.r (4 per block)
-__global__ __launch_bounds__(256) void k_fcg_dots(const float *znet, float s, const float *r,
-                                                  const float *rold, const uint8_t *fl, int nx,
-                                                  int ny, int V, int jacobi, float *z,
-                                                  double *part) {
-  __shared__ double red[4][256];
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  double q[4] = {0.0, 0.0, 0.0, 0.0};
-  if (v < V) {
-    float zz;
-    if (jacobi) {
-      zz = 0.f;
-      if (fl[v] == PL3D_FLUID_) {
-        const int nxy = nx * ny;
-        const int offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
-        int dg = 0;
-        for (int o = 0; o < 6; o++) dg += fl[v + offs[o]] != PL3D_SOLID_;
-        zz = r[v] / float(dg);
-      }
-    } else {
-      zz = s * znet[v];
-    }
-    z[v] = zz;
-    q[0] = double(zz) * double(r[v]);
-    q[1] = double(zz) * double(rold[v]);
-    q[2] = double(zz) * double(zz);
-    q[3] = double(r[v]) * double(r[v]);
+// recursive max|r|; a true-residual refresh at the stop test and at least every 25 iterations
+__global__ __launch_bounds__(256) void k_fcg_rcheck(const float *pmax, int nb, FcgState *st) {
+  if (st->done) return;
+  const double m = fixed_reduce(pmax, nb, 1, 1);
+  if (threadIdx.x == 0) {
+    st->rmax = m;
+    st->needTrue = (m <= st->target || st->it - st->lastTrue >= 25) ? 1 : 0;
   }
-  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = q[c];
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st)
-      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
-}
-
-// one damped-Jacobi step on A z = r after the network: zout = s zin + omega (r - A (s zin)) / diag
-// (the network captures the smooth part of A^-1 r; the smoothing step the rough part)
-__global__ __launch_bounds__(256) void k_fcg_smooth(const float *zin, float s, const float *r,
-                                                    const uint8_t *fl, int nx, int ny, int V,
-                                                    float omega, float *zout) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
-  float out = 0.f;
-  if (fl[v] == PL3D_FLUID_) {
-    const int nxy = nx * ny;
-    const int offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
-    float sum = 0.f;
-    int dg = 0;
-    for (int q = 0; q < 6; q++) {
-      const uint8_t f = fl[v + offs[q]];
-      dg += f != PL3D_SOLID_;
-      if (f == PL3D_FLUID_) sum += zin[v + offs[q]];
-    }
-    const float zv = s * zin[v];
-    const float az = float(dg) * zv - s * sum;
-    out = zv + omega * (r[v] - az) / float(dg);
-  }
-  zout[v] = out;
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(256) void k_fcg_dir(const float *z, double beta, int V, float *p) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v < V) p[v] = z[v] + float(beta) * p[v];
 }
 
 // FP64 true residual r = b - A x on FLUID (0 elsewhere); partial max|r|
 __global__ __launch_bounds__(256) void k_fcg_residual(const double *b, const double *x,
                                                       const uint8_t *fl, int nx, int ny, int V,
-                                                      float *r, double *pmax) {
+                                                      float *r, double *pmax,
+                                                      const FcgState *st) {
+  if (st && (st->done || !st->needTrue)) return;
   __shared__ double red[256];
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   double m = 0.0;
@@ -463,6 +442,140 @@ __global__ __launch_bounds__(256) void k_fcg_residual(const double *b, const dou
     __syncthreads();
   }
   if (threadIdx.x == 0) pmax[blockIdx.x] = red[0];
+}
+
+// after a refresh: its max|r| replaces the recursive one and decides convergence
+__global__ __launch_bounds__(256) void k_fcg_tcheck(const double *pmax, int nb, FcgState *st) {
+  if (st->done || !st->needTrue) return;
+  const double m = fixed_reduce(pmax, nb, 1, 1);
+  if (threadIdx.x == 0) {
+    st->lastTrue = st->it;
+    st->rmax = m;
+    st->needTrue = 0;
+    if (m <= st->target) {
+      st->done = 1;
+      st->converged = 1;
+    }
+  }
+}
+
+// the network's input [r / max|r|, FLUID, SOLID, 0], the scale read from the state
+__global__ __launch_bounds__(256) void k_fcg_pack(const float *r, const uint8_t *flags, int V,
+                                                  uint16_t *x0, const FcgState *st) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  const float inv_s = float(1.0 / st->rmax);
+  const uint8_t f = flags[v];
+  x0[4 * v] = f2bf(f == PL3D_FLUID_ ? r[v] * inv_s : 0.f);
+  x0[4 * v + 1] = f2bf(f == PL3D_FLUID_ ? 1.f : 0.f);
+  x0[4 * v + 2] = f2bf(f == PL3D_SOLID_ ? 1.f : 0.f);
+  x0[4 * v + 3] = 0;
+}
+
+// z = s * znet (the net's output for r / s; s from the state when sdev is set), or with
+// jacobi != 0 the fallback z = r / diag (only when the state asks for it); partials z.r, z.rold,
+// z.z and r.r (4 per block)
+__global__ __launch_bounds__(256) void k_fcg_dots(const float *znet, float s, const double *sdev,
+                                                  const float *r, const float *rold,
+                                                  const uint8_t *fl, int nx, int ny, int V,
+                                                  int jacobi, float *z, double *part,
+                                                  const FcgState *st) {
+  if (st->done || (jacobi && !st->fallback)) return;
+  if (sdev) s = float(*sdev);
+  __shared__ double red[4][256];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  if (v < V) {
+    float zz;
+    if (jacobi) {
+      zz = 0.f;
+      if (fl[v] == PL3D_FLUID_) {
+        const int nxy = nx * ny;
+        const int offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
+        int dg = 0;
+        for (int o = 0; o < 6; o++) dg += fl[v + offs[o]] != PL3D_SOLID_;
+        zz = r[v] / float(dg);
+      }
+    } else {
+      zz = s * znet[v];
+    }
+    z[v] = zz;
+    q[0] = double(zz) * double(r[v]);
+    q[1] = double(zz) * double(rold[v]);
+    q[2] = double(zz) * double(zz);
+    q[3] = double(r[v]) * double(r[v]);
+  }
+  for (int c = 0; c < 4; c++) red[c][threadIdx.x] = q[c];
+  __syncthreads();
+  for (int st2 = 128; st2 > 0; st2 >>= 1) {
+    if (threadIdx.x < st2)
+      for (int c = 0; c < 4; c++) red[c][threadIdx.x] += red[c][threadIdx.x + st2];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The dot sums of z (one block).  Pass 0 (the network's z): when z is not a descent direction,
+// cos(z, r) <= 1e-3, flags the Jacobi fallback for pass 1 and stops; otherwise, and in pass 1,
+// beta = (z.r - z.rold) / z.r_old (flexible, Polak-Ribiere; 0 for the first z) and z.r
+// becomes the direction's.
+__global__ __launch_bounds__(256) void k_fcg_dcheck(const double *part, int nb, int jacobi,
+                                                    FcgState *st) {
+  if (st->done || (jacobi && !st->fallback)) return;
+  double q[4];
+  for (int c = 0; c < 4; c++) {
+    q[c] = fixed_reduce(part + c, nb, 4, 0);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (!jacobi) {
+      if (!(q[0] > 1e-3 * sqrt(q[2] * q[3]))) {
+        st->fallback = 1;
+        st->fallbacks += 1;
+        return;
+      }
+      st->fallback = 0;
+    }
+    st->beta = st->init ? 0.0 : (q[0] - q[1]) / st->zr;
+    st->zr = q[0];
+    st->init = 0;
+  }
+}
+
+// one damped-Jacobi step on A z = r after the network: zout = s zin + omega (r - A (s zin)) / diag
+// (the network captures the smooth part of A^-1 r; the smoothing step the rough part); s from
+// the state when sdev is set
+__global__ __launch_bounds__(256) void k_fcg_smooth(const float *zin, float s, const double *sdev,
+                                                    const float *r, const uint8_t *fl, int nx,
+                                                    int ny, int V, float omega, float *zout) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  if (sdev) s = float(*sdev);
+  float out = 0.f;
+  if (fl[v] == PL3D_FLUID_) {
+    const int nxy = nx * ny;
+    const int offs[6] = {-1, 1, -nx, nx, -nxy, nxy};
+    float sum = 0.f;
+    int dg = 0;
+    for (int q = 0; q < 6; q++) {
+      const uint8_t f = fl[v + offs[q]];
+      dg += f != PL3D_SOLID_;
+      if (f == PL3D_FLUID_) sum += zin[v + offs[q]];
+    }
+    const float zv = s * zin[v];
+    const float az = float(dg) * zv - s * sum;
+    out = zv + omega * (r[v] - az) / float(dg);
+  }
+  zout[v] = out;
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(256) void k_fcg_dir(const float *z, int V, float *p,
+                                                 const FcgState *st) {
+  if (st->done) return;
+  const double beta = st->beta;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < V) p[v] = z[v] + float(beta) * p[v];
 }
 
 // ------------------------------------------------------------------------------ instantiations

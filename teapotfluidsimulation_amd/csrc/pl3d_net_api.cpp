@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -32,16 +33,23 @@ __global__ void k_poolT(const uint16_t *, const uint16_t *, int, int, int, int, 
                         uint16_t *);
 __global__ void k_loss(const float *, const float *, const uint8_t *, float, int, uint16_t *,
                        double *);
-__global__ void k_fcg_spmv(const float *, const uint8_t *, int, int, int, float *, double *);
-__global__ void k_fcg_update(double, const float *, const float *, double *, float *, float *,
-                             int, float *);
-__global__ void k_fcg_dots(const float *, float, const float *, const float *, const uint8_t *,
-                           int, int, int, int, float *, double *);
-__global__ void k_fcg_dir(const float *, double, int, float *);
-__global__ void k_fcg_smooth(const float *, float, const float *, const uint8_t *, int, int, int,
-                             float, float *);
+__global__ void k_fcg_spmv(const float *, const uint8_t *, int, int, int, float *, double *,
+                           const FcgState *);
+__global__ void k_fcg_alpha(const double *, int, FcgState *);
+__global__ void k_fcg_update(const float *, const float *, double *, float *, float *, int,
+                             float *, const FcgState *);
+__global__ void k_fcg_rcheck(const float *, int, FcgState *);
 __global__ void k_fcg_residual(const double *, const double *, const uint8_t *, int, int, int,
-                               float *, double *);
+                               float *, double *, const FcgState *);
+__global__ void k_fcg_tcheck(const double *, int, FcgState *);
+__global__ void k_fcg_pack(const float *, const uint8_t *, int, uint16_t *, const FcgState *);
+__global__ void k_fcg_dots(const float *, float, const double *, const float *, const float *,
+                           const uint8_t *, int, int, int, int, float *, double *,
+                           const FcgState *);
+__global__ void k_fcg_dcheck(const double *, int, int, FcgState *);
+__global__ void k_fcg_smooth(const float *, float, const double *, const float *, const uint8_t *,
+                             int, int, int, float, float *);
+__global__ void k_fcg_dir(const float *, int, float *, const FcgState *);
 
 namespace {
 
@@ -100,6 +108,7 @@ struct pl3d_net {
   float *fr = nullptr, *frold = nullptr, *fp = nullptr, *fAp = nullptr, *fz = nullptr,
         *fpart = nullptr;
   const float *zsel = nullptr;  // the preconditioned z before scaling (network or smoothed)
+  FcgState *fst = nullptr;      // the flexible CG's scalars
   // training set
   struct Example {
     uint8_t *flags;
@@ -373,6 +382,7 @@ int pl3d_net_create(int32_t nx, int32_t ny, int32_t nz, uint32_t seed, pl3d_net 
   A(&n->bq, V0); A(&n->xq, V0); A(&n->dp1, size_t(blocks(V0))); A(&n->dp2, 4 * size_t(blocks(V0)));
   A(&n->fr, V0); A(&n->frold, V0); A(&n->fp, V0); A(&n->fAp, V0); A(&n->fz, V0);
   A(&n->fpart, size_t(blocks(V0)));
+  A(&n->fst, 1);
   if (rc != PL3D_OK) {
     free_net(n);
     delete n;
@@ -596,96 +606,62 @@ int pl3d_net_solve(pl3d_net *n, const uint8_t *cell, const double *b, double *q,
   NHIP(hipMemsetAsync(n->xq, 0, V0 * 8, n->stream));
   NHIP(hipMemsetAsync(n->fp, 0, V0 * 4, n->stream));
   NHIP(hipMemsetAsync(n->frold, 0, V0 * 4, n->stream));
+  FcgState hs{};
+  hs.rmax = bnorm;
+  hs.target = target;
+  hs.init = 1;
+  NHIP(hipMemcpyAsync(n->fst, &hs, sizeof hs, hipMemcpyHostToDevice, n->stream));
   NHIP(hipEventRecord(e0, n->stream));
-  std::vector<double> h1(nb), h2(4 * size_t(nb));
-  int fallbacks = 0;
-  std::vector<float> hf(nb);
-  auto sum_d = [&](const double *d, std::vector<double> &h, double *s) -> int {
-    NHIP(hipMemcpyAsync(h.data(), d, nb * 8, hipMemcpyDeviceToHost, n->stream));
-    NHIP(hipStreamSynchronize(n->stream));
-    double a = 0.0;
-    for (double x : h) a += x;
-    *s = a;
-    return PL3D_OK;
-  };
-  // z = s N(r / s): the network on the current residual, s = max|r|; returns z.r and z.rold.
-  // Safeguard (the plain-PCG fallback): when the network's z is not a descent direction,
-  // cos(z, r) <= 1e-3, this iteration uses the Jacobi z = r / diag instead.
-  auto dots = [&](int jacobi, double s, double q4[4]) -> int {
-    hipLaunchKernelGGL(k_fcg_dots, dim3(nb), dim3(256), 0, n->stream, n->zsel, float(s), n->fr,
-                       n->frold, n->flags, nx, ny, int(V0), jacobi, n->fz, n->dp2);
-    NHIP(hipGetLastError());
-    NHIP(hipMemcpyAsync(h2.data(), n->dp2, size_t(nb) * 32, hipMemcpyDeviceToHost, n->stream));
-    NHIP(hipStreamSynchronize(n->stream));
-    for (int c = 0; c < 4; c++) q4[c] = 0.0;
-    for (int blk = 0; blk < nb; blk++)
-      for (int c = 0; c < 4; c++) q4[c] += h2[4 * size_t(blk) + c];
-    return PL3D_OK;
-  };
-  auto precondition = [&](double s, double *zr, double *zro) -> int {
-    hipLaunchKernelGGL(k_pack_input, dim3(nb), dim3(256), 0, n->stream, n->fr, n->flags,
-                       float(1.0 / s), int(V0), n->x0);
+  FcgState *st = n->fst;
+  // device address of the network's input scale (the state's rmax)
+  const double *sdev = reinterpret_cast<const double *>(reinterpret_cast<const char *>(n->fst) +
+                                                        offsetof(FcgState, rmax));
+  // z = s N(r / s) with s = max|r| (from the state), then the damped-Jacobi smoothing; z.r and
+  // z.rold on the device, with the plain-PCG fallback: when the network's z is not a descent
+  // direction, cos(z, r) <= 1e-3, this iteration uses the Jacobi z = r / diag instead.
+  auto precondition = [&]() {
+    hipLaunchKernelGGL(k_fcg_pack, dim3(nb), dim3(256), 0, n->stream, n->fr, n->flags, int(V0),
+                       n->x0, st);
     forward(n);
-    // damped-Jacobi post-smoothing of the network's z (ping-pong through fAp and fz; Ap is free)
     const float *zin = n->z;
-    float scale = float(s);
+    const double *scale = sdev;
     for (int k = 0; k < NET_POST_SMOOTH; k++) {
       float *zo = (k % 2 == 0) ? n->fAp : n->fz;
-      hipLaunchKernelGGL(k_fcg_smooth, dim3(nb), dim3(256), 0, n->stream, zin, scale, n->fr,
+      hipLaunchKernelGGL(k_fcg_smooth, dim3(nb), dim3(256), 0, n->stream, zin, 1.f, scale, n->fr,
                          n->flags, nx, ny, int(V0), NET_SMOOTH_OMEGA, zo);
       zin = zo;
-      scale = 1.f;
+      scale = nullptr;  // later steps smooth an already scaled z
     }
     n->zsel = zin;
-    double q4[4];
-    if (int rc = dots(0, scale, q4)) return rc;
-    if (!(q4[0] > 1e-3 * std::sqrt(q4[2] * q4[3]))) {
-      fallbacks++;
-      if (int rc = dots(1, s, q4)) return rc;
+    for (int jac = 0; jac < 2; jac++) {
+      hipLaunchKernelGGL(k_fcg_dots, dim3(nb), dim3(256), 0, n->stream, n->zsel, 1.f, scale,
+                         n->fr, n->frold, n->flags, nx, ny, int(V0), jac, n->fz, n->dp2, st);
+      hipLaunchKernelGGL(k_fcg_dcheck, dim3(1), dim3(256), 0, n->stream, n->dp2, nb, jac, st);
     }
-    *zr = q4[0];
-    *zro = q4[1];
-    return PL3D_OK;
+    hipLaunchKernelGGL(k_fcg_dir, dim3(nb), dim3(256), 0, n->stream, n->fz, int(V0), n->fp, st);
   };
-  double zr, zro;
-  if (int rc = precondition(bnorm, &zr, &zro)) return rc;
-  hipLaunchKernelGGL(k_fcg_dir, dim3(nb), dim3(256), 0, n->stream, n->fz, 0.0, int(V0), n->fp);
-  double rmax = bnorm;
-  int it = 0, lastTrue = 0;
-  bool converged = false;
-  while (it < max_iter) {
-    it++;
+  precondition();  // the first direction p = z
+  NHIP(hipGetLastError());
+  for (int k = 1; k <= max_iter; k++) {
     hipLaunchKernelGGL(k_fcg_spmv, dim3(nb), dim3(256), 0, n->stream, n->fp, n->flags, nx, ny,
-                       int(V0), n->fAp, n->dp1);
-    double pap;
-    if (int rc = sum_d(n->dp1, h1, &pap)) return rc;
-    if (!(pap > 0.0)) break;
-    const double alpha = zr / pap;
-    hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(256), 0, n->stream, alpha, n->fp, n->fAp,
-                       n->xq, n->fr, n->frold, int(V0), n->fpart);
-    NHIP(hipMemcpyAsync(hf.data(), n->fpart, nb * 4, hipMemcpyDeviceToHost, n->stream));
-    NHIP(hipStreamSynchronize(n->stream));
-    rmax = 0.0;
-    for (float x : hf) rmax = std::max(rmax, double(x));
-    if (rmax <= target || it - lastTrue >= 25) {  // FP64 true residual (reliable update)
-      lastTrue = it;
-      hipLaunchKernelGGL(k_fcg_residual, dim3(nb), dim3(256), 0, n->stream, n->bq, n->xq,
-                         n->flags, nx, ny, int(V0), n->fr, n->dp1);
-      NHIP(hipMemcpyAsync(h1.data(), n->dp1, nb * 8, hipMemcpyDeviceToHost, n->stream));
+                       int(V0), n->fAp, n->dp1, st);
+    hipLaunchKernelGGL(k_fcg_alpha, dim3(1), dim3(256), 0, n->stream, n->dp1, nb, st);
+    hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(256), 0, n->stream, n->fp, n->fAp, n->xq,
+                       n->fr, n->frold, int(V0), n->fpart, st);
+    hipLaunchKernelGGL(k_fcg_rcheck, dim3(1), dim3(256), 0, n->stream, n->fpart, nb, st);
+    // FP64 true residual (reliable update) at the stop test and every 25 iterations
+    hipLaunchKernelGGL(k_fcg_residual, dim3(nb), dim3(256), 0, n->stream, n->bq, n->xq, n->flags,
+                       nx, ny, int(V0), n->fr, n->dp1, st);
+    hipLaunchKernelGGL(k_fcg_tcheck, dim3(1), dim3(256), 0, n->stream, n->dp1, nb, st);
+    precondition();  // the next direction p = z + beta p
+    NHIP(hipGetLastError());
+    if (k % NET_FCG_POLL == 0 || k == max_iter) {
+      NHIP(hipMemcpyAsync(&hs, n->fst, sizeof hs, hipMemcpyDeviceToHost, n->stream));
       NHIP(hipStreamSynchronize(n->stream));
-      rmax = 0.0;
-      for (double x : h1) rmax = std::max(rmax, x);
-      if (rmax <= target) {
-        converged = true;
-        break;
-      }
+      if (hs.done) break;
     }
-    double zrn, zron;
-    if (int rc = precondition(rmax, &zrn, &zron)) return rc;
-    const double beta = (zrn - zron) / zr;  // flexible (Polak-Ribiere) beta
-    hipLaunchKernelGGL(k_fcg_dir, dim3(nb), dim3(256), 0, n->stream, n->fz, beta, int(V0), n->fp);
-    zr = zrn;
   }
+  NHIP(hipMemcpyAsync(&hs, n->fst, sizeof hs, hipMemcpyDeviceToHost, n->stream));
   NHIP(hipEventRecord(e1, n->stream));
   std::vector<double> hx(V0);
   NHIP(hipMemcpyAsync(hx.data(), n->xq, V0 * 8, hipMemcpyDeviceToHost, n->stream));
@@ -710,12 +686,12 @@ int pl3d_net_solve(pl3d_net *n, const uint8_t *cell, const double *b, double *q,
     }
     tr = std::max(tr, std::fabs(b[v] - (dg * hx[v] - s)));
   }
-  res.iterations = converged ? it : max_iter;
+  res.iterations = hs.converged ? hs.it : max_iter;
   res.converged = tr <= target ? 1 : 0;
-  res.residual = rmax;
+  res.residual = hs.rmax;
   res.true_residual = tr;
   res.kernel_ms = ms;
-  res.fallbacks = fallbacks;
+  res.fallbacks = hs.fallbacks;
   if (out) *out = res;
   return PL3D_OK;
 }
