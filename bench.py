@@ -72,7 +72,7 @@ def learned_comparison(a):
         logged = len(snaps)
         data = {"data": "this pour's systems every 0.05 s, every fourth held out"}
     progress(f"learned: {logged} systems logged, training on {len(tr)}")
-    net, info = learned.train(dims, tr, steps=4000, progress=True)
+    net, info = learned.train(dims, tr, steps=16000, progress=True)
     progress(f"learned: evaluating {len(te)} held-out systems")
     _, summary = learned.evaluate(net, dims, te, baselines=("mic0", "chebyshev"))
     net.destroy()
