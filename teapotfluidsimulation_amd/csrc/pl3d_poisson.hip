@@ -266,7 +266,9 @@ __device__ __forceinline__ int64_t ps_block() {
     cc[q] = (inf[q] & 0x8000u) ? c : safe;                                              \
   }                                                                                     \
   /* a wave with no FLUID cell (air or solid regions) skips its loads and stores */     \
-  const bool waveLiquid = __any(((inf[0] | inf[1] | inf[2] | inf[3]) & 0x8000u) != 0u);
+  uint32_t anyInf_ = 0u;                                                                \
+  _Pragma("unroll") for (int q = 0; q < PS_CPT; q++) anyInf_ |= inf[q];                 \
+  const bool waveLiquid = __any((anyInf_ & 0x8000u) != 0u);
 
 // As = A s, partial s.As per block
 template <class T>
