@@ -1,6 +1,6 @@
 """Writes tests/golden/apic_pours.json: full gentle pours with the APIC transfer computed by the FP64
 CPU oracle (the 3D reference has no APIC, so the oracle's own APIC restatement, pinned by the
-affine-preservation test, is the judge of the device's APIC pours).
+affine-preservation test, is the reference for the device's APIC pours).
 
 Usage: python tests/golden/make_apic_golden.py [cellMm ...]   (default 10 5 4; 4 mm takes ~30 min)
 """

@@ -1,6 +1,6 @@
 """Writes tests/golden/geometry_pours.json: full pours of the build-only geometry and motion
 variants computed by the FP64 CPU oracle (the reference has neither a rim lip nor a ramp motion,
-so the oracle's restatement, pinned by the geometry and ramp tests on the CPU, judges the
+so the oracle's restatement, pinned by the geometry and ramp tests on the CPU, checks the
 device's pours).  The configurations are built here from the oracle's gentle pour, and the
 library's presets (pl3d_lip_pour, pl3d_low_spout_pour) must equal them.
 

@@ -1,6 +1,6 @@
 """APIC transfer on the device against the FP64 oracle's APIC (3D MAC-APIC after
 src/sim/solver.ts:537-609 and :944-989; the 3D reference is PIC/FLIP only, so the oracle's own
-APIC, pinned by test_oracle.py::test_apic_affine_preservation, is the judge).
+APIC, pinned by test_oracle.py::test_apic_affine_preservation, is the reference).
 
 Bars, each stated at its assertion:
 - lockstep 20 substeps at 10 mm: the FLIP lockstep thresholds (e2e/gpuSolver.spec.ts:35-41), plus
