@@ -116,3 +116,15 @@ def slab(nx, ny, cell2d, b2d):
     b = np.zeros((3, ny, nx))
     b[1] = np.asarray(b2d).reshape(ny, nx)
     return cell.ravel(), b.ravel()
+
+
+def extrude(nx, ny, cell2d, b2d, nz):
+    """The 2D problem repeated on the layers k = 1 .. nz - 2 of a 3D grid (k = 0 and nz - 1
+    SOLID).  For a z-uniform vector the z couplings cancel (interior layers: +2 on the diagonal,
+    -1 from each layer; end layers: +1 and -1), so the 3D solution is the 2D one on every layer."""
+    cell = np.full((nz, ny, nx), SOLID, np.uint8)
+    b = np.zeros((nz, ny, nx))
+    for k in range(1, nz - 1):
+        cell[k] = np.asarray(cell2d).reshape(ny, nx)
+        b[k] = np.asarray(b2d).reshape(ny, nx)
+    return cell.ravel(), b.ravel()

@@ -55,3 +55,18 @@ def test_3d_oracle_on_a_slab_is_the_2d_solver(nx, ny, rows, seed, tol, cap):
     q3, r3 = O.pcg_solve(nx, ny, 3, c3, b3, tol, cap)
     assert r3.iterations == it2 and ok2
     assert np.abs(q3.reshape(3, ny, nx)[1].ravel() - q2).max() <= 1e-10 * np.abs(q2).max()
+
+
+@pytest.mark.parametrize("nx,ny,rows,seed,tol,cap", CASES[:2])
+def test_3d_oracle_on_an_extruded_problem_has_the_2d_solution(nx, ny, rows, seed, tol, cap):
+    """The 2D problem repeated over 10 layers (SOLID above and below): the 3D oracle's solution is
+    the 2D solver's on every layer, to the solve tolerance."""
+    cell = P.box(nx, ny, rows)
+    b = rhs(cell, seed)
+    q2, _, ok2, _ = P.solve(nx, ny, cell, b, 1e-12, 2000)
+    nz = 12
+    c3, b3 = P.extrude(nx, ny, cell, b, nz)
+    q3, r = O.pcg_solve(nx, ny, nz, c3, b3, 1e-12, 4000)
+    assert ok2 and r.converged
+    layers = q3.reshape(nz, ny, nx)[1:-1].reshape(nz - 2, -1)
+    assert np.abs(layers - q2[None, :]).max() <= 1e-9 * np.abs(q2).max()
