@@ -1,5 +1,5 @@
-"""FP64 restatement of the reference's 2D MIC(0)-PCG pressure solver (src/sim/pressure.ts), numpy
-and plain loops (2D test sizes only).
+"""FP64 restatement of the reference's 2D MIC(0)-PCG pressure solver (src/sim/pressure.ts) and its
+multigrid preconditioner (src/sim/multigrid.ts), numpy and plain loops (2D test sizes only).
 
 TEST INFRASTRUCTURE ONLY: the checker of the 2D cross-check, in which the device's 3D solver on a
 one-cell-thick slab (SOLID above and below) must reproduce this solver's iteration counts and
@@ -9,7 +9,12 @@ Followed: coefficients (pressure.ts:87-99: diag = non-solid neighbours, -1 coupl
 +x/+y neighbours), the MIC(0) factor (:178-197, tau 0.97, safety 0.25), the triangular solves in
 list order (:205-222), and the CG loop with the max-norm stop rule tested on the recursive
 residual after each update (:100-169).  Not followed: the gauge fixing of enclosed liquid
-components (:227-268); the problems used here all touch air.
+components (:227-268); the problems used here all touch air.  With precond="multigrid" the
+preconditioner is one V-cycle of multigrid.ts: levels halved (ceil) until a side is <= 8
+(:32-50); a coarse cell SOLID on the border, else AIR if any child is AIR, else FLUID if any child
+is FLUID, else SOLID (:56-79); per level the diagonal counts non-solid neighbours (:80-99); two
+damped-Jacobi sweeps (omega 0.8) down and up, 40 on the coarsest level, restriction = sum of the
+liquid children, piecewise-constant prolongation (:104-159).
 """
 import numpy as np
 
@@ -31,7 +36,105 @@ def box(nx, ny, fluid_rows, air_top=True):
     return cell
 
 
-def solve(nx, ny, cell, b, tol, max_iter):
+def mg_setup(nx, ny, cell):
+    """The level hierarchy (multigrid.ts:32-99): [(w, h, cell, diag, fluid list)]."""
+    dims = []
+    w, h = nx, ny
+    while True:
+        dims.append((w, h))
+        if w <= 8 or h <= 8:
+            break
+        w, h = (w + 1) // 2, (h + 1) // 2
+    levels = []
+    fine = np.asarray(cell, np.uint8)
+    for li, (w, h) in enumerate(dims):
+        if li == 0:
+            c = fine.copy()
+        else:
+            pw, ph, pc = levels[-1][0], levels[-1][1], levels[-1][2]
+            c = np.zeros(w * h, np.uint8)
+            for j in range(h):
+                for i in range(w):
+                    air = liquid = False
+                    for dj in (0, 1):
+                        for di in (0, 1):
+                            fi, fj = 2 * i + di, 2 * j + dj
+                            if fi >= pw or fj >= ph:
+                                continue
+                            t = pc[fi + fj * pw]
+                            air = air or t == AIR
+                            liquid = liquid or t == FLUID
+                    border = i == 0 or j == 0 or i == w - 1 or j == h - 1
+                    c[i + j * w] = SOLID if border else (AIR if air else (FLUID if liquid else SOLID))
+        diag = np.zeros(w * h)
+        lst = [k for k in range(w * h) if c[k] == FLUID]
+        for k in lst:
+            diag[k] = sum(c[k + o] != SOLID for o in (-1, 1, -w, w))
+        levels.append((w, h, c, diag, lst))
+    return levels
+
+
+def mg_apply(levels, r, omega=0.8, smooth=2, coarse_sweeps=40):
+    """z = M^-1 r, one V-cycle from zero (multigrid.ts:102-159)."""
+    xs = [np.zeros(L[0] * L[1]) for L in levels]
+    bs = [np.zeros(L[0] * L[1]) for L in levels]
+    w0, h0, c0, d0, l0 = levels[0]
+    for k in l0:
+        bs[0][k] = r[k]
+
+    def residual(li):
+        w, h, c, diag, lst = levels[li]
+        x, bb = xs[li], bs[li]
+        res = np.zeros(w * h)
+        for k in lst:
+            ax = diag[k] * x[k]
+            for o in (-1, 1, -w, w):
+                if c[k + o] == FLUID:
+                    ax -= x[k + o]
+            res[k] = bb[k] - ax
+        return res
+
+    def jacobi(li, sweeps):
+        w, h, c, diag, lst = levels[li]
+        for _ in range(sweeps):
+            res = residual(li)
+            for k in lst:
+                if diag[k] > 0:
+                    xs[li][k] += omega * res[k] / diag[k]
+
+    def vcycle(li):
+        w, h, c, diag, lst = levels[li]
+        for k in lst:
+            xs[li][k] = 0.0
+        if li == len(levels) - 1:
+            jacobi(li, coarse_sweeps)
+            return
+        jacobi(li, smooth)
+        res = residual(li)
+        cw, ch, cc, cd, cl = levels[li + 1]
+        for k in cl:
+            bs[li + 1][k] = 0.0
+        for k in lst:
+            i, j = k % w, k // w
+            p = (i >> 1) + (j >> 1) * cw
+            if cc[p] == FLUID:
+                bs[li + 1][p] += res[k]
+        vcycle(li + 1)
+        for k in lst:
+            i, j = k % w, k // w
+            p = (i >> 1) + (j >> 1) * cw
+            if cc[p] == FLUID:
+                xs[li][k] += xs[li + 1][p]
+        jacobi(li, smooth)
+
+    vcycle(0)
+    z = np.zeros(w0 * h0)
+    for k in l0:
+        z[k] = xs[0][k]
+    return z
+
+
+def solve(nx, ny, cell, b, tol, max_iter, precond="mic0"):
     """Returns (q, iterations, converged, residual)."""
     n = nx * ny
     fl = [c for c in range(n) if cell[c] == FLUID]
@@ -64,7 +167,11 @@ def solve(nx, ny, cell, b, tol, max_iter):
                     + aj[c - nx] * x[c - nx])
         return y
 
+    mg = mg_setup(nx, ny, cell) if precond == "multigrid" else None
+
     def precondition(r):
+        if mg is not None:
+            return mg_apply(mg, r)
         qv = np.zeros(n)
         z = np.zeros(n)
         for c in fl:

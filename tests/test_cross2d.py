@@ -70,3 +70,39 @@ def test_3d_oracle_on_an_extruded_problem_has_the_2d_solution(nx, ny, rows, seed
     assert ok2 and r.converged
     layers = q3.reshape(nz, ny, nx)[1:-1].reshape(nz - 2, -1)
     assert np.abs(layers - q2[None, :]).max() <= 1e-9 * np.abs(q2).max()
+
+
+def pool(nx, ny, rows):
+    """tests/multigrid.test.ts:11-18 (the box of pressure.test.ts)."""
+    return P.box(nx, ny, rows)
+
+
+def test_2d_multigrid_known_answers():
+    """tests/multigrid.test.ts:21-71: the hierarchy is deeper than two levels and the V-cycle is
+    a symmetric operator; MG-PCG converges at 1e-8 in at most 1.5 times MIC(0)'s iterations; and
+    both preconditioners give the same solution."""
+    nx, ny = 40, 36
+    cell = pool(nx, ny, 20)
+    levels = P.mg_setup(nx, ny, cell)
+    assert len(levels) > 2
+    rng = Mulberry32(4)
+    a, b = np.zeros(nx * ny), np.zeros(nx * ny)
+    for c in range(nx * ny):
+        if cell[c] == P.FLUID:
+            a[c] = rng() - 0.5
+            b[c] = rng() - 0.5
+    ab = float(b @ P.mg_apply(levels, a))
+    ba = float(a @ P.mg_apply(levels, b))
+    assert abs(ab - ba) < 1e-9 * abs(ab)
+    nx = ny = 60
+    cell = pool(nx, ny, 45)
+    rhs60 = np.zeros(nx * ny)
+    for c in range(nx * ny):
+        i, j = c % nx, c // nx
+        if cell[c] == P.FLUID:
+            rhs60[c] = np.sin(np.pi * i / nx) * np.cos(np.pi * j / 45)
+    qm, itm, okm, _ = P.solve(nx, ny, cell, rhs60, 1e-8, 500, precond="mic0")
+    qg, itg, okg, _ = P.solve(nx, ny, cell, rhs60, 1e-8, 500, precond="multigrid")
+    print("MIC(0)", itm, "MG", itg)
+    assert okm and okg and itg <= 1.5 * itm
+    assert np.abs(qg - qm).max() <= 1e-6 * np.abs(qm).max()
