@@ -273,6 +273,8 @@ def poisson_bench(a):
     pc = a.poisson_precond
     if pc == "chebyshev":
         ps.setPreconditioner("chebyshev", a.poisson_degree, a.poisson_lower)
+    elif pc == "mg":
+        ps.setPreconditioner("mg")
     ps.solve("refined", 1e-6, 5)  # warm-up (allocation, first launches)
     runs = {}
     for prec in ("fp64", "refined", "fp32"):
@@ -296,6 +298,15 @@ def poisson_bench(a):
     steps_b = 4.0 + (12.0 + 16.0 * (deg - 2) if deg >= 2 else 8.0)
     per_cell = PS_VECTOR_BYTES + (steps_b if deg else 0.0)
     per_flag = PS_FLAG_BYTES + (2.0 * max(deg - 1, 1) if deg else 0.0)
+    if pc == "mg":
+        # one V-cycle on level 0: 8 red-black half sweeps (b and the neighbours' x read, x written
+        # for the half of the FLUID cells of the colour: 6 B per FLUID cell each, 48 B), the
+        # restriction (x, b of the children: 10 B) and the prolongation (x read and written: 8 B),
+        # with the info words of every cell read by the 8 sweeps and the prolongation (18 B per
+        # cell); coarser levels add at most 1/7 of that (each has at most 1/8 of the cells); z.r
+        # reads z and r (8 B) and the info words (2 B)
+        per_cell = PS_VECTOR_BYTES + 8.0 + 66.0 * 8.0 / 7.0
+        per_flag = PS_FLAG_BYTES + 2.0 + 18.0 * 8.0 / 7.0
     it_bytes = sum((per_cell * mi) * x["iterations"] for mi, x in zip(m, r)) + \
         per_flag * cells * iters
     achieved = it_bytes / (kms / 1000) / 1e9
@@ -312,7 +323,7 @@ def poisson_bench(a):
         }
     line = {
         "metric": f"Poisson set: {P} problems on {n}^3 solved to 1e-6 per second (FP32 + FP64 "
-                  f"refinement, {'Chebyshev' if pc == 'chebyshev' else 'Jacobi'}-PCG)",
+                  f"refinement, {dict(chebyshev='Chebyshev', mg='MG', jacobi='Jacobi')[pc]}-PCG)",
         "value": P / wall,
         "unit": "problems/s",
         "n_gpus": 1,
@@ -333,7 +344,9 @@ def poisson_bench(a):
         "precisions": summary,
         "roofline": {"kernel": "batched PCG iteration (k_ps_spmv, k_ps_update, k_ps_dir + "
                                "per-problem reductions" +
-                               (", k_ps_cheb0, k_ps_cheb, k_ps_zdot, k_ps_dirz)" if deg else ")"),
+                               (", k_ps_cheb0, k_ps_cheb, k_ps_zdot, k_ps_dirz)" if deg else
+                                (", k_mg_sweep, k_mg_restrict, k_mg_prolong, k_mg_coarsest, "
+                                 "k_ps_zdot, k_ps_dirz)" if pc == "mg" else ")")),
                      "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -372,7 +385,7 @@ def main():
     ap.add_argument("--poisson-n", type=int, default=128)
     ap.add_argument("--poisson-problems", type=int, default=256)
     ap.add_argument("--poisson-max-iter", type=int, default=4000)
-    ap.add_argument("--poisson-precond", default="chebyshev", choices=["jacobi", "chebyshev"])
+    ap.add_argument("--poisson-precond", default="chebyshev", choices=["jacobi", "chebyshev", "mg"])
     ap.add_argument("--poisson-degree", type=int, default=8)
     ap.add_argument("--poisson-lower", type=float, default=0.01)
     ap.add_argument("--ensemble", type=int, default=0,

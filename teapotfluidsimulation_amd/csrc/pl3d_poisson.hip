@@ -635,6 +635,184 @@ __global__ __launch_bounds__(PS_THREADS) void k_ps_cheb(const uint16_t *__restri
   }
 }
 
+// ---- multigrid preconditioner (MG-PCG): one V-cycle of the pour solver's multigrid design
+// (restated in FP64 by the oracle's Multigrid3D), on all problems per launch.  Levels halve every
+// axis (ceil), a coarse cell is SOLID on the border, else AIR if any child is AIR, else FLUID if
+// any child is FLUID, else SOLID; per level the operator is the 7-point one on its FLUID cells.
+// Red-black Gauss-Seidel (red: i + j + k even), red then black twice going down, black then red
+// twice coming up, 8 + 8 mirrored sweep pairs on the coarsest level; restriction = half the sum of
+// the FLUID children's residuals, prolongation piecewise constant into FLUID cells.
+constexpr int MG_MAX_LEVELS = 4;
+constexpr int MG_COARSE_SWEEPS = 8;
+constexpr int MG_COARSE_MAX = 4096;  // cells of the coarsest level held in LDS (16^3)
+
+// coarse flags from fine flags (one coarse cell per thread)
+__global__ __launch_bounds__(256) void k_mg_coarsen(const uint8_t *__restrict__ ff, int nf,
+                                                    uint8_t *__restrict__ fc, int nc) {
+  const int64_t NC = int64_t(nc) * nc * nc, NF = int64_t(nf) * nf * nf;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= NC) return;
+  const int p = blockIdx.y;
+  const int i = int(c % nc), j = int((c / nc) % nc), k = int(c / (int64_t(nc) * nc));
+  uint8_t out = SOLID;
+  if (i != 0 && j != 0 && k != 0 && i != nc - 1 && j != nc - 1 && k != nc - 1) {
+    bool air = false, liquid = false;
+    for (int dk = 0; dk < 2; dk++)
+      for (int dj = 0; dj < 2; dj++)
+        for (int di = 0; di < 2; di++) {
+          const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+          if (fi >= nf || fj >= nf || fk >= nf) continue;
+          const uint8_t t = ff[int64_t(p) * NF + fi + int64_t(nf) * (fj + int64_t(nf) * fk)];
+          air = air || t == AIR;
+          liquid = liquid || t == FLUID;
+        }
+    out = air ? AIR : (liquid ? FLUID : SOLID);
+  }
+  fc[int64_t(p) * NC + c] = out;
+}
+
+// one colour of a red-black Gauss-Seidel sweep: x_c = (b_c + sum of FLUID neighbours' x) / diag
+// for the FLUID cells of that colour (a cell's own old value cancels); `first`: x = 0 before it
+template <class T>
+__global__ __launch_bounds__(PS_THREADS) void k_mg_sweep(const uint16_t *__restrict__ infov,
+                                                         T *__restrict__ xv,
+                                                         const T *__restrict__ bv,
+                                                         const Scal *__restrict__ sc, int n,
+                                                         int colour, int first) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const int64_t safe = 1 + n + int64_t(n) * n;
+  const uint16_t *info = infov + base;
+  T *x = xv + base;
+  const T *b = bv + base;
+  const int64_t off[6] = {-1, 1, -n, n, -int64_t(n) * n, int64_t(n) * n};
+  PS_LOAD_INFO();
+  if (!waveLiquid) return;
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) {  // cells of the other colour load from the safe cell
+    const int64_t c = cc[q];
+    const int ci = int(c % n), cj = int((c / n) % n), ck = int(c / (int64_t(n) * n));
+    if (((ci + cj + ck) & 1) != colour) inf[q] = 0u;
+    cc[q] = (inf[q] & 0x8000u) ? c : safe;
+  }
+  T nb[PS_CPT][6], bs[PS_CPT];
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) {
+    bs[q] = b[cc[q]];
+#pragma unroll
+    for (int o = 0; o < 6; o++) nb[q][o] = first ? T(0) : x[cc[q] + off[o]];
+  }
+#pragma unroll
+  for (int q = 0; q < PS_CPT; q++) {
+    const int dg = PS_DIAG(inf[q]);
+    if ((inf[q] & 0x8000u) && dg > 0) {
+      T sum = T(0);
+#pragma unroll
+      for (int o = 0; o < 6; o++) sum += PS_NB(inf[q], o) ? nb[q][o] : T(0);
+      x[cc[q]] = (bs[q] + sum) / T(dg);
+    }
+  }
+}
+
+// restriction: b_c = 0.5 * sum over the FLUID children of r = b - A x (one coarse FLUID cell per
+// thread, its children's residuals gathered; each fine cell has one parent)
+template <class T>
+__global__ __launch_bounds__(256) void k_mg_restrict(const uint16_t *__restrict__ infoF,
+                                                     const T *__restrict__ xF,
+                                                     const T *__restrict__ bF, int nf,
+                                                     const uint8_t *__restrict__ flagsC,
+                                                     T *__restrict__ bC, int nc,
+                                                     const Scal *__restrict__ sc) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t NC = int64_t(nc) * nc * nc, NF = int64_t(nf) * nf * nf;
+  const int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= NC || flagsC[int64_t(p) * NC + c] != FLUID) return;
+  const int i = int(c % nc), j = int((c / nc) % nc), k = int(c / (int64_t(nc) * nc));
+  const uint16_t *info = infoF + int64_t(p) * NF;
+  const T *x = xF + int64_t(p) * NF, *b = bF + int64_t(p) * NF;
+  const int64_t off[6] = {-1, 1, -nf, nf, -int64_t(nf) * nf, int64_t(nf) * nf};
+  T acc = T(0);
+  for (int dk = 0; dk < 2; dk++)
+    for (int dj = 0; dj < 2; dj++)
+      for (int di = 0; di < 2; di++) {
+        const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+        if (fi >= nf || fj >= nf || fk >= nf) continue;
+        const int64_t f = fi + int64_t(nf) * (fj + int64_t(nf) * fk);
+        const uint32_t inf = info[f];
+        if (!(inf & 0x8000u)) continue;
+        T sum = T(0);
+        for (int o = 0; o < 6; o++)
+          if (PS_NB(inf, o)) sum += x[f + off[o]];
+        acc += b[f] - (T(PS_DIAG(inf)) * x[f] - sum);
+      }
+  bC[int64_t(p) * NC + c] = T(0.5) * acc;
+}
+
+// prolongation: x_f += x_parent for FLUID fine cells whose parent is FLUID
+template <class T>
+__global__ __launch_bounds__(256) void k_mg_prolong(const uint16_t *__restrict__ infoF,
+                                                    T *__restrict__ xF, int nf,
+                                                    const uint8_t *__restrict__ flagsC,
+                                                    const T *__restrict__ xC, int nc,
+                                                    const Scal *__restrict__ sc) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int64_t NC = int64_t(nc) * nc * nc, NF = int64_t(nf) * nf * nf;
+  const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (f >= NF || !(infoF[int64_t(p) * NF + f] & 0x8000u)) return;
+  const int i = int(f % nf), j = int((f / nf) % nf), k = int(f / (int64_t(nf) * nf));
+  const int64_t pc = (i >> 1) + int64_t(nc) * ((j >> 1) + int64_t(nc) * (k >> 1));
+  if (flagsC[int64_t(p) * NC + pc] == FLUID) xF[int64_t(p) * NF + f] += xC[int64_t(p) * NC + pc];
+}
+
+// the coarsest level of one problem per workgroup, in LDS: from x = 0, `sweeps` red-black pairs
+// then `sweeps` black-red pairs
+template <class T>
+__global__ __launch_bounds__(256) void k_mg_coarsest(const uint16_t *__restrict__ infoC,
+                                                     T *__restrict__ xC,
+                                                     const T *__restrict__ bC, int nc,
+                                                     const Scal *__restrict__ sc, int sweeps) {
+  __shared__ T xs[MG_COARSE_MAX], bs[MG_COARSE_MAX];
+  __shared__ uint16_t is[MG_COARSE_MAX];
+  const int p = blockIdx.x;
+  if (!sc[p].active) return;
+  const int NC = nc * nc * nc;  // <= MG_COARSE_MAX, checked on the host
+  const int64_t base = int64_t(p) * NC;
+  for (int c = threadIdx.x; c < NC; c += 256) {
+    xs[c] = T(0);
+    is[c] = infoC[base + c];
+    bs[c] = (is[c] & 0x8000u) ? bC[base + c] : T(0);
+  }
+  __syncthreads();
+  const int off[6] = {-1, 1, -nc, nc, -nc * nc, nc * nc};
+  auto sweep = [&](int colour) {
+    for (int c = threadIdx.x; c < NC; c += 256) {
+      const uint32_t inf = is[c];
+      const int i = c % nc, j = (c / nc) % nc, k = c / (nc * nc);
+      const int dg = PS_DIAG(inf);
+      if ((inf & 0x8000u) && ((i + j + k) & 1) == colour && dg > 0) {
+        T sum = T(0);
+        for (int o = 0; o < 6; o++)
+          if (PS_NB(inf, o)) sum += xs[c + off[o]];
+        xs[c] = (bs[c] + sum) / T(dg);
+      }
+    }
+    __syncthreads();
+  };
+  for (int s = 0; s < sweeps; s++) {
+    sweep(0);
+    sweep(1);
+  }
+  for (int s = 0; s < sweeps; s++) {
+    sweep(1);
+    sweep(0);
+  }
+  for (int c = threadIdx.x; c < NC; c += 256)
+    if (is[c] & 0x8000u) xC[base + c] = xs[c];
+}
+
 // partial z.r over the FLUID cells (z from the preconditioner); only restart-flagged problems
 // with onlyRestart
 template <class T>
@@ -805,6 +983,14 @@ struct pl3d_poisson {
   int precond = PL3D_PRECOND_JACOBI, chebDegree = 6;
   double chebLower = 0.05;
   void *za = nullptr, *zb = nullptr;  // its two alternating z vectors (element size workElem)
+  // multigrid hierarchy (PL3D_PRECOND_MG): level 0 is the set itself; levels 1.. have their own
+  // flags, info words and x, b vectors (element size workElem)
+  int mgLevels = 0;
+  int mgN[MG_MAX_LEVELS] = {0, 0, 0, 0};
+  uint8_t *mgFlags[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
+  uint16_t *mgInfo[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
+  void *mgX[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
+  void *mgB[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
   bool destroyed = false;
 };
 
@@ -815,6 +1001,11 @@ void free_work(pl3d_poisson *ps) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
   }
+  for (int l = 1; l < MG_MAX_LEVELS; l++)
+    for (void **p : {&ps->mgX[l], &ps->mgB[l]}) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+    }
   if (ps->xhi) (void)hipFree(ps->xhi);
   ps->xhi = nullptr;
   ps->workElem = 0;
@@ -827,8 +1018,14 @@ int alloc_work(pl3d_poisson *ps, size_t elem, bool hi) {
     for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As}) PS_HIP(hipMalloc(p, cells * elem));
     ps->workElem = elem;
   }
-  if (ps->precond == PL3D_PRECOND_CHEBYSHEV && !ps->za)
+  if ((ps->precond == PL3D_PRECOND_CHEBYSHEV || ps->precond == PL3D_PRECOND_MG) && !ps->za)
     for (void **p : {&ps->za, &ps->zb}) PS_HIP(hipMalloc(p, cells * elem));
+  if (ps->precond == PL3D_PRECOND_MG)
+    for (int l = 1; l < ps->mgLevels; l++) {
+      const size_t nc = size_t(ps->mgN[l]) * ps->mgN[l] * ps->mgN[l] * size_t(ps->P);
+      if (!ps->mgX[l]) PS_HIP(hipMalloc(&ps->mgX[l], nc * elem));
+      if (!ps->mgB[l]) PS_HIP(hipMalloc(&ps->mgB[l], nc * elem));
+    }
   if (hi && !ps->xhi) PS_HIP(hipMalloc((void **)&ps->xhi, cells * 8));
   return PL3D_OK;
 }
@@ -836,6 +1033,39 @@ int alloc_work(pl3d_poisson *ps, size_t elem, bool hi) {
 int check_ps(pl3d_poisson *ps) {
   if (!ps) return perr(PL3D_EINVAL, "null Poisson set handle");
   if (ps->destroyed) return perr(PL3D_ESTATE, "Poisson set was destroyed");
+  return PL3D_OK;
+}
+
+// The multigrid hierarchy: levels while the next side is >= 4 (at most MG_MAX_LEVELS), flags
+// coarsened on the device for all problems, info words per problem; built once per set.
+int build_mg(pl3d_poisson *ps) {
+  if (ps->mgLevels) return PL3D_OK;
+  int L = 1, nl = ps->n;
+  ps->mgN[0] = nl;
+  ps->mgFlags[0] = ps->flags;
+  ps->mgInfo[0] = ps->info;
+  while (L < MG_MAX_LEVELS && (nl + 1) / 2 >= 4) {
+    nl = (nl + 1) / 2;
+    ps->mgN[L++] = nl;
+  }
+  if (L < 2 || int64_t(nl) * nl * nl > MG_COARSE_MAX)
+    return perr(PL3D_EINVAL, "the Poisson set's multigrid needs 8 <= n <= 128");
+  hipStream_t st = ps->stream;
+  const int T = 256;
+  for (int l = 1; l < L; l++) {
+    const int64_t nc = int64_t(ps->mgN[l]) * ps->mgN[l] * ps->mgN[l];
+    PS_HIP(hipMalloc((void **)&ps->mgFlags[l], size_t(nc) * ps->P));
+    PS_HIP(hipMalloc((void **)&ps->mgInfo[l], size_t(nc) * ps->P * 2));
+    hipLaunchKernelGGL(k_mg_coarsen, dim3(unsigned((nc + T - 1) / T), ps->P), dim3(T), 0, st,
+                       ps->mgFlags[l - 1], ps->mgN[l - 1], ps->mgFlags[l], ps->mgN[l]);
+    for (int p = 0; p < ps->P; p++)
+      hipLaunchKernelGGL(k_ps_info, dim3(unsigned((nc + T - 1) / T)), dim3(T), 0, st,
+                         ps->mgFlags[l] + size_t(p) * nc, ps->mgInfo[l] + size_t(p) * nc,
+                         ps->mgN[l]);
+    PS_HIP(hipGetLastError());
+  }
+  PS_HIP(hipStreamSynchronize(st));
+  ps->mgLevels = L;
   return PL3D_OK;
 }
 
@@ -874,6 +1104,51 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
                          n, bpp, 0);
     return cur;
   };
+  const bool mg = ps->precond == PL3D_PRECOND_MG;
+  // z = M r by one V-cycle (z in za) and the partial z.r per block (in ps->partial)
+  auto vcycle = [&]() -> T * {
+    T *xs[MG_MAX_LEVELS], *bs[MG_MAX_LEVELS];
+    xs[0] = za;
+    bs[0] = r;
+    for (int l = 1; l < ps->mgLevels; l++) {
+      xs[l] = static_cast<T *>(ps->mgX[l]);
+      bs[l] = static_cast<T *>(ps->mgB[l]);
+    }
+    auto cells = [&](int l) { return int64_t(ps->mgN[l]) * ps->mgN[l] * ps->mgN[l]; };
+    auto sweep = [&](int l, int colour, int first) {
+      const dim3 g(unsigned((cells(l) + PS_CPB - 1) / PS_CPB), P);
+      hipLaunchKernelGGL(k_mg_sweep<T>, g, blk, 0, st, ps->mgInfo[l], xs[l], bs[l], ps->sc,
+                         ps->mgN[l], colour, first);
+    };
+    const int L = ps->mgLevels;
+    for (int l = 0; l + 1 < L; l++) {
+      sweep(l, 0, 1);
+      sweep(l, 1, 0);
+      sweep(l, 0, 0);
+      sweep(l, 1, 0);
+      hipLaunchKernelGGL(k_mg_restrict<T>, dim3(unsigned((cells(l + 1) + 255) / 256), P),
+                         dim3(256), 0, st, ps->mgInfo[l], xs[l], bs[l], ps->mgN[l],
+                         ps->mgFlags[l + 1], bs[l + 1], ps->mgN[l + 1], ps->sc);
+    }
+    hipLaunchKernelGGL(k_mg_coarsest<T>, dim3(P), dim3(256), 0, st, ps->mgInfo[L - 1],
+                       xs[L - 1], bs[L - 1], ps->mgN[L - 1], ps->sc, MG_COARSE_SWEEPS);
+    for (int l = L - 2; l >= 0; l--) {
+      hipLaunchKernelGGL(k_mg_prolong<T>, dim3(unsigned((cells(l) + 255) / 256), P), dim3(256),
+                         0, st, ps->mgInfo[l], xs[l], ps->mgN[l], ps->mgFlags[l + 1], xs[l + 1],
+                         ps->mgN[l + 1], ps->sc);
+      sweep(l, 1, 0);
+      sweep(l, 0, 0);
+      sweep(l, 1, 0);
+      sweep(l, 0, 0);
+    }
+    hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, za, r, ps->partial, ps->sc, n,
+                       bpp, 0);
+    return za;
+  };
+  // z = M r for the Chebyshev or multigrid preconditioner (haveZ1: the update kernel did the
+  // first Chebyshev step)
+  auto precondition = [&](bool haveZ1) -> T * { return mg ? vcycle() : chebyshev(haveZ1); };
+  const bool poly = cheb || mg;
   hipEvent_t e0, e1;
   PS_HIP(hipEventCreate(&e0));
   PS_HIP(hipEventCreate(&e1));
@@ -881,8 +1156,8 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
   hipLaunchKernelGGL(k_ps_init<T>, grid, blk, 0, st, ps->flags, ps->b, x, r,
                      refined ? ps->xhi : nullptr, ps->partial, n, bpp);
   hipLaunchKernelGGL(k_ps_start, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, tol);
-  if (cheb) {
-    T *z = chebyshev(false);  // also z.r partials
+  if (poly) {
+    T *z = precondition(false);  // also z.r partials
     hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, z, s, ps->sc, n, 2);  // s = z
   } else {
     hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 2);  // s = z
@@ -899,10 +1174,10 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
     hipLaunchKernelGGL(k_ps_update<T>, grid, blk, 0, st, ps->info, x, r, s, As, ps->partial,
                        ps->partial2, ps->sc, n, bpp, cheb ? za : nullptr, float(chebTheta));
     T *z = nullptr;
-    if (cheb) z = chebyshev(true);  // z = M r; its z.r partials replace the update's Jacobi ones
+    if (poly) z = precondition(true);  // z = M r; its z.r partials replace the update's Jacobi ones
     hipLaunchKernelGGL(k_ps_beta, dim3(P), blk, 0, st, ps->partial, ps->partial2, ps->sc, bpp,
                        refined ? 1 : 0);
-    if (cheb)
+    if (poly)
       hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, z, s, ps->sc, n, 0);
     else
       hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 0);
@@ -931,8 +1206,8 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
         hipLaunchKernelGGL(k_ps_refresh_verdict, dim3(P), blk, 0, st, ps->partial, ps->partial2,
                            ps->sc, bpp);
         // restart where flagged: s = z, sigma = z.r
-        if (cheb) {  // z.r partials of every active problem; k_ps_sigma reads the flagged ones
-          T *zr = chebyshev(false);
+        if (poly) {  // z.r partials of every active problem; k_ps_sigma reads the flagged ones
+          T *zr = precondition(false);
           hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, zr, s, ps->sc, n, 1);
         } else {
           hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 1);
@@ -1041,6 +1316,10 @@ int pl3d_poisson_destroy(pl3d_poisson *ps) {
   if (int rc = check_ps(ps)) return rc;
   if (ps->stream) (void)hipStreamSynchronize(ps->stream);
   free_work(ps);
+  for (int l = 1; l < MG_MAX_LEVELS; l++) {
+    if (ps->mgFlags[l]) (void)hipFree(ps->mgFlags[l]);
+    if (ps->mgInfo[l]) (void)hipFree(ps->mgInfo[l]);
+  }
   for (void *p : {(void *)ps->flags, (void *)ps->info, (void *)ps->b, (void *)ps->partial, (void *)ps->partial2,
                   (void *)ps->sc, (void *)ps->sol[0], (void *)ps->sol[1], (void *)ps->sol[2]})
     if (p) (void)hipFree(p);
@@ -1067,10 +1346,14 @@ int pl3d_poisson_problem(pl3d_poisson *ps, int32_t k, uint8_t *flags, float *rhs
 int pl3d_poisson_set_preconditioner(pl3d_poisson *ps, int32_t precond, int32_t degree,
                                     double lower) {
   if (int rc = check_ps(ps)) return rc;
-  if (precond != PL3D_PRECOND_JACOBI && precond != PL3D_PRECOND_CHEBYSHEV)
-    return perr(PL3D_EINVAL, "the Poisson set solves with Jacobi or Chebyshev");
+  if (precond != PL3D_PRECOND_JACOBI && precond != PL3D_PRECOND_CHEBYSHEV &&
+      precond != PL3D_PRECOND_MG)
+    return perr(PL3D_EINVAL, "the Poisson set solves with Jacobi, Chebyshev or multigrid");
   if (degree < 0 || degree > 64 || !(lower >= 0.0 && lower < 2.0))
     return perr(PL3D_EINVAL, "Chebyshev degree must be in 0..64 and its lower bound in [0, 2)");
+  if (precond == PL3D_PRECOND_MG)
+    if (int rc = build_mg(ps)) return rc;
+  if (precond != ps->precond) free_work(ps);  // the next solve allocates what it needs
   ps->precond = precond;
   ps->chebDegree = degree > 0 ? degree : PL3D_CHEB_DEGREE_DEFAULT;
   ps->chebLower = lower > 0.0 ? lower : PL3D_CHEB_LOWER_DEFAULT;
