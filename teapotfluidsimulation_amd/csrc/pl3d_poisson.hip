@@ -189,7 +189,9 @@ __global__ void k_ps_info(const uint8_t *__restrict__ fl, uint16_t *__restrict__
     d += f != SOLID ? 1u : 0u;
     w |= f == FLUID ? (1u << o) : 0u;
   }
-  info[c] = uint16_t(w | (d << 8));
+  const int64_t i = c % n, j = (c / n) % n, k = c / (int64_t(n) * n);
+  const uint32_t parity = uint32_t((i + j + k) & 1);  // red-black colour for the multigrid
+  info[c] = uint16_t(w | (d << 8) | (parity << 11));
 }
 
 // ------------------------------------------------------------------------------- solver kernels
@@ -230,7 +232,7 @@ __device__ __forceinline__ int64_t ps_block() {
 // Each thread takes PS_CPT cells, PS_THREADS apart.  The cells' info words are loaded first, all
 // together; the body then runs for the FLUID cells with `c` (cell index) and `inf` (its word:
 // bit 15 FLUID, bits 0-5 the -x,+x,-y,+y,-z,+z neighbours that are FLUID, bits 8-10 the
-// non-solid neighbour count) in scope.  FLUID cells are never on the border, so all six
+// non-solid neighbour count, bit 11 the red-black colour (i + j + k) & 1) in scope.  FLUID cells are never on the border, so all six
 // neighbours of c exist and may be loaded unconditionally.
 #define PS_CELLS(...)                                                                     \
   {                                                                                       \
@@ -248,6 +250,7 @@ __device__ __forceinline__ int64_t ps_block() {
     }                                                                                     \
   }
 #define PS_DIAG(inf) int(((inf) >> 8) & 7u)
+#define PS_COLOUR(inf) int(((inf) >> 11) & 1u)
 #define PS_NB(inf, o) (((inf) >> (o)) & 1u)
 
 // Two-phase cell loop of the streaming kernels: the PS_CPT info words of this thread, then every
@@ -691,10 +694,8 @@ __global__ __launch_bounds__(PS_THREADS) void k_mg_sweep(const uint16_t *__restr
   if (!waveLiquid) return;
 #pragma unroll
   for (int q = 0; q < PS_CPT; q++) {  // cells of the other colour load from the safe cell
-    const int64_t c = cc[q];
-    const int ci = int(c % n), cj = int((c / n) % n), ck = int(c / (int64_t(n) * n));
-    if (((ci + cj + ck) & 1) != colour) inf[q] = 0u;
-    cc[q] = (inf[q] & 0x8000u) ? c : safe;
+    if (PS_COLOUR(inf[q]) != colour) inf[q] = 0u;
+    cc[q] = (inf[q] & 0x8000u) ? cc[q] : safe;
   }
   T nb[PS_CPT][6], bs[PS_CPT];
 #pragma unroll
@@ -790,9 +791,8 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(const uint16_t *__restrict_
   auto sweep = [&](int colour) {
     for (int c = threadIdx.x; c < NC; c += 256) {
       const uint32_t inf = is[c];
-      const int i = c % nc, j = (c / nc) % nc, k = c / (nc * nc);
       const int dg = PS_DIAG(inf);
-      if ((inf & 0x8000u) && ((i + j + k) & 1) == colour && dg > 0) {
+      if ((inf & 0x8000u) && PS_COLOUR(inf) == colour && dg > 0) {
         T sum = T(0);
         for (int o = 0; o < 6; o++)
           if (PS_NB(inf, o)) sum += xs[c + off[o]];
