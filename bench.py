@@ -273,8 +273,8 @@ def poisson_bench(a):
     pc = a.poisson_precond
     if pc == "chebyshev":
         ps.setPreconditioner("chebyshev", a.poisson_degree, a.poisson_lower)
-    elif pc == "mg":
-        ps.setPreconditioner("mg")
+    elif pc in ("mg", "mic0"):
+        ps.setPreconditioner(pc)
     ps.solve("refined", 1e-6, 5)  # warm-up (allocation, first launches)
     runs = {}
     for prec in ("fp64", "refined", "fp32"):
@@ -307,6 +307,12 @@ def poisson_bench(a):
         # reads z and r (8 B) and the info words (2 B)
         per_cell = PS_VECTOR_BYTES + 8.0 + 66.0 * 8.0 / 7.0
         per_flag = PS_FLAG_BYTES + 2.0 + 18.0 * 8.0 / 7.0
+    elif pc == "mic0":
+        # forward solve (r, the factor and the backward neighbours' q read, q written: 16 B per
+        # FLUID cell) and backward solve (q, the factor, the forward neighbours' z, z written:
+        # 16 B), each reading every cell's info word once over its hyperplane launches; z.r (8 B)
+        per_cell = PS_VECTOR_BYTES + 8.0 + 32.0
+        per_flag = PS_FLAG_BYTES + 2.0 + 4.0
     it_bytes = sum((per_cell * mi) * x["iterations"] for mi, x in zip(m, r)) + \
         per_flag * cells * iters
     achieved = it_bytes / (kms / 1000) / 1e9
@@ -323,7 +329,7 @@ def poisson_bench(a):
         }
     line = {
         "metric": f"Poisson set: {P} problems on {n}^3 solved to 1e-6 per second (FP32 + FP64 "
-                  f"refinement, {dict(chebyshev='Chebyshev', mg='MG', jacobi='Jacobi')[pc]}-PCG)",
+                  f"refinement, {dict(chebyshev='Chebyshev', mg='MG', jacobi='Jacobi', mic0='MIC(0)')[pc]}-PCG)",
         "value": P / wall,
         "unit": "problems/s",
         "n_gpus": 1,
@@ -346,7 +352,9 @@ def poisson_bench(a):
                                "per-problem reductions" +
                                (", k_ps_cheb0, k_ps_cheb, k_ps_zdot, k_ps_dirz)" if deg else
                                 (", k_mg_sweep, k_mg_restrict, k_mg_prolong, k_mg_coarsest, "
-                                 "k_ps_zdot, k_ps_dirz)" if pc == "mg" else ")")),
+                                 "k_ps_zdot, k_ps_dirz)" if pc == "mg" else
+                                 (", k_mic_fwd, k_mic_bwd, k_ps_zdot, k_ps_dirz)"
+                                  if pc == "mic0" else ")"))),
                      "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -385,7 +393,7 @@ def main():
     ap.add_argument("--poisson-n", type=int, default=128)
     ap.add_argument("--poisson-problems", type=int, default=256)
     ap.add_argument("--poisson-max-iter", type=int, default=4000)
-    ap.add_argument("--poisson-precond", default="chebyshev", choices=["jacobi", "chebyshev", "mg"])
+    ap.add_argument("--poisson-precond", default="chebyshev", choices=["jacobi", "chebyshev", "mg", "mic0"])
     ap.add_argument("--poisson-degree", type=int, default=8)
     ap.add_argument("--poisson-lower", type=float, default=0.01)
     ap.add_argument("--ensemble", type=int, default=0,

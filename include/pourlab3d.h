@@ -356,12 +356,15 @@ int pl3d_poisson_destroy(pl3d_poisson *ps);
 /* flags (n^3 bytes), rhs (n^3 floats) and FLUID count of problem k; any pointer may be NULL */
 int pl3d_poisson_problem(pl3d_poisson *ps, int32_t k, uint8_t *flags, float *rhs,
                          int64_t *liquid_cells);
-/* Solve every problem to max|r| <= max(tol max|b|, 1e-14) (solver3d.ts:910) at one precision;
- * out receives `problems` results.  The solution is kept per precision. */
-/* the preconditioner of later solves: PL3D_PRECOND_JACOBI (the default) or
- * PL3D_PRECOND_CHEBYSHEV with a degree and interval lower bound (0: the library defaults) */
+/* the preconditioner of later solves: PL3D_PRECOND_JACOBI (the default),
+ * PL3D_PRECOND_CHEBYSHEV with a degree and interval lower bound (0: the library defaults),
+ * PL3D_PRECOND_MG (one V-cycle of the pour solver's multigrid; 8 <= n <= 128) or
+ * PL3D_PRECOND_MIC0 (the pour solver's MIC(0), hyperplane-scheduled); the MG hierarchy and the
+ * MIC(0) factor are built on the first call that asks for them */
 int pl3d_poisson_set_preconditioner(pl3d_poisson *ps, int32_t precond, int32_t degree,
                                     double lower);
+/* Solve every problem to max|r| <= max(tol max|b|, 1e-14) (solver3d.ts:910) at one precision;
+ * out receives `problems` results.  The solution is kept per precision. */
 int pl3d_poisson_solve(pl3d_poisson *ps, int32_t precision, double tol, int32_t max_iter,
                        pl3d_poisson_result *out);
 /* the kept solution of problem k at a precision, n^3 doubles (0 off the liquid) */

@@ -813,6 +813,103 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(const uint16_t *__restrict_
     if (is[c] & 0x8000u) xC[base + c] = xs[c];
 }
 
+// ---- MIC(0) preconditioner (MIC-PCG), the pour solver's Pcg3D (solver3d.ts:884-900, :955-973)
+// on all problems per launch, level-scheduled: a cell's factor and forward-solve value depend
+// only on its -x, -y, -z neighbours, which lie on the previous hyperplane i + j + k = s - 1, so
+// one launch per hyperplane (3n - 2 of them) takes every cell of the plane of every problem; the
+// backward solve walks the planes in reverse.  Thread t of a launch takes (i, j) = (t mod n,
+// t / n) and k = s - i - j.
+
+// the factor pc = 1 / sqrt(e) of the FLUID cells of hyperplane s, in FP64 (tau 0.97, safety
+// 0.25 of the diagonal); the A+ coefficients are -1 toward FLUID neighbours (the info bits)
+__global__ __launch_bounds__(256) void k_mic_factor(const uint16_t *__restrict__ infov,
+                                                    double *__restrict__ pcv, int n, int s) {
+  const int64_t N = int64_t(n) * n * n, base = int64_t(blockIdx.y) * N;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int i = t % n, j = t / n, k = s - i - j;
+  if (k < 0 || k >= n) return;
+  const int64_t c = i + int64_t(n) * (j + int64_t(n) * k);
+  const uint16_t *info = infov + base;
+  double *pc = pcv + base;
+  const uint32_t inf = info[c];
+  if (!(inf & 0x8000u)) return;
+  const int64_t nxy = int64_t(n) * n;
+  const int64_t back[3] = {c - 1, c - n, c - nxy};
+  // info bit of the +x, +y, +z neighbour: 1, 3, 5; of the backward one: 0, 2, 4
+  double e = double(PS_DIAG(inf));
+  for (int a = 0; a < 3; a++) {
+    if (!PS_NB(inf, 2 * a)) continue;  // the backward neighbour on axis a is not FLUID
+    const uint32_t nb = info[back[a]];
+    const double p = pc[back[a]];
+    // A+ of the neighbour toward c is -1; its other two A+ coefficients, -1 when FLUID
+    const double others = -double(PS_NB(nb, 2 * ((a + 1) % 3) + 1)) -
+                          double(PS_NB(nb, 2 * ((a + 2) % 3) + 1));
+    e -= p * p;                 // (a+ p)^2 with a+ = -1
+    e -= 0.97 * (-1.0) * others * p * p;
+  }
+  if (e < 0.25 * double(PS_DIAG(inf))) e = double(PS_DIAG(inf));
+  pc[c] = e > 0.0 ? 1.0 / sqrt(e) : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mic_tofloat(const double *__restrict__ a,
+                                                     float *__restrict__ b, int64_t count) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+
+// forward solve on hyperplane s: qv_c = (r_c + sum over FLUID backward neighbours of pc qv) pc_c
+template <class T>
+__global__ __launch_bounds__(256) void k_mic_fwd(const uint16_t *__restrict__ infov,
+                                                 const T *__restrict__ pcv,
+                                                 const T *__restrict__ rv, T *__restrict__ qv,
+                                                 int n, int s, const Scal *__restrict__ sc) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int i = t % n, j = t / n, k = s - i - j;
+  if (k < 0 || k >= n) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const int64_t c = i + int64_t(n) * (j + int64_t(n) * k);
+  const uint32_t inf = infov[base + c];
+  if (!(inf & 0x8000u)) return;
+  const T *pc = pcv + base;
+  T *q = qv + base;
+  const int64_t back[3] = {c - 1, c - n, c - int64_t(n) * n};
+  T v = rv[base + c];
+  for (int a = 0; a < 3; a++)
+    if (PS_NB(inf, 2 * a)) v += pc[back[a]] * q[back[a]];
+  q[c] = v * pc[c];
+}
+
+// backward solve on hyperplane s: z_c = (qv_c + pc_c * sum over FLUID forward neighbours of z) pc_c
+template <class T>
+__global__ __launch_bounds__(256) void k_mic_bwd(const uint16_t *__restrict__ infov,
+                                                 const T *__restrict__ pcv,
+                                                 const T *__restrict__ qv, T *__restrict__ zv,
+                                                 int n, int s, const Scal *__restrict__ sc) {
+  const int p = blockIdx.y;
+  if (!sc[p].active) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * n) return;
+  const int i = t % n, j = t / n, k = s - i - j;
+  if (k < 0 || k >= n) return;
+  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
+  const int64_t c = i + int64_t(n) * (j + int64_t(n) * k);
+  const uint32_t inf = infov[base + c];
+  if (!(inf & 0x8000u)) return;
+  const T *pc = pcv + base;
+  T *z = zv + base;
+  const int64_t fwd[3] = {c + 1, c + n, c + int64_t(n) * n};
+  T sum = T(0);
+  for (int a = 0; a < 3; a++)
+    if (PS_NB(inf, 2 * a + 1)) sum += z[fwd[a]];
+  const T pcc = pc[c];
+  z[c] = (qv[base + c] + pcc * sum) * pcc;
+}
+
 // partial z.r over the FLUID cells (z from the preconditioner); only restart-flagged problems
 // with onlyRestart
 template <class T>
@@ -991,6 +1088,9 @@ struct pl3d_poisson {
   uint16_t *mgInfo[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
   void *mgX[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
   void *mgB[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
+  // MIC(0) factor (PL3D_PRECOND_MIC0), FP64 and its FP32 copy, one value per cell
+  double *micPcD = nullptr;
+  float *micPcF = nullptr;
   bool destroyed = false;
 };
 
@@ -1018,7 +1118,8 @@ int alloc_work(pl3d_poisson *ps, size_t elem, bool hi) {
     for (void **p : {&ps->x, &ps->r, &ps->s, &ps->As}) PS_HIP(hipMalloc(p, cells * elem));
     ps->workElem = elem;
   }
-  if ((ps->precond == PL3D_PRECOND_CHEBYSHEV || ps->precond == PL3D_PRECOND_MG) && !ps->za)
+  if ((ps->precond == PL3D_PRECOND_CHEBYSHEV || ps->precond == PL3D_PRECOND_MG ||
+       ps->precond == PL3D_PRECOND_MIC0) && !ps->za)
     for (void **p : {&ps->za, &ps->zb}) PS_HIP(hipMalloc(p, cells * elem));
   if (ps->precond == PL3D_PRECOND_MG)
     for (int l = 1; l < ps->mgLevels; l++) {
@@ -1066,6 +1167,24 @@ int build_mg(pl3d_poisson *ps) {
   }
   PS_HIP(hipStreamSynchronize(st));
   ps->mgLevels = L;
+  return PL3D_OK;
+}
+
+// The MIC(0) factor of every problem, one launch per hyperplane in increasing order; built once
+// per set.
+int build_mic(pl3d_poisson *ps) {
+  if (ps->micPcD) return PL3D_OK;
+  const size_t cells = size_t(ps->P) * size_t(ps->N);
+  PS_HIP(hipMalloc((void **)&ps->micPcD, cells * 8));
+  PS_HIP(hipMalloc((void **)&ps->micPcF, cells * 4));
+  const int n = ps->n;
+  const dim3 g(unsigned((n * n + 255) / 256), ps->P);
+  for (int s = 0; s <= 3 * (n - 1); s++)
+    hipLaunchKernelGGL(k_mic_factor, g, dim3(256), 0, ps->stream, ps->info, ps->micPcD, n, s);
+  hipLaunchKernelGGL(k_mic_tofloat, dim3(1024), dim3(256), 0, ps->stream, ps->micPcD,
+                     ps->micPcF, int64_t(cells));
+  PS_HIP(hipGetLastError());
+  PS_HIP(hipStreamSynchronize(ps->stream));
   return PL3D_OK;
 }
 
@@ -1145,10 +1264,26 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
                        bpp, 0);
     return za;
   };
-  // z = M r for the Chebyshev or multigrid preconditioner (haveZ1: the update kernel did the
-  // first Chebyshev step)
-  auto precondition = [&](bool haveZ1) -> T * { return mg ? vcycle() : chebyshev(haveZ1); };
-  const bool poly = cheb || mg;
+  const bool mic = ps->precond == PL3D_PRECOND_MIC0;
+  // z = M r by the MIC(0) triangular solves (q in zb, z in za) and the partial z.r per block
+  auto micsolve = [&]() -> T * {
+    const T *pcT = reinterpret_cast<const T *>(sizeof(T) == 8 ? static_cast<void *>(ps->micPcD)
+                                                              : static_cast<void *>(ps->micPcF));
+    const dim3 g(unsigned((n * n + 255) / 256), P);
+    for (int s2 = 0; s2 <= 3 * (n - 1); s2++)
+      hipLaunchKernelGGL(k_mic_fwd<T>, g, dim3(256), 0, st, ps->info, pcT, r, zb, n, s2, ps->sc);
+    for (int s2 = 3 * (n - 1); s2 >= 0; s2--)
+      hipLaunchKernelGGL(k_mic_bwd<T>, g, dim3(256), 0, st, ps->info, pcT, zb, za, n, s2, ps->sc);
+    hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, za, r, ps->partial, ps->sc, n,
+                       bpp, 0);
+    return za;
+  };
+  // z = M r for the Chebyshev, multigrid or MIC(0) preconditioner (haveZ1: the update kernel
+  // did the first Chebyshev step)
+  auto precondition = [&](bool haveZ1) -> T * {
+    return mg ? vcycle() : (mic ? micsolve() : chebyshev(haveZ1));
+  };
+  const bool poly = cheb || mg || mic;
   hipEvent_t e0, e1;
   PS_HIP(hipEventCreate(&e0));
   PS_HIP(hipEventCreate(&e1));
@@ -1158,6 +1293,7 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
   hipLaunchKernelGGL(k_ps_start, dim3(P), blk, 0, st, ps->partial, ps->sc, bpp, tol);
   if (poly) {
     T *z = precondition(false);  // also z.r partials
+    if (!z) return perr(PL3D_EDEVICE, "preconditioner graph capture or launch failed");
     hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, z, s, ps->sc, n, 2);  // s = z
   } else {
     hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 2);  // s = z
@@ -1174,7 +1310,8 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
     hipLaunchKernelGGL(k_ps_update<T>, grid, blk, 0, st, ps->info, x, r, s, As, ps->partial,
                        ps->partial2, ps->sc, n, bpp, cheb ? za : nullptr, float(chebTheta));
     T *z = nullptr;
-    if (poly) z = precondition(true);  // z = M r; its z.r partials replace the update's Jacobi ones
+    if (poly && !(z = precondition(true)))  // z = M r; its z.r partials replace the Jacobi ones
+      return perr(PL3D_EDEVICE, "preconditioner graph capture or launch failed");
     hipLaunchKernelGGL(k_ps_beta, dim3(P), blk, 0, st, ps->partial, ps->partial2, ps->sc, bpp,
                        refined ? 1 : 0);
     if (poly)
@@ -1208,6 +1345,7 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
         // restart where flagged: s = z, sigma = z.r
         if (poly) {  // z.r partials of every active problem; k_ps_sigma reads the flagged ones
           T *zr = precondition(false);
+          if (!zr) return perr(PL3D_EDEVICE, "preconditioner graph capture or launch failed");
           hipLaunchKernelGGL(k_ps_dirz<T>, grid, blk, 0, st, ps->info, zr, s, ps->sc, n, 1);
         } else {
           hipLaunchKernelGGL(k_ps_dir<T>, grid, blk, 0, st, ps->info, r, s, ps->sc, n, 1);
@@ -1320,6 +1458,8 @@ int pl3d_poisson_destroy(pl3d_poisson *ps) {
     if (ps->mgFlags[l]) (void)hipFree(ps->mgFlags[l]);
     if (ps->mgInfo[l]) (void)hipFree(ps->mgInfo[l]);
   }
+  if (ps->micPcD) (void)hipFree(ps->micPcD);
+  if (ps->micPcF) (void)hipFree(ps->micPcF);
   for (void *p : {(void *)ps->flags, (void *)ps->info, (void *)ps->b, (void *)ps->partial, (void *)ps->partial2,
                   (void *)ps->sc, (void *)ps->sol[0], (void *)ps->sol[1], (void *)ps->sol[2]})
     if (p) (void)hipFree(p);
@@ -1347,12 +1487,14 @@ int pl3d_poisson_set_preconditioner(pl3d_poisson *ps, int32_t precond, int32_t d
                                     double lower) {
   if (int rc = check_ps(ps)) return rc;
   if (precond != PL3D_PRECOND_JACOBI && precond != PL3D_PRECOND_CHEBYSHEV &&
-      precond != PL3D_PRECOND_MG)
-    return perr(PL3D_EINVAL, "the Poisson set solves with Jacobi, Chebyshev or multigrid");
+      precond != PL3D_PRECOND_MG && precond != PL3D_PRECOND_MIC0)
+    return perr(PL3D_EINVAL, "unknown preconditioner %d for the Poisson set", int(precond));
   if (degree < 0 || degree > 64 || !(lower >= 0.0 && lower < 2.0))
     return perr(PL3D_EINVAL, "Chebyshev degree must be in 0..64 and its lower bound in [0, 2)");
   if (precond == PL3D_PRECOND_MG)
     if (int rc = build_mg(ps)) return rc;
+  if (precond == PL3D_PRECOND_MIC0)
+    if (int rc = build_mic(ps)) return rc;
   if (precond != ps->precond) free_work(ps);  // the next solve allocates what it needs
   ps->precond = precond;
   ps->chebDegree = degree > 0 ? degree : PL3D_CHEB_DEGREE_DEFAULT;

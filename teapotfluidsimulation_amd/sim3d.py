@@ -524,8 +524,9 @@ class PoissonSet:
         return fl, b, m.value
 
     def setPreconditioner(self, name: str = "jacobi", degree: int = 0, lower: float = 0.0):
-        """"jacobi" (default), "chebyshev" (degree, interval [lower, 2]; 0: library defaults) or
-        "mg" (one multigrid V-cycle) for the following solves."""
+        """"jacobi" (default), "chebyshev" (degree, interval [lower, 2]; 0: library defaults),
+        "mg" (one multigrid V-cycle) or "mic0" (the MIC(0) triangular solves) for the following
+        solves."""
         check(load().pl3d_poisson_set_preconditioner(self._handle(), abi.PRECONDITIONERS[name],
                                                     int(degree), float(lower)))
 
