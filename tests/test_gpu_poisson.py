@@ -139,14 +139,25 @@ def test_mic0_preconditioned_set():
     ps.destroy()
 
 
-def test_empty_rhs_and_bad_arguments():
+@pytest.mark.parametrize("precond", ["jacobi", "chebyshev", "mg", "mic0"])
+def test_empty_rhs_and_bad_arguments(precond):
     ps = PoissonSet(8, 2, 5)  # an 8^3 grid: the wall leaves at most the four central columns
-    r = ps.solve("fp32", 1e-6, 50)
-    for k, x in enumerate(r):
-        fl, b, m = ps.problem(k)
-        assert x["converged"]
-        if m == 0 or np.abs(b).max() < 1e-14:
-            assert x["iterations"] == 0
+    ps.setPreconditioner(precond)  # MG on 8^3: two levels, the 4^3 one solved in LDS
+    for prec in ("fp32", "refined", "fp64"):
+        r = ps.solve(prec, 1e-6, 50)
+        for k, x in enumerate(r):
+            fl, b, m = ps.problem(k)
+            assert x["converged"], (prec, k)
+            if m == 0 or np.abs(b).max() < 1e-14:
+                assert x["iterations"] == 0
     with pytest.raises(RuntimeError):
         ps.solve("fp64", -1.0, 10)
+    ps.destroy()
+
+
+def test_mg_coarsest_level_must_fit_in_lds():
+    ps = PoissonSet(136, 1, 5)  # levels 136, 68, 34, 17: 17^3 cells exceed the 16^3 LDS level
+    with pytest.raises(RuntimeError):
+        ps.setPreconditioner("mg")
+    ps.setPreconditioner("chebyshev")  # the set stays usable
     ps.destroy()
