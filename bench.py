@@ -108,13 +108,14 @@ def solver_settings(a):
                     f"autotuner result {os.path.relpath(path, ROOT)}")
         return dict(autotune.DEFAULT_SOLVER), "library defaults (no tuned file)"
     return {"preconditioner": a.precond}, "command line"
-# HBM bytes per PCG launch on the 4 mm APIC pour (Jacobi; Chebyshev of degree 12), measured with
+# HBM bytes per PCG launch on the 4 mm APIC pour (Jacobi; Chebyshev of degree 12 and 8), measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE and calibrated on the library's copy kernel (FETCH x 2.0,
 # WRITE x 1.0), with the profile each comes from; other settings report traffic null.  The solve keeps its
 # vectors in LDS and registers, so this is far below the algorithmic bytes.
 MEASURED_TRAFFIC = {
     ("apic", "jacobi", 4.0, None): (1.433e6, "profiles/pour4mm_apic_jacobi_hbm_traffic.md"),
     ("apic", "chebyshev", 4.0, 12): (1.116e6, "profiles/pour4mm_apic_chebyshev_hbm_traffic.md"),
+    ("apic", "chebyshev", 4.0, 8): (1.060e6, "profiles/pour4mm_apic_chebyshev8_hbm_traffic.md"),
 }
 # Published reference figure for this metric: the reference's WebGPU FP32 4 mm pour, 0.87x real
 # time on an RTX A2000 Laptop (benchmarks/gpu3d.json:97-137).

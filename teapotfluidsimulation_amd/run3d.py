@@ -1,21 +1,29 @@
 """Full pour on the GPU, as benchmarks/run3d.ts:9-38 drives the CPU solver.
 
 Usage: python -m teapotfluidsimulation_amd.run3d [cellMm] [seconds] [--precond mic0|jacobi|mg|chebyshev]
-Prints run3d.json-shaped JSON (result, dischargedFractions, wallSeconds, history).
+       [--transfer flip|apic] [--tuned] [--kernel-times]
+Prints run3d.json-shaped JSON (result, dischargedFractions, wallSeconds, history; per-kernel
+device times with --kernel-times).
 """
 import argparse
 import json
+import os
 import sys
 import time
 
 from .sim3d import Simulation3D, gentlePour3D
 
 
-def run_pour(cell_mm=4.0, seconds=20.0, precond="mic0", progress=False, chunk=0.5, extra=None):
+def run_pour(cell_mm=4.0, seconds=20.0, precond="mic0", progress=False, chunk=0.5, extra=None,
+             kernel_times=False):
+    """kernel_times: launch every kernel directly with HIP events around it instead of replaying
+    the captured substep graph, and report the per-kernel device times."""
     cfg = gentlePour3D(cell_mm, seconds)
     cfg["preconditioner"] = precond
     cfg.update(extra or {})
     sim = Simulation3D(cfg)
+    if kernel_times:
+        sim.set_timing(True)
     rows = []
     done_at = -1.0
     max_it = 0
@@ -51,6 +59,10 @@ def run_pour(cell_mm=4.0, seconds=20.0, precond="mic0", progress=False, chunk=0.
         "realTimeFactor": d["time"] / wall,
         "history": rows,
     }
+    if kernel_times:
+        out["kernels"] = {k: {"total_ms": v[0], "launches": v[1],
+                              "avg_us": 1000 * v[0] / max(v[1], 1)}
+                          for k, v in sim.kernel_times().items()}
     sim.destroy()
     return out
 
@@ -60,8 +72,26 @@ def main():
     ap.add_argument("cell", nargs="?", type=float, default=4.0)
     ap.add_argument("seconds", nargs="?", type=float, default=20.0)
     ap.add_argument("--precond", default="mic0")
+    ap.add_argument("--transfer", default="flip", choices=["flip", "apic"])
+    ap.add_argument("--tuned", action="store_true",
+                    help="the autotuner's committed settings for this cell size and transfer "
+                         "(teapotfluidsimulation_amd/tuned/), overriding --precond")
+    ap.add_argument("--kernel-times", action="store_true",
+                    help="direct launches with per-kernel HIP events instead of the substep graph")
     a = ap.parse_args()
-    print(json.dumps(run_pour(a.cell, a.seconds, a.precond, progress=True), indent=1))
+    extra = {"transfer": a.transfer}
+    precond = a.precond
+    if a.tuned:
+        from . import autotune
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned",
+                            f"pour{a.cell:g}mm_{a.transfer}.json")
+        with open(path) as f:
+            t = json.load(f)
+        extra.update({k: t[k] for k in autotune.TUNED_KEYS if k in t})
+        precond = extra.pop("preconditioner", precond)
+    out = run_pour(a.cell, a.seconds, precond, progress=True, extra=extra,
+                   kernel_times=a.kernel_times)
+    print(json.dumps(out, indent=1))
 
 
 if __name__ == "__main__":
