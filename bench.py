@@ -309,11 +309,12 @@ def poisson_bench(a):
         per_cell = PS_VECTOR_BYTES + 8.0 + 66.0 * 8.0 / 7.0
         per_flag = PS_FLAG_BYTES + 2.0 + 18.0 * 8.0 / 7.0
     elif pc == "mic0":
-        # forward solve (r, the factor and the backward neighbours' q read, q written: 16 B per
-        # FLUID cell) and backward solve (q, the factor, the forward neighbours' z, z written:
-        # 16 B), each reading every cell's info word once over its hyperplane launches; z.r (8 B)
-        per_cell = PS_VECTOR_BYTES + 8.0 + 32.0
-        per_flag = PS_FLAG_BYTES + 2.0 + 4.0
+        # in plane order: the forward solve reads the position's cell and three backward links,
+        # the factor, r through the grid and the backward neighbours' q, and writes q (32 B per
+        # FLUID cell); the backward solve reads the cell, three forward links, the factor, q and
+        # the forward neighbours' z, and writes z in plane order and to the grid (36 B); z.r (8 B)
+        per_cell = PS_VECTOR_BYTES + 8.0 + 68.0
+        per_flag = PS_FLAG_BYTES + 2.0
     it_bytes = sum((per_cell * mi) * x["iterations"] for mi, x in zip(m, r)) + \
         per_flag * cells * iters
     achieved = it_bytes / (kms / 1000) / 1e9
@@ -354,7 +355,7 @@ def poisson_bench(a):
                                (", k_ps_cheb0, k_ps_cheb, k_ps_zdot, k_ps_dirz)" if deg else
                                 (", k_mg_sweep, k_mg_restrict, k_mg_prolong, k_mg_coarsest, "
                                  "k_ps_zdot, k_ps_dirz)" if pc == "mg" else
-                                 (", k_mic_fwd, k_mic_bwd, k_ps_zdot, k_ps_dirz)"
+                                 (", k_micp_fwd, k_micp_bwd, k_ps_zdot, k_ps_dirz)"
                                   if pc == "mic0" else ")"))),
                      "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

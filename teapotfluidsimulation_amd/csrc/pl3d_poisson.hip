@@ -816,40 +816,93 @@ __global__ __launch_bounds__(256) void k_mg_coarsest(const uint16_t *__restrict_
 // ---- MIC(0) preconditioner (MIC-PCG), the pour solver's Pcg3D (solver3d.ts:884-900, :955-973)
 // on all problems per launch, level-scheduled: a cell's factor and forward-solve value depend
 // only on its -x, -y, -z neighbours, which lie on the previous hyperplane i + j + k = s - 1, so
-// one launch per hyperplane (3n - 2 of them) takes every cell of the plane of every problem; the
-// backward solve walks the planes in reverse.  Thread t of a launch takes (i, j) = (t mod n,
-// t / n) and k = s - i - j.
+// one launch per hyperplane (3n - 2 of them) takes that plane of every problem; the backward
+// solve walks the planes in reverse.  A hyperplane has no contiguous run in the i-fastest grid,
+// so each problem's FLUID cells are renumbered plane by plane ("plane order": position t, with
+// the plane's cells in any order, since none depends on another of its plane): the factor and
+// the triangular solves' vectors live in plane order, with each position's grid cell and the
+// plane-order positions of its six neighbours (-1 when not FLUID) stored beside them.  Only the
+// read of r and the write of z go through the grid.
 
-// the factor pc = 1 / sqrt(e) of the FLUID cells of hyperplane s, in FP64 (tau 0.97, safety
-// 0.25 of the diagonal); the A+ coefficients are -1 toward FLUID neighbours (the info bits)
-__global__ __launch_bounds__(256) void k_mic_factor(const uint16_t *__restrict__ infov,
-                                                    double *__restrict__ pcv, int n, int s) {
-  const int64_t N = int64_t(n) * n * n, base = int64_t(blockIdx.y) * N;
+// FLUID cells per hyperplane, per problem
+__global__ __launch_bounds__(256) void k_micp_count(const uint16_t *__restrict__ infov, int n,
+                                                    int *__restrict__ counts, int S) {
+  const int64_t N = int64_t(n) * n * n, c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int p = blockIdx.y;
+  if (!(infov[int64_t(p) * N + c] & 0x8000u)) return;
+  const int s = int(c % n + (c / n) % n + c / (int64_t(n) * n));
+  atomicAdd(&counts[int64_t(p) * S + s], 1);
+}
+
+// plane-order position of every FLUID cell: its plane's start plus a slot taken atomically
+__global__ __launch_bounds__(256) void k_micp_place(const uint16_t *__restrict__ infov, int n,
+                                                    const int *__restrict__ start, int S,
+                                                    int *__restrict__ fill,
+                                                    const int64_t *__restrict__ base,
+                                                    int *__restrict__ pos,
+                                                    int *__restrict__ cellOf) {
+  const int64_t N = int64_t(n) * n * n, c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int p = blockIdx.y;
+  if (!(infov[int64_t(p) * N + c] & 0x8000u)) return;
+  const int s = int(c % n + (c / n) % n + c / (int64_t(n) * n));
+  const int t = start[int64_t(p) * (S + 1) + s] + atomicAdd(&fill[int64_t(p) * S + s], 1);
+  pos[int64_t(p) * N + c] = t;
+  cellOf[base[p] + t] = int(c);
+}
+
+// plane-order positions of each position's -x, -y, -z (links 0-2) and +x, +y, +z (3-5) FLUID
+// neighbours, -1 for the others
+__global__ __launch_bounds__(256) void k_micp_links(const uint16_t *__restrict__ infov, int n,
+                                                    const int64_t *__restrict__ base,
+                                                    const int *__restrict__ count,
+                                                    const int *__restrict__ pos,
+                                                    const int *__restrict__ cellOf,
+                                                    int *__restrict__ links, int64_t total) {
+  const int p = blockIdx.y;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * n) return;
-  const int i = t % n, j = t / n, k = s - i - j;
-  if (k < 0 || k >= n) return;
-  const int64_t c = i + int64_t(n) * (j + int64_t(n) * k);
-  const uint16_t *info = infov + base;
-  double *pc = pcv + base;
+  if (t >= count[p]) return;
+  const int64_t N = int64_t(n) * n * n, g = base[p] + t;
+  const int64_t c = cellOf[g];
+  const uint32_t inf = infov[int64_t(p) * N + c];
+  const int64_t off[3] = {1, n, int64_t(n) * n};
+  for (int a = 0; a < 3; a++) {
+    links[a * total + g] = PS_NB(inf, 2 * a) ? pos[int64_t(p) * N + c - off[a]] : -1;
+    links[(3 + a) * total + g] = PS_NB(inf, 2 * a + 1) ? pos[int64_t(p) * N + c + off[a]] : -1;
+  }
+}
+
+// the factor pc = 1 / sqrt(e) of hyperplane s, in FP64 (tau 0.97, safety 0.25 of the
+// diagonal); the A+ coefficients are -1 toward FLUID neighbours (the info bits)
+__global__ __launch_bounds__(256) void k_micp_factor(const uint16_t *__restrict__ infov, int n,
+                                                     const int64_t *__restrict__ base,
+                                                     const int *__restrict__ start, int S, int s,
+                                                     const int *__restrict__ cellOf,
+                                                     const int *__restrict__ links,
+                                                     int64_t total, double *__restrict__ pc) {
+  const int p = blockIdx.y;
+  const int t = start[int64_t(p) * (S + 1) + s] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= start[int64_t(p) * (S + 1) + s + 1]) return;
+  const int64_t N = int64_t(n) * n * n, g = base[p] + t;
+  const int64_t c = cellOf[g];
+  const uint16_t *info = infov + int64_t(p) * N;
   const uint32_t inf = info[c];
-  if (!(inf & 0x8000u)) return;
-  const int64_t nxy = int64_t(n) * n;
-  const int64_t back[3] = {c - 1, c - n, c - nxy};
-  // info bit of the +x, +y, +z neighbour: 1, 3, 5; of the backward one: 0, 2, 4
+  const int64_t off[3] = {1, n, int64_t(n) * n};
   double e = double(PS_DIAG(inf));
   for (int a = 0; a < 3; a++) {
-    if (!PS_NB(inf, 2 * a)) continue;  // the backward neighbour on axis a is not FLUID
-    const uint32_t nb = info[back[a]];
-    const double p = pc[back[a]];
-    // A+ of the neighbour toward c is -1; its other two A+ coefficients, -1 when FLUID
+    const int b = links[a * total + g];
+    if (b < 0) continue;  // the backward neighbour on axis a is not FLUID
+    const uint32_t nb = info[c - off[a]];
+    const double q = pc[base[p] + b];
+    // the neighbour's A+ toward c is -1; its other two A+ coefficients are -1 when FLUID
     const double others = -double(PS_NB(nb, 2 * ((a + 1) % 3) + 1)) -
                           double(PS_NB(nb, 2 * ((a + 2) % 3) + 1));
-    e -= p * p;                 // (a+ p)^2 with a+ = -1
-    e -= 0.97 * (-1.0) * others * p * p;
+    e -= q * q;
+    e -= 0.97 * (-1.0) * others * q * q;
   }
   if (e < 0.25 * double(PS_DIAG(inf))) e = double(PS_DIAG(inf));
-  pc[c] = e > 0.0 ? 1.0 / sqrt(e) : 0.0;
+  pc[g] = e > 0.0 ? 1.0 / sqrt(e) : 0.0;
 }
 
 __global__ __launch_bounds__(256) void k_mic_tofloat(const double *__restrict__ a,
@@ -859,55 +912,53 @@ __global__ __launch_bounds__(256) void k_mic_tofloat(const double *__restrict__ 
     b[i] = float(a[i]);
 }
 
-// forward solve on hyperplane s: qv_c = (r_c + sum over FLUID backward neighbours of pc qv) pc_c
+// forward solve on hyperplane s: q_t = (r_c + sum over FLUID backward neighbours of pc q) pc_t
 template <class T>
-__global__ __launch_bounds__(256) void k_mic_fwd(const uint16_t *__restrict__ infov,
-                                                 const T *__restrict__ pcv,
-                                                 const T *__restrict__ rv, T *__restrict__ qv,
-                                                 int n, int s, const Scal *__restrict__ sc) {
+__global__ __launch_bounds__(256) void k_micp_fwd(const int64_t *__restrict__ base,
+                                                  const int *__restrict__ start, int S, int s,
+                                                  const int *__restrict__ cellOf,
+                                                  const int *__restrict__ links, int64_t total,
+                                                  const T *__restrict__ pc,
+                                                  const T *__restrict__ rv, T *__restrict__ q,
+                                                  int n, const Scal *__restrict__ sc) {
   const int p = blockIdx.y;
   if (!sc[p].active) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * n) return;
-  const int i = t % n, j = t / n, k = s - i - j;
-  if (k < 0 || k >= n) return;
-  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const int64_t c = i + int64_t(n) * (j + int64_t(n) * k);
-  const uint32_t inf = infov[base + c];
-  if (!(inf & 0x8000u)) return;
-  const T *pc = pcv + base;
-  T *q = qv + base;
-  const int64_t back[3] = {c - 1, c - n, c - int64_t(n) * n};
-  T v = rv[base + c];
-  for (int a = 0; a < 3; a++)
-    if (PS_NB(inf, 2 * a)) v += pc[back[a]] * q[back[a]];
-  q[c] = v * pc[c];
+  const int t = start[int64_t(p) * (S + 1) + s] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= start[int64_t(p) * (S + 1) + s + 1]) return;
+  const int64_t g = base[p] + t;
+  T v = rv[int64_t(p) * n * n * n + cellOf[g]];
+  for (int a = 0; a < 3; a++) {
+    const int b = links[a * total + g];
+    if (b >= 0) v += pc[base[p] + b] * q[base[p] + b];
+  }
+  q[g] = v * pc[g];
 }
 
-// backward solve on hyperplane s: z_c = (qv_c + pc_c * sum over FLUID forward neighbours of z) pc_c
+// backward solve on hyperplane s: z_t = (q_t + pc_t * sum over FLUID forward neighbours of z)
+// pc_t, written in plane order and to the grid
 template <class T>
-__global__ __launch_bounds__(256) void k_mic_bwd(const uint16_t *__restrict__ infov,
-                                                 const T *__restrict__ pcv,
-                                                 const T *__restrict__ qv, T *__restrict__ zv,
-                                                 int n, int s, const Scal *__restrict__ sc) {
+__global__ __launch_bounds__(256) void k_micp_bwd(const int64_t *__restrict__ base,
+                                                  const int *__restrict__ start, int S, int s,
+                                                  const int *__restrict__ cellOf,
+                                                  const int *__restrict__ links, int64_t total,
+                                                  const T *__restrict__ pc,
+                                                  const T *__restrict__ q, T *__restrict__ zp,
+                                                  T *__restrict__ zv, int n,
+                                                  const Scal *__restrict__ sc) {
   const int p = blockIdx.y;
   if (!sc[p].active) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * n) return;
-  const int i = t % n, j = t / n, k = s - i - j;
-  if (k < 0 || k >= n) return;
-  const int64_t N = int64_t(n) * n * n, base = int64_t(p) * N;
-  const int64_t c = i + int64_t(n) * (j + int64_t(n) * k);
-  const uint32_t inf = infov[base + c];
-  if (!(inf & 0x8000u)) return;
-  const T *pc = pcv + base;
-  T *z = zv + base;
-  const int64_t fwd[3] = {c + 1, c + n, c + int64_t(n) * n};
+  const int t = start[int64_t(p) * (S + 1) + s] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= start[int64_t(p) * (S + 1) + s + 1]) return;
+  const int64_t g = base[p] + t;
   T sum = T(0);
-  for (int a = 0; a < 3; a++)
-    if (PS_NB(inf, 2 * a + 1)) sum += z[fwd[a]];
-  const T pcc = pc[c];
-  z[c] = (qv[base + c] + pcc * sum) * pcc;
+  for (int a = 0; a < 3; a++) {
+    const int f = links[(3 + a) * total + g];
+    if (f >= 0) sum += zp[base[p] + f];
+  }
+  const T pcc = pc[g];
+  const T z = (q[g] + pcc * sum) * pcc;
+  zp[g] = z;
+  zv[int64_t(p) * n * n * n + cellOf[g]] = z;
 }
 
 // partial z.r over the FLUID cells (z from the preconditioner); only restart-flagged problems
@@ -1088,9 +1139,18 @@ struct pl3d_poisson {
   uint16_t *mgInfo[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
   void *mgX[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
   void *mgB[MG_MAX_LEVELS] = {nullptr, nullptr, nullptr, nullptr};
-  // MIC(0) factor (PL3D_PRECOND_MIC0), FP64 and its FP32 copy, one value per cell
+  // MIC(0) (PL3D_PRECOND_MIC0) in plane order: per problem its base in the concatenated plane
+  // order, FLUID count and plane starts (S + 1 per problem, S = 3n - 2 planes; the widest plane
+  // over the problems sets each launch's grid), the position -> cell map, six neighbour links per
+  // position, the factor (FP64 and an FP32 copy) and the triangular solves' q and z (workElem)
+  int micS = 0;
+  int64_t micTotal = 0;
+  std::vector<int> micWidest;
+  int64_t *micBase = nullptr;
+  int *micCount = nullptr, *micStart = nullptr, *micCell = nullptr, *micLinks = nullptr;
   double *micPcD = nullptr;
   float *micPcF = nullptr;
+  void *micQ = nullptr, *micZ = nullptr;
   bool destroyed = false;
 };
 
@@ -1106,6 +1166,10 @@ void free_work(pl3d_poisson *ps) {
       if (*p) (void)hipFree(*p);
       *p = nullptr;
     }
+  for (void **p : {&ps->micQ, &ps->micZ}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
   if (ps->xhi) (void)hipFree(ps->xhi);
   ps->xhi = nullptr;
   ps->workElem = 0;
@@ -1121,6 +1185,10 @@ int alloc_work(pl3d_poisson *ps, size_t elem, bool hi) {
   if ((ps->precond == PL3D_PRECOND_CHEBYSHEV || ps->precond == PL3D_PRECOND_MG ||
        ps->precond == PL3D_PRECOND_MIC0) && !ps->za)
     for (void **p : {&ps->za, &ps->zb}) PS_HIP(hipMalloc(p, cells * elem));
+  if (ps->precond == PL3D_PRECOND_MIC0 && !ps->micQ) {
+    PS_HIP(hipMalloc(&ps->micQ, size_t(std::max<int64_t>(ps->micTotal, 1)) * elem));
+    PS_HIP(hipMalloc(&ps->micZ, size_t(std::max<int64_t>(ps->micTotal, 1)) * elem));
+  }
   if (ps->precond == PL3D_PRECOND_MG)
     for (int l = 1; l < ps->mgLevels; l++) {
       const size_t nc = size_t(ps->mgN[l]) * ps->mgN[l] * ps->mgN[l] * size_t(ps->P);
@@ -1170,21 +1238,82 @@ int build_mg(pl3d_poisson *ps) {
   return PL3D_OK;
 }
 
-// The MIC(0) factor of every problem, one launch per hyperplane in increasing order; built once
-// per set.
+// The MIC(0) plane order and factor of every problem, built once per set: FLUID cells counted
+// per hyperplane, positions assigned, neighbour links, then the factor plane by plane.
 int build_mic(pl3d_poisson *ps) {
   if (ps->micPcD) return PL3D_OK;
-  const size_t cells = size_t(ps->P) * size_t(ps->N);
-  PS_HIP(hipMalloc((void **)&ps->micPcD, cells * 8));
-  PS_HIP(hipMalloc((void **)&ps->micPcF, cells * 4));
-  const int n = ps->n;
-  const dim3 g(unsigned((n * n + 255) / 256), ps->P);
-  for (int s = 0; s <= 3 * (n - 1); s++)
-    hipLaunchKernelGGL(k_mic_factor, g, dim3(256), 0, ps->stream, ps->info, ps->micPcD, n, s);
-  hipLaunchKernelGGL(k_mic_tofloat, dim3(1024), dim3(256), 0, ps->stream, ps->micPcD,
-                     ps->micPcF, int64_t(cells));
-  PS_HIP(hipGetLastError());
-  PS_HIP(hipStreamSynchronize(ps->stream));
+  const int n = ps->n, P = ps->P, S = 3 * (n - 1) + 1;
+  const int64_t N = ps->N;
+  hipStream_t st = ps->stream;
+  const int T = 256;
+  const dim3 gcells(unsigned((N + T - 1) / T), P);
+  int *counts = nullptr, *fill = nullptr, *pos = nullptr;
+  auto cleanup = [&]() {
+    for (int *q : {counts, fill, pos})
+      if (q) (void)hipFree(q);
+  };
+  auto fail = [&](hipError_t e) {
+    cleanup();
+    return perr(PL3D_EDEVICE, "MIC(0) plane order: %s", hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipMalloc((void **)&counts, size_t(P) * S * 4)) ||
+      (e = hipMemsetAsync(counts, 0, size_t(P) * S * 4, st)))
+    return fail(e);
+  hipLaunchKernelGGL(k_micp_count, gcells, dim3(T), 0, st, ps->info, n, counts, S);
+  std::vector<int> hc(size_t(P) * S), hstart(size_t(P) * (S + 1)), hcount(P);
+  std::vector<int64_t> hbase(P);
+  if ((e = hipMemcpyAsync(hc.data(), counts, hc.size() * 4, hipMemcpyDeviceToHost, st)) ||
+      (e = hipStreamSynchronize(st)))
+    return fail(e);
+  ps->micWidest.assign(S, 0);
+  int64_t total = 0;
+  for (int p = 0; p < P; p++) {
+    int acc = 0;
+    for (int k = 0; k < S; k++) {
+      hstart[size_t(p) * (S + 1) + k] = acc;
+      acc += hc[size_t(p) * S + k];
+      ps->micWidest[k] = std::max(ps->micWidest[k], hc[size_t(p) * S + k]);
+    }
+    hstart[size_t(p) * (S + 1) + S] = acc;
+    hcount[p] = acc;
+    hbase[p] = total;
+    total += acc;
+  }
+  const size_t tot = size_t(std::max<int64_t>(total, 1));
+  if ((e = hipMalloc((void **)&ps->micBase, size_t(P) * 8)) ||
+      (e = hipMalloc((void **)&ps->micCount, size_t(P) * 4)) ||
+      (e = hipMalloc((void **)&ps->micStart, hstart.size() * 4)) ||
+      (e = hipMalloc((void **)&ps->micCell, tot * 4)) ||
+      (e = hipMalloc((void **)&ps->micLinks, tot * 6 * 4)) ||
+      (e = hipMalloc((void **)&ps->micPcD, tot * 8)) ||
+      (e = hipMalloc((void **)&ps->micPcF, tot * 4)) ||
+      (e = hipMalloc((void **)&fill, size_t(P) * S * 4)) ||
+      (e = hipMalloc((void **)&pos, size_t(P) * size_t(N) * 4)) ||
+      (e = hipMemcpyAsync(ps->micBase, hbase.data(), size_t(P) * 8, hipMemcpyHostToDevice, st)) ||
+      (e = hipMemcpyAsync(ps->micCount, hcount.data(), size_t(P) * 4, hipMemcpyHostToDevice,
+                          st)) ||
+      (e = hipMemcpyAsync(ps->micStart, hstart.data(), hstart.size() * 4, hipMemcpyHostToDevice,
+                          st)) ||
+      (e = hipMemsetAsync(fill, 0, size_t(P) * S * 4, st)))
+    return fail(e);
+  ps->micS = S;
+  ps->micTotal = total;
+  hipLaunchKernelGGL(k_micp_place, gcells, dim3(T), 0, st, ps->info, n, ps->micStart, S, fill,
+                     ps->micBase, pos, ps->micCell);
+  int widest = 1;
+  for (int w : hcount) widest = std::max(widest, w);
+  hipLaunchKernelGGL(k_micp_links, dim3(unsigned((widest + T - 1) / T), P), dim3(T), 0, st,
+                     ps->info, n, ps->micBase, ps->micCount, pos, ps->micCell, ps->micLinks,
+                     total);
+  for (int k = 0; k < S; k++)
+    if (ps->micWidest[k] > 0)
+      hipLaunchKernelGGL(k_micp_factor, dim3(unsigned((ps->micWidest[k] + T - 1) / T), P),
+                         dim3(T), 0, st, ps->info, n, ps->micBase, ps->micStart, S, k,
+                         ps->micCell, ps->micLinks, total, ps->micPcD);
+  hipLaunchKernelGGL(k_mic_tofloat, dim3(1024), dim3(T), 0, st, ps->micPcD, ps->micPcF, total);
+  if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return fail(e);
+  cleanup();
   return PL3D_OK;
 }
 
@@ -1269,11 +1398,18 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
   auto micsolve = [&]() -> T * {
     const T *pcT = reinterpret_cast<const T *>(sizeof(T) == 8 ? static_cast<void *>(ps->micPcD)
                                                               : static_cast<void *>(ps->micPcF));
-    const dim3 g(unsigned((n * n + 255) / 256), P);
-    for (int s2 = 0; s2 <= 3 * (n - 1); s2++)
-      hipLaunchKernelGGL(k_mic_fwd<T>, g, dim3(256), 0, st, ps->info, pcT, r, zb, n, s2, ps->sc);
-    for (int s2 = 3 * (n - 1); s2 >= 0; s2--)
-      hipLaunchKernelGGL(k_mic_bwd<T>, g, dim3(256), 0, st, ps->info, pcT, zb, za, n, s2, ps->sc);
+    T *q = static_cast<T *>(ps->micQ), *zp = static_cast<T *>(ps->micZ);
+    const int S = ps->micS;
+    for (int s2 = 0; s2 < S; s2++)
+      if (ps->micWidest[s2] > 0)
+        hipLaunchKernelGGL(k_micp_fwd<T>, dim3(unsigned((ps->micWidest[s2] + 255) / 256), P),
+                           dim3(256), 0, st, ps->micBase, ps->micStart, S, s2, ps->micCell,
+                           ps->micLinks, ps->micTotal, pcT, r, q, n, ps->sc);
+    for (int s2 = S - 1; s2 >= 0; s2--)
+      if (ps->micWidest[s2] > 0)
+        hipLaunchKernelGGL(k_micp_bwd<T>, dim3(unsigned((ps->micWidest[s2] + 255) / 256), P),
+                           dim3(256), 0, st, ps->micBase, ps->micStart, S, s2, ps->micCell,
+                           ps->micLinks, ps->micTotal, pcT, q, zp, za, n, ps->sc);
     hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, za, r, ps->partial, ps->sc, n,
                        bpp, 0);
     return za;
@@ -1458,8 +1594,10 @@ int pl3d_poisson_destroy(pl3d_poisson *ps) {
     if (ps->mgFlags[l]) (void)hipFree(ps->mgFlags[l]);
     if (ps->mgInfo[l]) (void)hipFree(ps->mgInfo[l]);
   }
-  if (ps->micPcD) (void)hipFree(ps->micPcD);
-  if (ps->micPcF) (void)hipFree(ps->micPcF);
+  for (void *q : {(void *)ps->micBase, (void *)ps->micCount, (void *)ps->micStart,
+                  (void *)ps->micCell, (void *)ps->micLinks, (void *)ps->micPcD,
+                  (void *)ps->micPcF})
+    if (q) (void)hipFree(q);
   for (void *p : {(void *)ps->flags, (void *)ps->info, (void *)ps->b, (void *)ps->partial, (void *)ps->partial2,
                   (void *)ps->sc, (void *)ps->sol[0], (void *)ps->sol[1], (void *)ps->sol[2]})
     if (p) (void)hipFree(p);
