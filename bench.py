@@ -355,7 +355,7 @@ def poisson_bench(a):
                                (", k_ps_cheb0, k_ps_cheb, k_ps_zdot, k_ps_dirz)" if deg else
                                 (", k_mg_sweep, k_mg_restrict, k_mg_prolong, k_mg_coarsest, "
                                  "k_ps_zdot, k_ps_dirz)" if pc == "mg" else
-                                 (", k_micp_fwd, k_micp_bwd, k_ps_zdot, k_ps_dirz)"
+                                 (", k_micp_solve, k_ps_zdot, k_ps_dirz)"
                                   if pc == "mic0" else ")"))),
                      "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -912,53 +912,89 @@ __global__ __launch_bounds__(256) void k_mic_tofloat(const double *__restrict__ 
     b[i] = float(a[i]);
 }
 
-// forward solve on hyperplane s: q_t = (r_c + sum over FLUID backward neighbours of pc q) pc_t
+// Both triangular solves of one problem in one workgroup (a problem's hyperplanes depend only on
+// each other, not on other problems): the workgroup walks the planes forward, then backward,
+// with a barrier between planes, so an application is one launch of one workgroup per problem
+// instead of 2 (3n - 2) launches.  Forward: q_t = (r_c + sum over the FLUID backward neighbours
+// of pc q) pc_t; backward: z_t = (q_t + pc_t * sum over the FLUID forward neighbours of z) pc_t,
+// written in plane order and to the grid.
+constexpr int MIC_THREADS = 1024;
+constexpr int MIC_ILP = 4;
 template <class T>
-__global__ __launch_bounds__(256) void k_micp_fwd(const int64_t *__restrict__ base,
-                                                  const int *__restrict__ start, int S, int s,
-                                                  const int *__restrict__ cellOf,
-                                                  const int *__restrict__ links, int64_t total,
-                                                  const T *__restrict__ pc,
-                                                  const T *__restrict__ rv, T *__restrict__ q,
-                                                  int n, const Scal *__restrict__ sc) {
-  const int p = blockIdx.y;
+__global__ __launch_bounds__(MIC_THREADS) void k_micp_solve(
+    const int64_t *__restrict__ base, const int *__restrict__ start, int S,
+    const int *__restrict__ cellOf, const int *__restrict__ links, int64_t total,
+    const T *__restrict__ pc, const T *__restrict__ rv, T *__restrict__ q, T *__restrict__ zp,
+    T *__restrict__ zv, int n, const Scal *__restrict__ sc) {
+  const int p = blockIdx.x;
   if (!sc[p].active) return;
-  const int t = start[int64_t(p) * (S + 1) + s] + blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= start[int64_t(p) * (S + 1) + s + 1]) return;
-  const int64_t g = base[p] + t;
-  T v = rv[int64_t(p) * n * n * n + cellOf[g]];
-  for (int a = 0; a < 3; a++) {
-    const int b = links[a * total + g];
-    if (b >= 0) v += pc[base[p] + b] * q[base[p] + b];
+  const int64_t b0 = base[p], grid0 = int64_t(p) * n * n * n;
+  const int *st = start + int64_t(p) * (S + 1);
+  // each thread takes MIC_ILP positions of a plane per round, all their loads issued before
+  // their arithmetic (the chain cell -> r and link -> neighbour is latency-bound otherwise)
+  for (int s = 0; s < S; s++) {
+    for (int t0 = st[s] + threadIdx.x; t0 < st[s + 1]; t0 += MIC_THREADS * MIC_ILP) {
+      int64_t g[MIC_ILP];
+      int cell[MIC_ILP], lk[MIC_ILP][3];
+      bool ok[MIC_ILP];
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        ok[u] = t < st[s + 1];
+        g[u] = b0 + (ok[u] ? t : st[s]);
+        cell[u] = cellOf[g[u]];
+#pragma unroll
+        for (int a = 0; a < 3; a++) lk[u][a] = links[a * total + g[u]];
+      }
+      T v[MIC_ILP], pcs[MIC_ILP];
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        v[u] = rv[grid0 + cell[u]];
+        pcs[u] = pc[g[u]];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+          if (lk[u][a] >= 0) v[u] += pc[b0 + lk[u][a]] * q[b0 + lk[u][a]];
+      }
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++)
+        if (ok[u]) q[g[u]] = v[u] * pcs[u];
+    }
+    __syncthreads();  // plane s complete (its q is read by plane s + 1)
   }
-  q[g] = v * pc[g];
-}
-
-// backward solve on hyperplane s: z_t = (q_t + pc_t * sum over FLUID forward neighbours of z)
-// pc_t, written in plane order and to the grid
-template <class T>
-__global__ __launch_bounds__(256) void k_micp_bwd(const int64_t *__restrict__ base,
-                                                  const int *__restrict__ start, int S, int s,
-                                                  const int *__restrict__ cellOf,
-                                                  const int *__restrict__ links, int64_t total,
-                                                  const T *__restrict__ pc,
-                                                  const T *__restrict__ q, T *__restrict__ zp,
-                                                  T *__restrict__ zv, int n,
-                                                  const Scal *__restrict__ sc) {
-  const int p = blockIdx.y;
-  if (!sc[p].active) return;
-  const int t = start[int64_t(p) * (S + 1) + s] + blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= start[int64_t(p) * (S + 1) + s + 1]) return;
-  const int64_t g = base[p] + t;
-  T sum = T(0);
-  for (int a = 0; a < 3; a++) {
-    const int f = links[(3 + a) * total + g];
-    if (f >= 0) sum += zp[base[p] + f];
+  for (int s = S - 1; s >= 0; s--) {
+    for (int t0 = st[s] + threadIdx.x; t0 < st[s + 1]; t0 += MIC_THREADS * MIC_ILP) {
+      int64_t g[MIC_ILP];
+      int cell[MIC_ILP], lk[MIC_ILP][3];
+      bool ok[MIC_ILP];
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        ok[u] = t < st[s + 1];
+        g[u] = b0 + (ok[u] ? t : st[s]);
+        cell[u] = cellOf[g[u]];
+#pragma unroll
+        for (int a = 0; a < 3; a++) lk[u][a] = links[(3 + a) * total + g[u]];
+      }
+      T sum[MIC_ILP], qs[MIC_ILP], pcs[MIC_ILP];
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        sum[u] = T(0);
+        qs[u] = q[g[u]];
+        pcs[u] = pc[g[u]];
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+          if (lk[u][a] >= 0) sum[u] += zp[b0 + lk[u][a]];
+      }
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++)
+        if (ok[u]) {
+          const T z = (qs[u] + pcs[u] * sum[u]) * pcs[u];
+          zp[g[u]] = z;
+          zv[grid0 + cell[u]] = z;
+        }
+    }
+    __syncthreads();  // plane s complete (its z is read by plane s - 1)
   }
-  const T pcc = pc[g];
-  const T z = (q[g] + pcc * sum) * pcc;
-  zp[g] = z;
-  zv[int64_t(p) * n * n * n + cellOf[g]] = z;
 }
 
 // partial z.r over the FLUID cells (z from the preconditioner); only restart-flagged problems
@@ -1399,17 +1435,9 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
     const T *pcT = reinterpret_cast<const T *>(sizeof(T) == 8 ? static_cast<void *>(ps->micPcD)
                                                               : static_cast<void *>(ps->micPcF));
     T *q = static_cast<T *>(ps->micQ), *zp = static_cast<T *>(ps->micZ);
-    const int S = ps->micS;
-    for (int s2 = 0; s2 < S; s2++)
-      if (ps->micWidest[s2] > 0)
-        hipLaunchKernelGGL(k_micp_fwd<T>, dim3(unsigned((ps->micWidest[s2] + 255) / 256), P),
-                           dim3(256), 0, st, ps->micBase, ps->micStart, S, s2, ps->micCell,
-                           ps->micLinks, ps->micTotal, pcT, r, q, n, ps->sc);
-    for (int s2 = S - 1; s2 >= 0; s2--)
-      if (ps->micWidest[s2] > 0)
-        hipLaunchKernelGGL(k_micp_bwd<T>, dim3(unsigned((ps->micWidest[s2] + 255) / 256), P),
-                           dim3(256), 0, st, ps->micBase, ps->micStart, S, s2, ps->micCell,
-                           ps->micLinks, ps->micTotal, pcT, q, zp, za, n, ps->sc);
+    hipLaunchKernelGGL(k_micp_solve<T>, dim3(P), dim3(MIC_THREADS), 0, st, ps->micBase,
+                       ps->micStart, ps->micS, ps->micCell, ps->micLinks, ps->micTotal, pcT, r,
+                       q, zp, za, n, ps->sc);
     hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, za, r, ps->partial, ps->sc, n,
                        bpp, 0);
     return za;
