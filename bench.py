@@ -309,11 +309,12 @@ def poisson_bench(a):
         per_cell = PS_VECTOR_BYTES + 8.0 + 66.0 * 8.0 / 7.0
         per_flag = PS_FLAG_BYTES + 2.0 + 18.0 * 8.0 / 7.0
     elif pc == "mic0":
-        # in plane order: the forward solve reads the position's cell and three backward links,
-        # the factor, r through the grid and the backward neighbours' q, and writes q (32 B per
-        # FLUID cell); the backward solve reads the cell, three forward links, the factor, q and
-        # the forward neighbours' z, and writes z in plane order and to the grid (36 B); z.r (8 B)
-        per_cell = PS_VECTOR_BYTES + 8.0 + 68.0
+        # in plane order, with the neighbouring planes in LDS: r gathered into plane order (the
+        # cell, r through the grid, the plane-order write: 12 B per FLUID cell); the forward solve
+        # reads r, the factor and three backward links and writes q (24 B); the backward solve
+        # reads q, the factor, the cell and three forward links and writes z to the grid (28 B);
+        # z.r (8 B)
+        per_cell = PS_VECTOR_BYTES + 8.0 + 64.0
         per_flag = PS_FLAG_BYTES + 2.0
     it_bytes = sum((per_cell * mi) * x["iterations"] for mi, x in zip(m, r)) + \
         per_flag * cells * iters
@@ -355,7 +356,7 @@ def poisson_bench(a):
                                (", k_ps_cheb0, k_ps_cheb, k_ps_zdot, k_ps_dirz)" if deg else
                                 (", k_mg_sweep, k_mg_restrict, k_mg_prolong, k_mg_coarsest, "
                                  "k_ps_zdot, k_ps_dirz)" if pc == "mg" else
-                                 (", k_micp_solve, k_ps_zdot, k_ps_dirz)"
+                                 (", k_micp_solve_lds, k_ps_zdot, k_ps_dirz)"
                                   if pc == "mic0" else ")"))),
                      "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

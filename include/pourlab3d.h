@@ -359,8 +359,10 @@ int pl3d_poisson_problem(pl3d_poisson *ps, int32_t k, uint8_t *flags, float *rhs
 /* the preconditioner of later solves: PL3D_PRECOND_JACOBI (the default),
  * PL3D_PRECOND_CHEBYSHEV with a degree and interval lower bound (0: the library defaults),
  * PL3D_PRECOND_MG (one V-cycle of the pour solver's multigrid; 8 <= n <= 128) or
- * PL3D_PRECOND_MIC0 (the pour solver's MIC(0), hyperplane-scheduled); the MG hierarchy and the
- * MIC(0) factor are built on the first call that asks for them */
+ * PL3D_PRECOND_MIC0 (the pour solver's MIC(0), hyperplane-scheduled; degree 0 keeps the planes
+ * next to the one being walked in LDS when two of the widest fit, 1 walks them in global memory,
+ * with the same results up to rounding); the MG hierarchy and the MIC(0) factor are built on the
+ * first call that asks for them */
 int pl3d_poisson_set_preconditioner(pl3d_poisson *ps, int32_t precond, int32_t degree,
                                     double lower);
 /* Solve every problem to max|r| <= max(tol max|b|, 1e-14) (solver3d.ts:910) at one precision;

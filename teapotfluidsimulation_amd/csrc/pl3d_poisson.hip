@@ -997,6 +997,112 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve(
   }
 }
 
+// The same solves with the planes next to the one being walked held in LDS.  A position's
+// backward neighbours all lie on the previous hyperplane and its forward ones on the next, so two
+// LDS buffers of the widest plane, used alternately, hold what a plane reads: pc q of the previous
+// plane going forward (the product taken by its writer), z of the next plane going backward.  r is
+// first gathered into plane order (rp), each position by the thread that reads it back later, so
+// every global load of a plane step is indexed by the plane's own positions (contiguous) and only
+// one global latency, not a chain of two, stands between two barriers.
+template <class T>
+__global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
+    const int64_t *__restrict__ base, const int *__restrict__ start, int S,
+    const int *__restrict__ cellOf, const int *__restrict__ links, int64_t total,
+    const T *__restrict__ pc, const T *__restrict__ rv, T *__restrict__ q, T *__restrict__ rp,
+    T *__restrict__ zv, int n, int wide, const Scal *__restrict__ sc) {
+  extern __shared__ __align__(16) unsigned char mic_lds[];
+  T *const ring0 = reinterpret_cast<T *>(mic_lds), *const ring1 = ring0 + wide;
+  const int p = blockIdx.x;
+  if (!sc[p].active) return;
+  const int64_t b0 = base[p], grid0 = int64_t(p) * n * n * n;
+  const int *st = start + int64_t(p) * (S + 1);
+  // r in plane order, with the walks' mapping of positions to threads
+  for (int s = 0; s < S; s++) {
+    const int a1 = st[s + 1];
+    for (int t0 = st[s] + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
+      int cell[MIC_ILP];
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        cell[u] = t < a1 ? cellOf[b0 + t] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        if (t < a1) rp[b0 + t] = rv[grid0 + cell[u]];
+      }
+    }
+  }
+  // forward: plane s reads pc q of plane s - 1 from ring (s + 1) & 1, writes its own to ring s & 1
+  for (int s = 0; s < S; s++) {
+    const int a0 = st[s], a1 = st[s + 1], prev = s > 0 ? st[s - 1] : 0;
+    const T *in = (s & 1) ? ring0 : ring1;
+    T *out = (s & 1) ? ring1 : ring0;
+    for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
+      int lk[MIC_ILP][3];
+      T rs[MIC_ILP], pcs[MIC_ILP];
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        const int64_t g = b0 + (t < a1 ? t : a0);
+        rs[u] = rp[g];
+        pcs[u] = pc[g];
+#pragma unroll
+        for (int a = 0; a < 3; a++) lk[u][a] = links[a * total + g];
+      }
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        if (t < a1) {
+          T v = rs[u];
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+            if (lk[u][a] >= 0) v += in[lk[u][a] - prev];
+          const T qt = v * pcs[u];
+          q[b0 + t] = qt;
+          out[t - a0] = pcs[u] * qt;
+        }
+      }
+    }
+    __syncthreads();  // plane s complete in LDS; every read of plane s - 1 done
+  }
+  // backward: plane s reads z of plane s + 1 from ring (s + 1) & 1, writes its own to ring s & 1
+  // (the last plane has no forward neighbours, so its read of the stale ring never happens)
+  for (int s = S - 1; s >= 0; s--) {
+    const int a0 = st[s], a1 = st[s + 1];
+    const T *in = (s & 1) ? ring0 : ring1;
+    T *out = (s & 1) ? ring1 : ring0;
+    for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
+      int lk[MIC_ILP][3], cell[MIC_ILP];
+      T qs[MIC_ILP], pcs[MIC_ILP];
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        const int64_t g = b0 + (t < a1 ? t : a0);
+        qs[u] = q[g];
+        pcs[u] = pc[g];
+        cell[u] = cellOf[g];
+#pragma unroll
+        for (int a = 0; a < 3; a++) lk[u][a] = links[(3 + a) * total + g];
+      }
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        const int t = t0 + u * MIC_THREADS;
+        if (t < a1) {
+          T sum = T(0);
+#pragma unroll
+          for (int a = 0; a < 3; a++)
+            if (lk[u][a] >= 0) sum += in[lk[u][a] - a1];
+          const T z = (qs[u] + pcs[u] * sum) * pcs[u];
+          out[t - a0] = z;
+          zv[grid0 + cell[u]] = z;
+        }
+      }
+    }
+    __syncthreads();  // plane s complete in LDS; every read of plane s + 1 done
+  }
+}
+
 // partial z.r over the FLUID cells (z from the preconditioner); only restart-flagged problems
 // with onlyRestart
 template <class T>
@@ -1178,7 +1284,8 @@ struct pl3d_poisson {
   // MIC(0) (PL3D_PRECOND_MIC0) in plane order: per problem its base in the concatenated plane
   // order, FLUID count and plane starts (S + 1 per problem, S = 3n - 2 planes; the widest plane
   // over the problems sets each launch's grid), the position -> cell map, six neighbour links per
-  // position, the factor (FP64 and an FP32 copy) and the triangular solves' q and z (workElem)
+  // position, the factor (FP64 and an FP32 copy) and the triangular solves' q and, in plane
+  // order, z (global-memory walk) or r (LDS walk) (workElem)
   int micS = 0;
   int64_t micTotal = 0;
   std::vector<int> micWidest;
@@ -1187,6 +1294,11 @@ struct pl3d_poisson {
   double *micPcD = nullptr;
   float *micPcF = nullptr;
   void *micQ = nullptr, *micZ = nullptr;
+  // the widest plane over all problems; whether two planes of FP32 / FP64 values fit the LDS of a
+  // workgroup (k_micp_solve_lds), and whether the caller asked for the global-memory walk anyway
+  int micWide = 0;
+  bool micLdsFits[2] = {false, false};
+  bool micGlobal = false;
   bool destroyed = false;
 };
 
@@ -1350,6 +1462,22 @@ int build_mic(pl3d_poisson *ps) {
   hipLaunchKernelGGL(k_mic_tofloat, dim3(1024), dim3(T), 0, st, ps->micPcD, ps->micPcF, total);
   if ((e = hipGetLastError()) || (e = hipStreamSynchronize(st))) return fail(e);
   cleanup();
+  // two planes in LDS: within the device's per-workgroup limit, raised for the kernel when needed
+  ps->micWide = 1;
+  for (int w : ps->micWidest) ps->micWide = std::max(ps->micWide, w);
+  int dev = 0, ldsMax = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ldsMax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    ldsMax = 0;
+  const void *kern[2] = {reinterpret_cast<const void *>(&k_micp_solve_lds<float>),
+                         reinterpret_cast<const void *>(&k_micp_solve_lds<double>)};
+  for (int i = 0; i < 2; i++) {
+    const size_t bytes = size_t(2) * size_t(ps->micWide) * (i ? 8 : 4);
+    ps->micLdsFits[i] = bytes <= size_t(ldsMax) &&
+                        hipFuncSetAttribute(kern[i], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            int(bytes)) == hipSuccess;
+  }
+  (void)hipGetLastError();  // a refused attribute only selects the global-memory walk
   return PL3D_OK;
 }
 
@@ -1435,9 +1563,15 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
     const T *pcT = reinterpret_cast<const T *>(sizeof(T) == 8 ? static_cast<void *>(ps->micPcD)
                                                               : static_cast<void *>(ps->micPcF));
     T *q = static_cast<T *>(ps->micQ), *zp = static_cast<T *>(ps->micZ);
-    hipLaunchKernelGGL(k_micp_solve<T>, dim3(P), dim3(MIC_THREADS), 0, st, ps->micBase,
-                       ps->micStart, ps->micS, ps->micCell, ps->micLinks, ps->micTotal, pcT, r,
-                       q, zp, za, n, ps->sc);
+    if (!ps->micGlobal && ps->micLdsFits[sizeof(T) == 8])
+      hipLaunchKernelGGL(k_micp_solve_lds<T>, dim3(P), dim3(MIC_THREADS),
+                         size_t(2) * size_t(ps->micWide) * sizeof(T), st, ps->micBase,
+                         ps->micStart, ps->micS, ps->micCell, ps->micLinks, ps->micTotal, pcT, r,
+                         q, zp, za, n, ps->micWide, ps->sc);
+    else
+      hipLaunchKernelGGL(k_micp_solve<T>, dim3(P), dim3(MIC_THREADS), 0, st, ps->micBase,
+                         ps->micStart, ps->micS, ps->micCell, ps->micLinks, ps->micTotal, pcT, r,
+                         q, zp, za, n, ps->sc);
     hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, za, r, ps->partial, ps->sc, n,
                        bpp, 0);
     return za;
@@ -1657,12 +1791,15 @@ int pl3d_poisson_set_preconditioner(pl3d_poisson *ps, int32_t precond, int32_t d
     return perr(PL3D_EINVAL, "unknown preconditioner %d for the Poisson set", int(precond));
   if (degree < 0 || degree > 64 || !(lower >= 0.0 && lower < 2.0))
     return perr(PL3D_EINVAL, "Chebyshev degree must be in 0..64 and its lower bound in [0, 2)");
+  if (precond == PL3D_PRECOND_MIC0 && degree > 1)
+    return perr(PL3D_EINVAL, "MIC(0) mode must be 0 (planes in LDS when they fit) or 1 (global)");
   if (precond == PL3D_PRECOND_MG)
     if (int rc = build_mg(ps)) return rc;
   if (precond == PL3D_PRECOND_MIC0)
     if (int rc = build_mic(ps)) return rc;
   if (precond != ps->precond) free_work(ps);  // the next solve allocates what it needs
   ps->precond = precond;
+  ps->micGlobal = precond == PL3D_PRECOND_MIC0 && degree == 1;
   ps->chebDegree = degree > 0 ? degree : PL3D_CHEB_DEGREE_DEFAULT;
   ps->chebLower = lower > 0.0 ? lower : PL3D_CHEB_LOWER_DEFAULT;
   return PL3D_OK;

@@ -525,8 +525,9 @@ class PoissonSet:
 
     def setPreconditioner(self, name: str = "jacobi", degree: int = 0, lower: float = 0.0):
         """"jacobi" (default), "chebyshev" (degree, interval [lower, 2]; 0: library defaults),
-        "mg" (one multigrid V-cycle) or "mic0" (the MIC(0) triangular solves) for the following
-        solves."""
+        "mg" (one multigrid V-cycle) or "mic0" (the MIC(0) triangular solves; degree 0 keeps
+        the neighbouring hyperplanes in LDS when they fit, 1 walks them in global memory) for the
+        following solves."""
         check(load().pl3d_poisson_set_preconditioner(self._handle(), abi.PRECONDITIONERS[name],
                                                     int(degree), float(lower)))
 

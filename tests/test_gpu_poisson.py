@@ -111,31 +111,37 @@ def test_mg_preconditioned_set():
     ps.destroy()
 
 
-def test_mic0_preconditioned_set():
-    """The batched MIC(0)-PCG (level-scheduled over hyperplanes, every problem per launch)
-    against the oracle's restatement of the reference's MIC(0)-PCG (Pcg3D, solver3d.ts:827-973)
-    on the same problems: FP64 iteration counts within 1 (the device sums its dot products in
-    another order) and solutions to 1e-6 of max|x|; the refined solve meets its FP64 true
-    residual in fewer iterations than Jacobi-PCG."""
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mic0_preconditioned_set(mode):
+    """The batched MIC(0)-PCG (level-scheduled over hyperplanes, one workgroup per problem; mode
+    0 keeps the neighbouring planes in LDS, mode 1 walks them in global memory) against the
+    oracle's restatement of the reference's MIC(0)-PCG (Pcg3D, solver3d.ts:827-973) on the same
+    problems: FP64 iteration counts within 1 (the device sums its dot products in another order)
+    and solutions to 1e-6 of max|x|; the refined solve meets its FP64 true residual in fewer
+    iterations than Jacobi-PCG."""
     from oracle import oracle3d as O
     n, P, seed0 = 32, 3, 4000
     ps = PoissonSet(n, P, seed0)
     rj = ps.solve("refined", 1e-6, 3000)
-    ps.setPreconditioner("mic0")
+    ps.setPreconditioner("mic0", mode)
     r64 = ps.solve("fp64", 1e-8, 1000)
     rref = ps.solve("refined", 1e-6, 1000)
+    r32 = ps.solve("fp32", 1e-6, 1000)
     for k in range(P):
         fl, b, m = ps.problem(k)
         bd = b.astype(np.float64)
         qo, ro = O.pcg_solve(n, n, n, fl, bd, 1e-8, 1000)
         x = ps.solution("fp64", k)
         print(k, "iterations jacobi", rj[k]["iterations"], "mic0 refined", rref[k]["iterations"],
-              "mic0 fp64", r64[k]["iterations"], "oracle", ro.iterations, flush=True)
-        assert ro.converged and r64[k]["converged"]
+              "fp32", r32[k]["iterations"], "mic0 fp64", r64[k]["iterations"], "oracle",
+              ro.iterations, flush=True)
+        assert ro.converged and r64[k]["converged"] and r32[k]["converged"]
         assert abs(r64[k]["iterations"] - ro.iterations) <= 1
         assert np.abs(x - qo).max() <= 1e-6 * np.abs(qo).max()
         assert rref[k]["converged"] and rref[k]["trueResidual"] <= 1e-6 * np.abs(bd).max()
         assert rref[k]["iterations"] < rj[k]["iterations"]
+    with pytest.raises(RuntimeError):
+        ps.setPreconditioner("mic0", 2)  # modes are 0 and 1
     ps.destroy()
 
 
