@@ -1033,14 +1033,14 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
       }
     }
   }
-  // forward: plane s reads pc q of plane s - 1 from ring (s + 1) & 1, writes its own to ring s & 1
-  for (int s = 0; s < S; s++) {
-    const int a0 = st[s], a1 = st[s + 1], prev = s > 0 ? st[s - 1] : 0;
-    const T *in = (s & 1) ? ring0 : ring1;
-    T *out = (s & 1) ? ring1 : ring0;
-    for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
-      int lk[MIC_ILP][3];
-      T rs[MIC_ILP], pcs[MIC_ILP];
+  // forward: plane s reads pc q of plane s - 1 from ring (s + 1) & 1, writes its own to ring s & 1.
+  // The first round of plane s + 1 is loaded before plane s is computed (those loads depend on no
+  // result of plane s), so their latency overlaps the plane's work and barrier.  A load is issued
+  // only for a non-empty plane; lanes past its end read its first position and store nothing.
+  {
+    T crs[MIC_ILP], cpc[MIC_ILP], nrs[MIC_ILP], npc[MIC_ILP];
+    int clk[MIC_ILP][3], nlk[MIC_ILP][3];
+    auto fload = [&](int a0, int a1, int t0, T *rs, T *pcs, int (*lk)[3]) {
 #pragma unroll
       for (int u = 0; u < MIC_ILP; u++) {
         const int t = t0 + u * MIC_THREADS;
@@ -1050,31 +1050,51 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
 #pragma unroll
         for (int a = 0; a < 3; a++) lk[u][a] = links[a * total + g];
       }
+    };
 #pragma unroll
-      for (int u = 0; u < MIC_ILP; u++) {
-        const int t = t0 + u * MIC_THREADS;
-        if (t < a1) {
-          T v = rs[u];
+    for (int u = 0; u < MIC_ILP; u++) {
+      crs[u] = cpc[u] = nrs[u] = npc[u] = T(0);
+      clk[u][0] = clk[u][1] = clk[u][2] = nlk[u][0] = nlk[u][1] = nlk[u][2] = -1;
+    }
+    if (st[0] < st[1]) fload(st[0], st[1], st[0] + threadIdx.x, crs, cpc, clk);
+    for (int s = 0; s < S; s++) {
+      const int a0 = st[s], a1 = st[s + 1], prev = s > 0 ? st[s - 1] : 0;
+      if (s + 1 < S && a1 < st[s + 2]) fload(a1, st[s + 2], a1 + threadIdx.x, nrs, npc, nlk);
+      const T *in = (s & 1) ? ring0 : ring1;
+      T *out = (s & 1) ? ring1 : ring0;
+      for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
+        if (t0 >= a0 + MIC_THREADS * MIC_ILP) fload(a0, a1, t0, crs, cpc, clk);  // later rounds
 #pragma unroll
-          for (int a = 0; a < 3; a++)
-            if (lk[u][a] >= 0) v += in[lk[u][a] - prev];
-          const T qt = v * pcs[u];
-          q[b0 + t] = qt;
-          out[t - a0] = pcs[u] * qt;
+        for (int u = 0; u < MIC_ILP; u++) {
+          const int t = t0 + u * MIC_THREADS;
+          if (t < a1) {
+            T v = crs[u];
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+              if (clk[u][a] >= 0) v += in[clk[u][a] - prev];
+            const T qt = v * cpc[u];
+            q[b0 + t] = qt;
+            out[t - a0] = cpc[u] * qt;
+          }
         }
       }
+      __syncthreads();  // plane s complete in LDS; every read of plane s - 1 done
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        crs[u] = nrs[u];
+        cpc[u] = npc[u];
+#pragma unroll
+        for (int a = 0; a < 3; a++) clk[u][a] = nlk[u][a];
+      }
     }
-    __syncthreads();  // plane s complete in LDS; every read of plane s - 1 done
   }
   // backward: plane s reads z of plane s + 1 from ring (s + 1) & 1, writes its own to ring s & 1
-  // (the last plane has no forward neighbours, so its read of the stale ring never happens)
-  for (int s = S - 1; s >= 0; s--) {
-    const int a0 = st[s], a1 = st[s + 1];
-    const T *in = (s & 1) ? ring0 : ring1;
-    T *out = (s & 1) ? ring1 : ring0;
-    for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
-      int lk[MIC_ILP][3], cell[MIC_ILP];
-      T qs[MIC_ILP], pcs[MIC_ILP];
+  // (the last plane has no forward neighbours, so its read of the stale ring never happens); the
+  // first round of plane s - 1 is loaded before plane s is computed, as going forward
+  {
+    T cq[MIC_ILP], cpc[MIC_ILP], nq[MIC_ILP], npc[MIC_ILP];
+    int ccell[MIC_ILP], clk[MIC_ILP][3], ncell[MIC_ILP], nlk[MIC_ILP][3];
+    auto bload = [&](int a0, int a1, int t0, T *qs, T *pcs, int *cell, int (*lk)[3]) {
 #pragma unroll
       for (int u = 0; u < MIC_ILP; u++) {
         const int t = t0 + u * MIC_THREADS;
@@ -1085,21 +1105,45 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
 #pragma unroll
         for (int a = 0; a < 3; a++) lk[u][a] = links[(3 + a) * total + g];
       }
+    };
 #pragma unroll
-      for (int u = 0; u < MIC_ILP; u++) {
-        const int t = t0 + u * MIC_THREADS;
-        if (t < a1) {
-          T sum = T(0);
+    for (int u = 0; u < MIC_ILP; u++) {
+      cq[u] = cpc[u] = nq[u] = npc[u] = T(0);
+      ccell[u] = ncell[u] = 0;
+      clk[u][0] = clk[u][1] = clk[u][2] = nlk[u][0] = nlk[u][1] = nlk[u][2] = -1;
+    }
+    if (st[S - 1] < st[S]) bload(st[S - 1], st[S], st[S - 1] + threadIdx.x, cq, cpc, ccell, clk);
+    for (int s = S - 1; s >= 0; s--) {
+      const int a0 = st[s], a1 = st[s + 1];
+      if (s > 0 && st[s - 1] < a0) bload(st[s - 1], a0, st[s - 1] + threadIdx.x, nq, npc, ncell, nlk);
+      const T *in = (s & 1) ? ring0 : ring1;
+      T *out = (s & 1) ? ring1 : ring0;
+      for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
+        if (t0 >= a0 + MIC_THREADS * MIC_ILP) bload(a0, a1, t0, cq, cpc, ccell, clk);
 #pragma unroll
-          for (int a = 0; a < 3; a++)
-            if (lk[u][a] >= 0) sum += in[lk[u][a] - a1];
-          const T z = (qs[u] + pcs[u] * sum) * pcs[u];
-          out[t - a0] = z;
-          zv[grid0 + cell[u]] = z;
+        for (int u = 0; u < MIC_ILP; u++) {
+          const int t = t0 + u * MIC_THREADS;
+          if (t < a1) {
+            T sum = T(0);
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+              if (clk[u][a] >= 0) sum += in[clk[u][a] - a1];
+            const T z = (cq[u] + cpc[u] * sum) * cpc[u];
+            out[t - a0] = z;
+            zv[grid0 + ccell[u]] = z;
+          }
         }
       }
+      __syncthreads();  // plane s complete in LDS; every read of plane s + 1 done
+#pragma unroll
+      for (int u = 0; u < MIC_ILP; u++) {
+        cq[u] = nq[u];
+        cpc[u] = npc[u];
+        ccell[u] = ncell[u];
+#pragma unroll
+        for (int a = 0; a < 3; a++) clk[u][a] = nlk[u][a];
+      }
     }
-    __syncthreads();  // plane s complete in LDS; every read of plane s + 1 done
   }
 }
 
