@@ -1009,8 +1009,10 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
     const int64_t *__restrict__ base, const int *__restrict__ start, int S,
     const int *__restrict__ cellOf, const int *__restrict__ links, int64_t total,
     const T *__restrict__ pc, const T *__restrict__ rv, T *__restrict__ q, T *__restrict__ rp,
-    T *__restrict__ zv, int n, int wide, const Scal *__restrict__ sc) {
+    T *__restrict__ zv, int n, int wide, const Scal *__restrict__ sc,
+    double *__restrict__ partial, int bpp) {
   extern __shared__ __align__(16) unsigned char mic_lds[];
+  __shared__ double wsum[MIC_THREADS / 64];
   T *const ring0 = reinterpret_cast<T *>(mic_lds), *const ring1 = ring0 + wide;
   const int p = blockIdx.x;
   if (!sc[p].active) return;
@@ -1090,17 +1092,20 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
   }
   // backward: plane s reads z of plane s + 1 from ring (s + 1) & 1, writes its own to ring s & 1
   // (the last plane has no forward neighbours, so its read of the stale ring never happens); the
-  // first round of plane s - 1 is loaded before plane s is computed, as going forward
+  // first round of plane s - 1 is loaded before plane s is computed, as going forward.  z.r is
+  // summed on the way (FP64), so no separate pass over the grid forms it.
+  double acc = 0.0;
   {
-    T cq[MIC_ILP], cpc[MIC_ILP], nq[MIC_ILP], npc[MIC_ILP];
+    T cq[MIC_ILP], cpc[MIC_ILP], cr[MIC_ILP], nq[MIC_ILP], npc[MIC_ILP], nr[MIC_ILP];
     int ccell[MIC_ILP], clk[MIC_ILP][3], ncell[MIC_ILP], nlk[MIC_ILP][3];
-    auto bload = [&](int a0, int a1, int t0, T *qs, T *pcs, int *cell, int (*lk)[3]) {
+    auto bload = [&](int a0, int a1, int t0, T *qs, T *pcs, T *rs, int *cell, int (*lk)[3]) {
 #pragma unroll
       for (int u = 0; u < MIC_ILP; u++) {
         const int t = t0 + u * MIC_THREADS;
         const int64_t g = b0 + (t < a1 ? t : a0);
         qs[u] = q[g];
         pcs[u] = pc[g];
+        rs[u] = rp[g];
         cell[u] = cellOf[g];
 #pragma unroll
         for (int a = 0; a < 3; a++) lk[u][a] = links[(3 + a) * total + g];
@@ -1108,18 +1113,20 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
     };
 #pragma unroll
     for (int u = 0; u < MIC_ILP; u++) {
-      cq[u] = cpc[u] = nq[u] = npc[u] = T(0);
+      cq[u] = cpc[u] = cr[u] = nq[u] = npc[u] = nr[u] = T(0);
       ccell[u] = ncell[u] = 0;
       clk[u][0] = clk[u][1] = clk[u][2] = nlk[u][0] = nlk[u][1] = nlk[u][2] = -1;
     }
-    if (st[S - 1] < st[S]) bload(st[S - 1], st[S], st[S - 1] + threadIdx.x, cq, cpc, ccell, clk);
+    if (st[S - 1] < st[S])
+      bload(st[S - 1], st[S], st[S - 1] + threadIdx.x, cq, cpc, cr, ccell, clk);
     for (int s = S - 1; s >= 0; s--) {
       const int a0 = st[s], a1 = st[s + 1];
-      if (s > 0 && st[s - 1] < a0) bload(st[s - 1], a0, st[s - 1] + threadIdx.x, nq, npc, ncell, nlk);
+      if (s > 0 && st[s - 1] < a0)
+        bload(st[s - 1], a0, st[s - 1] + threadIdx.x, nq, npc, nr, ncell, nlk);
       const T *in = (s & 1) ? ring0 : ring1;
       T *out = (s & 1) ? ring1 : ring0;
       for (int t0 = a0 + threadIdx.x; t0 < a1; t0 += MIC_THREADS * MIC_ILP) {
-        if (t0 >= a0 + MIC_THREADS * MIC_ILP) bload(a0, a1, t0, cq, cpc, ccell, clk);
+        if (t0 >= a0 + MIC_THREADS * MIC_ILP) bload(a0, a1, t0, cq, cpc, cr, ccell, clk);
 #pragma unroll
         for (int u = 0; u < MIC_ILP; u++) {
           const int t = t0 + u * MIC_THREADS;
@@ -1131,6 +1138,7 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
             const T z = (cq[u] + cpc[u] * sum) * cpc[u];
             out[t - a0] = z;
             zv[grid0 + ccell[u]] = z;
+            acc += double(z) * double(cr[u]);
           }
         }
       }
@@ -1139,12 +1147,25 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
       for (int u = 0; u < MIC_ILP; u++) {
         cq[u] = nq[u];
         cpc[u] = npc[u];
+        cr[u] = nr[u];
         ccell[u] = ncell[u];
 #pragma unroll
         for (int a = 0; a < 3; a++) clk[u][a] = nlk[u][a];
       }
     }
   }
+  // the problem's z.r in its first partial slot (fixed order: lanes, then waves), zeros in the
+  // others, as the per-block partials of the grid-order pass would be summed
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < MIC_THREADS / 64; w++) t += wsum[w];
+    partial[int64_t(p) * bpp] = t;
+  }
+  for (int i = 1 + threadIdx.x; i < bpp; i += MIC_THREADS) partial[int64_t(p) * bpp + i] = 0.0;
 }
 
 // partial z.r over the FLUID cells (z from the preconditioner); only restart-flagged problems
@@ -1517,7 +1538,7 @@ int build_mic(pl3d_poisson *ps) {
                          reinterpret_cast<const void *>(&k_micp_solve_lds<double>)};
   for (int i = 0; i < 2; i++) {
     const size_t bytes = size_t(2) * size_t(ps->micWide) * (i ? 8 : 4);
-    ps->micLdsFits[i] = bytes <= size_t(ldsMax) &&
+    ps->micLdsFits[i] = bytes + 8 * (MIC_THREADS / 64) <= size_t(ldsMax) &&  // + the z.r sums
                         hipFuncSetAttribute(kern[i], hipFuncAttributeMaxDynamicSharedMemorySize,
                                             int(bytes)) == hipSuccess;
   }
@@ -1607,15 +1628,16 @@ int run_solve(pl3d_poisson *ps, bool refined, double tol, int maxIter, float *km
     const T *pcT = reinterpret_cast<const T *>(sizeof(T) == 8 ? static_cast<void *>(ps->micPcD)
                                                               : static_cast<void *>(ps->micPcF));
     T *q = static_cast<T *>(ps->micQ), *zp = static_cast<T *>(ps->micZ);
-    if (!ps->micGlobal && ps->micLdsFits[sizeof(T) == 8])
+    if (!ps->micGlobal && ps->micLdsFits[sizeof(T) == 8]) {  // z.r partials included
       hipLaunchKernelGGL(k_micp_solve_lds<T>, dim3(P), dim3(MIC_THREADS),
                          size_t(2) * size_t(ps->micWide) * sizeof(T), st, ps->micBase,
                          ps->micStart, ps->micS, ps->micCell, ps->micLinks, ps->micTotal, pcT, r,
-                         q, zp, za, n, ps->micWide, ps->sc);
-    else
-      hipLaunchKernelGGL(k_micp_solve<T>, dim3(P), dim3(MIC_THREADS), 0, st, ps->micBase,
-                         ps->micStart, ps->micS, ps->micCell, ps->micLinks, ps->micTotal, pcT, r,
-                         q, zp, za, n, ps->sc);
+                         q, zp, za, n, ps->micWide, ps->sc, ps->partial, bpp);
+      return za;
+    }
+    hipLaunchKernelGGL(k_micp_solve<T>, dim3(P), dim3(MIC_THREADS), 0, st, ps->micBase,
+                       ps->micStart, ps->micS, ps->micCell, ps->micLinks, ps->micTotal, pcT, r,
+                       q, zp, za, n, ps->sc);
     hipLaunchKernelGGL(k_ps_zdot<T>, grid, blk, 0, st, ps->info, za, r, ps->partial, ps->sc, n,
                        bpp, 0);
     return za;
