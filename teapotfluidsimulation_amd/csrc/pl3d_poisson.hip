@@ -835,21 +835,37 @@ __global__ __launch_bounds__(256) void k_micp_count(const uint16_t *__restrict__
   atomicAdd(&counts[int64_t(p) * S + s], 1);
 }
 
-// plane-order position of every FLUID cell: its plane's start plus a slot taken atomically
-__global__ __launch_bounds__(256) void k_micp_place(const uint16_t *__restrict__ infov, int n,
-                                                    const int *__restrict__ start, int S,
-                                                    int *__restrict__ fill,
-                                                    const int64_t *__restrict__ base,
-                                                    int *__restrict__ pos,
-                                                    int *__restrict__ cellOf) {
-  const int64_t N = int64_t(n) * n * n, c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  const int p = blockIdx.y;
-  if (!(infov[int64_t(p) * N + c] & 0x8000u)) return;
-  const int s = int(c % n + (c / n) % n + c / (int64_t(n) * n));
-  const int t = start[int64_t(p) * (S + 1) + s] + atomicAdd(&fill[int64_t(p) * S + s], 1);
-  pos[int64_t(p) * N + c] = t;
-  cellOf[base[p] + t] = int(c);
+// plane-order position of every FLUID cell, one thread per hyperplane s of a problem: the thread
+// walks the grid's lines (j, k) in order, each of which meets plane s at most once (i = s - j - k),
+// and numbers the FLUID cells it meets.  A plane's positions are thus ordered by (k, j), the same
+// in every run (the z.r sum of the triangular-solve walk follows this order), and the cells at
+// neighbouring positions of a plane sit on neighbouring lines, as do the cells at the same rank of
+// the next plane, one step further along i: a walk over consecutive planes revisits the grid's
+// cache lines.  Consecutive threads read consecutive cells of a line.
+__global__ __launch_bounds__(1024) void k_micp_place(const uint16_t *__restrict__ infov, int n,
+                                                     const int *__restrict__ start, int S,
+                                                     const int64_t *__restrict__ base,
+                                                     int *__restrict__ pos,
+                                                     int *__restrict__ cellOf) {
+  const int p = blockIdx.x;
+  const int64_t N = int64_t(n) * n * n;
+  const uint16_t *info = infov + int64_t(p) * N;
+  int *posp = pos + int64_t(p) * N;
+  int *cells = cellOf + base[p];
+  for (int s = threadIdx.x; s < S; s += blockDim.x) {  // S = 3n - 2 may exceed the block
+    int t = start[int64_t(p) * (S + 1) + s];
+    for (int k = 0; k < n; k++) {
+      const int j0 = max(0, s - k - (n - 1)), j1 = min(n - 1, s - k);  // 0 <= i < n
+      for (int j = j0; j <= j1; j++) {
+        const int64_t c = int64_t(s - j - k) + int64_t(n) * (j + int64_t(n) * k);
+        if (info[c] & 0x8000u) {
+          posp[c] = t;
+          cells[t] = int(c);
+          t++;
+        }
+      }
+    }
+  }
 }
 
 // plane-order positions of each position's -x, -y, -z (links 0-2) and +x, +y, +z (3-5) FLUID
@@ -1460,9 +1476,9 @@ int build_mic(pl3d_poisson *ps) {
   hipStream_t st = ps->stream;
   const int T = 256;
   const dim3 gcells(unsigned((N + T - 1) / T), P);
-  int *counts = nullptr, *fill = nullptr, *pos = nullptr;
+  int *counts = nullptr, *pos = nullptr;
   auto cleanup = [&]() {
-    for (int *q : {counts, fill, pos})
+    for (int *q : {counts, pos})
       if (q) (void)hipFree(q);
   };
   auto fail = [&](hipError_t e) {
@@ -1501,18 +1517,16 @@ int build_mic(pl3d_poisson *ps) {
       (e = hipMalloc((void **)&ps->micLinks, tot * 6 * 4)) ||
       (e = hipMalloc((void **)&ps->micPcD, tot * 8)) ||
       (e = hipMalloc((void **)&ps->micPcF, tot * 4)) ||
-      (e = hipMalloc((void **)&fill, size_t(P) * S * 4)) ||
       (e = hipMalloc((void **)&pos, size_t(P) * size_t(N) * 4)) ||
       (e = hipMemcpyAsync(ps->micBase, hbase.data(), size_t(P) * 8, hipMemcpyHostToDevice, st)) ||
       (e = hipMemcpyAsync(ps->micCount, hcount.data(), size_t(P) * 4, hipMemcpyHostToDevice,
                           st)) ||
       (e = hipMemcpyAsync(ps->micStart, hstart.data(), hstart.size() * 4, hipMemcpyHostToDevice,
-                          st)) ||
-      (e = hipMemsetAsync(fill, 0, size_t(P) * S * 4, st)))
+                          st)))
     return fail(e);
   ps->micS = S;
   ps->micTotal = total;
-  hipLaunchKernelGGL(k_micp_place, gcells, dim3(T), 0, st, ps->info, n, ps->micStart, S, fill,
+  hipLaunchKernelGGL(k_micp_place, dim3(P), dim3(1024), 0, st, ps->info, n, ps->micStart, S,
                      ps->micBase, pos, ps->micCell);
   int widest = 1;
   for (int w : hcount) widest = std::max(widest, w);
