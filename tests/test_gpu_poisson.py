@@ -145,6 +145,22 @@ def test_mic0_preconditioned_set(mode):
     ps.destroy()
 
 
+def test_mic0_set_is_deterministic():
+    """Two sets built from the same seeds renumber their cells identically and give bit-identical
+    MIC(0)-PCG results (the plane order, and with it every sum, is fixed by the grid)."""
+    n, P, seed0 = 32, 3, 4100
+    out = []
+    for _ in range(2):
+        ps = PoissonSet(n, P, seed0)
+        ps.setPreconditioner("mic0")
+        r = ps.solve("refined", 1e-6, 1000)
+        out.append(([x["iterations"] for x in r], [ps.solution("refined", k) for k in range(P)]))
+        ps.destroy()
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("precond", ["jacobi", "chebyshev", "mg", "mic0"])
 def test_empty_rhs_and_bad_arguments(precond):
     ps = PoissonSet(8, 2, 5)  # an 8^3 grid: the wall leaves at most the four central columns
