@@ -1053,8 +1053,8 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
   }
   // forward: plane s reads pc q of plane s - 1 from ring (s + 1) & 1, writes its own to ring s & 1.
   // The first round of plane s + 1 is loaded before plane s is computed (those loads depend on no
-  // result of plane s), so their latency overlaps the plane's work and barrier.  A load is issued
-  // only for a non-empty plane; lanes past its end read its first position and store nothing.
+  // result of plane s), so their latency overlaps the plane's work and barrier.  Lanes past a
+  // plane's end load nothing (rp and q are read only at positions the same thread wrote).
   {
     T crs[MIC_ILP], cpc[MIC_ILP], nrs[MIC_ILP], npc[MIC_ILP];
     int clk[MIC_ILP][3], nlk[MIC_ILP][3];
@@ -1062,11 +1062,13 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
 #pragma unroll
       for (int u = 0; u < MIC_ILP; u++) {
         const int t = t0 + u * MIC_THREADS;
-        const int64_t g = b0 + (t < a1 ? t : a0);
-        rs[u] = rp[g];
-        pcs[u] = pc[g];
+        if (t < a1) {
+          const int64_t g = b0 + t;
+          rs[u] = rp[g];
+          pcs[u] = pc[g];
 #pragma unroll
-        for (int a = 0; a < 3; a++) lk[u][a] = links[a * total + g];
+          for (int a = 0; a < 3; a++) lk[u][a] = links[a * total + g];
+        }
       }
     };
 #pragma unroll
@@ -1118,13 +1120,15 @@ __global__ __launch_bounds__(MIC_THREADS) void k_micp_solve_lds(
 #pragma unroll
       for (int u = 0; u < MIC_ILP; u++) {
         const int t = t0 + u * MIC_THREADS;
-        const int64_t g = b0 + (t < a1 ? t : a0);
-        qs[u] = q[g];
-        pcs[u] = pc[g];
-        rs[u] = rp[g];
-        cell[u] = cellOf[g];
+        if (t < a1) {
+          const int64_t g = b0 + t;
+          qs[u] = q[g];
+          pcs[u] = pc[g];
+          rs[u] = rp[g];
+          cell[u] = cellOf[g];
 #pragma unroll
-        for (int a = 0; a < 3; a++) lk[u][a] = links[(3 + a) * total + g];
+          for (int a = 0; a < 3; a++) lk[u][a] = links[(3 + a) * total + g];
+        }
       }
     };
 #pragma unroll
